@@ -1,0 +1,364 @@
+/* sankhya_gpu -- PDHG/PDLP engine for bounded LP and convex QP on AMD Instinct
+ * MI355X (gfx950).
+ *
+ * Plain C ABI: every entry point is extern "C", takes plain pointers and
+ * sizes, reports failure through a return code plus a thread-local message,
+ * and never lets an exception or a HIP type cross the boundary.
+ *
+ * Each declaration cites the reference interface it replaces
+ * (Jayaragul/The-solver, paths relative to that repository).
+ *
+ * Threading: the library owns one HIP stream per device at the default
+ * priority; calls from several host threads on one device are serialised by
+ * an internal mutex.  The caller selects the device (hipSetDevice) as with
+ * the reference's cudaSetDevice (apps/sankhya_pdhg.cu:79).
+ */
+#ifndef SANKHYA_GPU_H
+#define SANKHYA_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * Model / solution / options types.
+ *
+ * Same field order, types and enum values as c/include/sankhya.h:17-161, so a
+ * caller holding an sk_model built by the reference C core (sk_read_mps,
+ * sk_model_alloc) passes it here unchanged.  When sankhya.h was included
+ * first its definitions are used instead of these.
+ * ---------------------------------------------------------------------- */
+#ifndef SANKHYA_H
+
+typedef enum sk_status {            /* c/include/sankhya.h:17-26 */
+    SK_OK = 0,
+    SK_ERR_ARG = 1,
+    SK_ERR_MEMORY = 2,
+    SK_ERR_STRUCTURE = 3,
+    SK_ERR_IO = 4,
+    SK_ERR_SINGULAR = 5,
+    SK_ERR_UNSUPPORTED = 6,
+    SK_ERR_NUMERIC = 7
+} sk_status;
+
+typedef enum sk_result {            /* c/include/sankhya.h:29-38 */
+    SK_RESULT_OPTIMAL = 0,
+    SK_RESULT_INFEASIBLE = 1,
+    SK_RESULT_UNBOUNDED = 2,
+    SK_RESULT_ITERATION_LIMIT = 3,
+    SK_RESULT_TIME_LIMIT = 4,
+    SK_RESULT_NUMERIC_FAILURE = 5,
+    SK_RESULT_NOT_RUN = 6,
+    SK_RESULT_GAP_LIMIT = 7
+} sk_result;
+
+typedef enum sk_vartype { SK_CONTINUOUS = 0, SK_INTEGER = 1 } sk_vartype;
+
+enum { SK_LP_AUTO = 0, SK_LP_SIMPLEX = 1, SK_LP_FIRST_ORDER = 2 };
+
+#define SK_INFINITY 1.0e30
+#define SK_IS_INF(x)     ((x) >=  SK_INFINITY)
+#define SK_IS_NEG_INF(x) ((x) <= -SK_INFINITY)
+
+typedef struct sk_csc {             /* c/include/sankhya.h:64-71 */
+    int     nrow;
+    int     ncol;
+    int    *p;
+    int    *i;
+    double *x;
+    int     nzmax;
+} sk_csc;
+
+typedef struct sk_model {           /* c/include/sankhya.h:82-98 */
+    int         nrow;
+    int         ncol;
+    sk_csc      A;
+    sk_csc     *Q;
+    double     *c;
+    double     *rlow;
+    double     *rupp;
+    double     *clow;
+    double     *cupp;
+    unsigned char *vartype;
+    double      objshift;
+    double      objsense;
+    char      **rowname;
+    char      **colname;
+    char       *name;
+} sk_model;
+
+typedef struct sk_solution {        /* c/include/sankhya.h:114-132 */
+    sk_result result;
+    double    objective;
+    double   *x;
+    double   *y;
+    double   *rc;
+    double   *rowact;
+    int       ncol;
+    int       nrow;
+    long long iterations;
+    long long nodes;
+    double    solve_seconds;
+    double    primal_infeasibility;
+    double    dual_infeasibility;
+    double    complementarity;
+    double    mip_gap;
+    double    dual_bound;
+} sk_solution;
+
+typedef struct sk_options {         /* c/include/sankhya.h:139-161 */
+    double primal_tol;
+    double dual_tol;
+    double pivot_tol;
+    double time_limit;
+    long long iteration_limit;
+    long long node_limit;
+    int    presolve;
+    int    scaling;
+    int    refactor_interval;
+    int    verbosity;
+    int    threads;
+    double mip_gap_abs;
+    double mip_gap_rel;
+    int    mip_cuts;
+    int    mip_heuristics;
+    int    dual_simplex;
+    int    lp_engine;
+    unsigned long long random_seed;
+} sk_options;
+
+#endif /* SANKHYA_H */
+
+/* Model/solution helpers with the reference's semantics.  Exported under
+ * their own names so that this library never clashes with a linked
+ * reference C core. */
+void      sk_gpu_model_init(sk_model *m);                       /* sk_core.c:74-78   sk_model_init     */
+void      sk_gpu_model_free(sk_model *m);                       /* sk_core.c:88-101  sk_model_free     */
+sk_status sk_gpu_model_alloc(sk_model *m, int nrow, int ncol, int nzmax); /* sk_core.c:103-124 */
+sk_status sk_gpu_model_validate(const sk_model *m);             /* sk_core.c:140-155 sk_model_validate */
+void      sk_gpu_solution_init(sk_solution *s);                 /* sk_core.c:166-173 sk_solution_init  */
+void      sk_gpu_solution_free(sk_solution *s);                 /* sk_core.c:175-180 sk_solution_free  */
+void      sk_gpu_options_default(sk_options *o);                /* sk_core.c:184-207 sk_options_default*/
+const char *sk_gpu_result_name(sk_result r);                    /* sk_core.c:31-43   sk_result_name    */
+const char *sk_gpu_status_name(sk_status s);                    /* sk_core.c:45-58   sk_status_name    */
+
+/* ------------------------------------------------------------------------
+ * Device-resident sparse matrix.
+ * Replaces SankhyaCudaCSR (cuda/include/sankhya_cuda.h:14-21).  The first six
+ * fields keep the reference's meaning; `values` are FP64.  The handle also
+ * keeps CSR(A^T) on the device so that A^T*y is a gather, not an atomic
+ * scatter, and an opaque row-bin plan for the binned SpMV kernels.
+ * ---------------------------------------------------------------------- */
+typedef struct sk_gpu_csr {
+    int   rows;
+    int   cols;
+    int   nnz;
+    void *device_row_offsets;       /* int32[rows+1]                      */
+    void *device_column_indices;    /* int32[nnz]                         */
+    void *device_values;            /* double[nnz]                        */
+    void *plan;                     /* opaque: CSR(A^T), bins, FP32 copy  */
+} sk_gpu_csr;
+
+/* cuda/include/sankhya_cuda.h:25-32 sankhya_cuda_csr_create.  Validates
+ * (canonical offsets, in-range columns, finite values) before any device
+ * allocation; frees previous content; 0 on success, -1 on failure. */
+int  sk_gpu_csr_create(sk_gpu_csr *matrix, int rows, int cols, int nnz,
+                       const int *row_offsets, const int *column_indices,
+                       const double *values);
+/* cuda/include/sankhya_cuda.h:33 sankhya_cuda_csr_destroy.  Idempotent. */
+void sk_gpu_csr_destroy(sk_gpu_csr *matrix);
+
+/* cuda/include/sankhya_cuda.h:38-42.  Device pointers; enqueued on the
+ * library stream, return after launch.  0, -1 (bad argument) or a positive
+ * hipError_t.  The transpose product is deterministic (no atomics). */
+int sk_gpu_spmv_device_f64(const sk_gpu_csr *matrix, const double *device_x,
+                           double *device_y);
+int sk_gpu_spmv_transpose_device_f64(const sk_gpu_csr *matrix,
+                                     const double *device_y, double *device_x);
+int sk_gpu_axpy_device_f64(int n, double alpha, const double *device_x,
+                           double *device_y);
+/* Waits for all work the library has queued on the current device. */
+int sk_gpu_synchronize(void);
+
+/* cuda/include/sankhya_cuda.h:153-164: host-memory convenience calls. */
+int sk_gpu_spmv_f64(int rows, int cols, int nnz, const int *row_offsets,
+                    const int *column_indices, const double *values,
+                    const double *x, double *y);
+int sk_gpu_axpy_f64(int n, double alpha, const double *x, double *y);
+
+/* cuda/include/sankhya_cuda.h:167: thread-local text of the last failure. */
+const char *sk_gpu_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Plain PDHG with host-visible checkpoints (the reference's sankhya_cuda_*
+ * solver family, cuda/src/sankhya_cuda_lp.cu:101-437).
+ * ---------------------------------------------------------------------- */
+typedef struct sk_gpu_lp_settings {  /* cuda/include/sankhya_cuda.h:44-52 */
+    int    max_iterations;
+    int    check_every;
+    double tau;
+    double sigma;
+    double theta;
+    double tolerance;
+    double time_limit;               /* seconds, <= 0 disables           */
+} sk_gpu_lp_settings;
+
+typedef struct sk_gpu_lp_result {    /* cuda/include/sankhya_cuda.h:54-68 */
+    int    status;                   /* 0 converged, 1 iteration limit, 2 time limit, -2 numeric */
+    int    iterations;
+    double objective;
+    double maximum_row_violation;
+    double maximum_step;
+    double maximum_kkt_residual;
+    double maximum_dual_residual;
+    double maximum_complementarity;
+} sk_gpu_lp_result;
+
+/* cuda/include/sankhya_cuda.h:72-81 sankhya_cuda_lp_pdhg */
+int sk_gpu_lp_pdhg(const sk_gpu_csr *matrix, const double *c,
+                   const double *row_lower, const double *row_upper,
+                   const double *col_lower, const double *col_upper,
+                   sk_gpu_lp_settings settings, double *solution,
+                   sk_gpu_lp_result *result);
+/* cuda/include/sankhya_cuda.h:85-99 sankhya_cuda_qp_pdhg_preconditioned_with_dual */
+int sk_gpu_qp_pdhg_preconditioned_with_dual(
+    const sk_gpu_csr *matrix, const sk_gpu_csr *hessian,
+    const double *quadratic_diagonal, const double *c,
+    const double *row_lower, const double *row_upper,
+    const double *col_lower, const double *col_upper,
+    const double *primal_steps, const double *dual_steps,
+    sk_gpu_lp_settings settings, double *solution, double *dual_solution,
+    sk_gpu_lp_result *result);
+/* cuda/include/sankhya_cuda.h:106-116 sankhya_cuda_diagonal_qp_pdhg */
+int sk_gpu_diagonal_qp_pdhg(const sk_gpu_csr *matrix,
+                            const double *quadratic_diagonal, const double *c,
+                            const double *row_lower, const double *row_upper,
+                            const double *col_lower, const double *col_upper,
+                            sk_gpu_lp_settings settings, double *solution,
+                            sk_gpu_lp_result *result);
+/* cuda/include/sankhya_cuda.h:121-131 sankhya_cuda_sparse_qp_pdhg */
+int sk_gpu_sparse_qp_pdhg(const sk_gpu_csr *matrix, const sk_gpu_csr *hessian,
+                          const double *c, const double *row_lower,
+                          const double *row_upper, const double *col_lower,
+                          const double *col_upper, sk_gpu_lp_settings settings,
+                          double *solution, sk_gpu_lp_result *result);
+/* cuda/include/sankhya_cuda.h:137-150 sankhya_cuda_qp_pdhg_preconditioned */
+int sk_gpu_qp_pdhg_preconditioned(
+    const sk_gpu_csr *matrix, const sk_gpu_csr *hessian,
+    const double *quadratic_diagonal, const double *c,
+    const double *row_lower, const double *row_upper,
+    const double *col_lower, const double *col_upper,
+    const double *primal_steps, const double *dual_steps,
+    sk_gpu_lp_settings settings, double *solution, sk_gpu_lp_result *result);
+
+/* ------------------------------------------------------------------------
+ * Restarted-average PDHG (PDLP).  Replaces sihps::GpuPdlp
+ * (src/cuda/GpuPdlp.hpp:19-196) together with its scaling front end
+ * run_first_order (src/lp/LpSolver.cpp:62-99).
+ * ---------------------------------------------------------------------- */
+enum {
+    SK_GPU_PREC_FP64  = 0,   /* FP64 iterates and SpMV ("plain FP64")      */
+    SK_GPU_PREC_MIXED = 1    /* FP32 iterates and SpMV, FP64 checks + tail */
+};
+
+typedef struct sk_gpu_pdlp_params {  /* src/cuda/GpuPdlp.hpp:19-67 PdlpParams */
+    double eps_optimal;      /* relative KKT tolerance (default 1e-6)       */
+    int    max_iterations;   /* default 500000                              */
+    int    restart_period;   /* iterations between host checks (64)         */
+    double time_limit_seconds;
+    int    power_iterations; /* ||A||_2 power iterations (30)               */
+    int    adaptive_step;    /* Applegate et al. 2021 s3.1 (1)              */
+    double eta_min_factor;   /* 1e-6                                        */
+    double eta_max_factor;   /* 1e4                                         */
+    int    precision;        /* SK_GPU_PREC_*                               */
+    double tail_switch;      /* MIXED: KKT at which FP64 takes over (1e-5)  */
+    int    tail_stall_windows; /* MIXED: windows without progress before the
+                                  FP64 tail starts early (8)                */
+    int    scaling;          /* 1: Ruiz then Pock-Chambolle, 0: none        */
+    int    ruiz_iterations;  /* 100                                         */
+    double ruiz_tol;         /* 1e-2                                        */
+    int    use_graph;        /* capture each restart window in a hipGraph   */
+    int    verbosity;
+} sk_gpu_pdlp_params;
+
+typedef struct sk_gpu_pdlp_stats {   /* src/cuda/GpuPdlp.hpp:69-92 PdlpStats */
+    int    converged;
+    int    cancelled;
+    int    iterations;
+    int    restarts;
+    double primal_objective;
+    double dual_objective;
+    double relative_primal_residual;
+    double relative_dual_residual;
+    double relative_gap;
+    double matrix_norm;
+    double seconds;
+    int    rejected_steps;
+    double final_step_size;
+    double step_size_ratio;
+    int    host_syncs;
+    /* additions */
+    int    fp32_iterations;      /* iterations run with FP32 iterates       */
+    int    fp64_iterations;      /* iterations run with FP64 iterates       */
+    long long kernel_launches;
+    double loop_seconds;         /* device-resident loop only               */
+    double setup_seconds;        /* scaling, binning, upload                */
+    double primal_weight;
+} sk_gpu_pdlp_stats;
+
+void sk_gpu_pdlp_params_default(sk_gpu_pdlp_params *p);
+
+/* Solve min c'x (+ 1/2 x'Qx) s.t. row_lower <= A x <= row_upper,
+ * col_lower <= x <= col_upper.  A is CSR in host memory (FP64); Q, if given,
+ * is a full symmetric CSR (n x n).  Bounds use IEEE +-inf or the +-1e30
+ * sentinel.  x0/y0 are an optional warm start (either may be NULL).  Outputs
+ * x[n], y[m] in host memory, in the original (unscaled) space, with the
+ * sign convention of sk_verify: rc = c + Qx + A'y.  Returns 0 (read
+ * stats->converged), -1 invalid input / device failure, -2 non-finite
+ * iterate. */
+int sk_gpu_pdlp_solve(int rows, int cols, int nnz, const int *row_offsets,
+                      const int *column_indices, const double *values,
+                      const int *q_row_offsets, const int *q_column_indices,
+                      const double *q_values, int q_nnz,
+                      const double *c, const double *row_lower,
+                      const double *row_upper, const double *col_lower,
+                      const double *col_upper, const double *x0,
+                      const double *y0, const sk_gpu_pdlp_params *params,
+                      double *x, double *y, sk_gpu_pdlp_stats *stats);
+
+/* ------------------------------------------------------------------------
+ * sk_solve-style entry point.  Mirrors sk_solve (c/include/sankhya.h:173,
+ * c/src/sk_solve.c:1009-1046) for continuous LP / convex QP, running the
+ * GPU engine; OPTIMAL is reported only after the FP64 certificate computed
+ * on the device passes.
+ * ---------------------------------------------------------------------- */
+typedef struct sk_gpu_options {
+    sk_gpu_pdlp_params pdlp;
+    const double *x0;        /* optional warm start, original space          */
+    const double *y0;
+    sk_gpu_pdlp_stats stats; /* out: solve statistics                        */
+} sk_gpu_options;
+
+void      sk_gpu_solver_options_default(sk_gpu_options *g);
+sk_status sk_gpu_solve(const sk_model *m, const sk_options *o,
+                       sk_gpu_options *g, sk_solution *s);
+/* c/include/sankhya.h:177 sk_verify, evaluated on the device in FP64. */
+sk_status sk_gpu_verify(const sk_model *m, sk_solution *s);
+
+/* PDLP relative KKT triple in FP64 on the device
+ * (src/cuda/GpuPdlp.cpp:171-176, 226-249): out[0] primal, out[1] dual,
+ * out[2] gap, out[3] primal objective, out[4] dual objective. */
+int sk_gpu_relative_kkt(int rows, int cols, int nnz, const int *row_offsets,
+                        const int *column_indices, const double *values,
+                        const double *c, const double *row_lower,
+                        const double *row_upper, const double *col_lower,
+                        const double *col_upper, const double *x,
+                        const double *y, double out[5]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SANKHYA_GPU_H */

@@ -1,0 +1,193 @@
+"""ctypes loader for the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg,
+always as the checker.  ``liboracle.so`` is the FP64 C restatement
+(sk_oracle.c); ``_ref/libsankhya_ref.so`` is the reference C core compiled
+from its own sources (oracle/Makefile), present only where it was built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE.parent / "the-solver_amd"))
+import sankhya_gpu as sg  # noqa: E402  (ctypes structure definitions only)
+
+ORACLE_LIB = HERE / "liboracle.so"
+REF_LIB = HERE / "_ref" / "libsankhya_ref.so"
+
+
+class OrcCsr(C.Structure):
+    _fields_ = [("rows", C.c_int), ("cols", C.c_int), ("nnz", C.c_int),
+                ("p", sg.c_int_p), ("j", sg.c_int_p), ("v", sg.c_double_p)]
+
+
+_orc = None
+_ref = None
+
+
+def _decl(lib, name, restype, argtypes):
+    f = getattr(lib, name)
+    f.restype, f.argtypes = restype, argtypes
+
+
+def load():
+    global _orc
+    if _orc is None:
+        if not ORACLE_LIB.exists():
+            raise RuntimeError(f"{ORACLE_LIB} is not built (make -C oracle)")
+        lib = C.CDLL(str(ORACLE_LIB))
+        M, S = C.POINTER(sg.SkModel), C.POINTER(sg.SkSolution)
+        R = C.POINTER(OrcCsr)
+        D = sg.c_double_p
+        _decl(lib, "orc_verify", C.c_int, [M, S])
+        _decl(lib, "orc_model_validate", C.c_int, [M])
+        _decl(lib, "orc_interval_infeasible", C.c_int, [M, C.c_double])
+        _decl(lib, "orc_solve_first_order", C.c_int, [M, C.POINTER(sg.SkOptions), S])
+        _decl(lib, "orc_solution_free", None, [S])
+        _decl(lib, "orc_options_default", None, [C.POINTER(sg.SkOptions)])
+        _decl(lib, "orc_cuda_pdhg", C.c_int, [R, R, D, D, D, D, D, D, sg.LpSettings, D, D, D, D,
+                                              C.POINTER(sg.LpResult)])
+        _decl(lib, "orc_ruiz", None, [R, C.c_int, C.c_double, D, D])
+        _decl(lib, "orc_pock_chambolle", None, [R, C.c_double, D, D])
+        _decl(lib, "orc_power_norm", C.c_double, [R, C.c_int])
+        _decl(lib, "orc_pdlp_solve", C.c_int, [R, D, D, D, D, D, C.POINTER(sg.PdlpParams), D, D, D, D,
+                                               C.POINTER(sg.PdlpStats)])
+        _decl(lib, "orc_first_order", C.c_int, [R, D, D, D, D, D, C.POINTER(sg.PdlpParams), D, D,
+                                                C.POINTER(sg.PdlpStats)])
+        _decl(lib, "orc_relative_kkt", None, [R, R, D, D, D, D, D, D, D, D])
+        _orc = lib
+    return _orc
+
+
+def load_ref():
+    """The reference C core, or None where oracle/_ref was not built."""
+    global _ref
+    if _ref is None and REF_LIB.exists():
+        lib = C.CDLL(str(REF_LIB))
+        M, S = C.POINTER(sg.SkModel), C.POINTER(sg.SkSolution)
+        _decl(lib, "sk_solve", C.c_int, [M, C.POINTER(sg.SkOptions), S])
+        _decl(lib, "sk_verify", C.c_int, [M, S])
+        _decl(lib, "sk_options_default", None, [C.POINTER(sg.SkOptions)])
+        _decl(lib, "sk_solution_init", None, [S])
+        _decl(lib, "sk_solution_free", None, [S])
+        _decl(lib, "sk_model_validate", C.c_int, [M])
+        _ref = lib
+    return _ref
+
+
+class Csr:
+    """An orc_csr over numpy arrays (kept alive by this object)."""
+
+    def __init__(self, rows, cols, rp, cj, v):
+        self.rp = np.ascontiguousarray(rp, np.int32)
+        self.cj = np.ascontiguousarray(cj, np.int32) if len(cj) else np.zeros(1, np.int32)
+        self.v = np.ascontiguousarray(v, np.float64) if len(v) else np.zeros(1, np.float64)
+        self.s = OrcCsr(rows, cols, int(self.rp[-1]), sg.iptr(self.rp), sg.iptr(self.cj), sg.dptr(self.v))
+
+    @classmethod
+    def from_model(cls, model):
+        p, i, x = model.csc()
+        rp, cj, v = sg.csc_to_csr(model.nrow, model.ncol, p, i, x)
+        return cls(model.nrow, model.ncol, rp, cj, v)
+
+    @classmethod
+    def from_q(cls, model):
+        q = model.q_csc()
+        if q is None:
+            return None
+        # Q is symmetric: its CSC is its CSR
+        return cls(model.ncol, model.ncol, q[0], q[1], q[2])
+
+    def ref(self):
+        return C.byref(self.s)
+
+
+def solution_arrays(sol):
+    x = np.ctypeslib.as_array(sol.x, shape=(sol.ncol,)).copy() if sol.x else None
+    y = np.ctypeslib.as_array(sol.y, shape=(sol.nrow,)).copy() if sol.y and sol.nrow else None
+    return x, y
+
+
+def first_order_ref(model, iteration_limit=0, threads=1):
+    """Reference sk_solve with lp_engine = SK_LP_FIRST_ORDER (None if absent)."""
+    ref = load_ref()
+    if ref is None:
+        return None
+    o = sg.SkOptions()
+    ref.sk_options_default(C.byref(o))
+    o.lp_engine = sg.SK_LP_FIRST_ORDER
+    o.iteration_limit = iteration_limit
+    o.threads = threads
+    o.verbosity = 0
+    s = sg.SkSolution()
+    rc = ref.sk_solve(model.ptr(), C.byref(o), C.byref(s))
+    return rc, s
+
+
+def first_order_orc(model, iteration_limit=0):
+    orc = load()
+    o = sg.SkOptions()
+    orc.orc_options_default(C.byref(o))
+    o.lp_engine = sg.SK_LP_FIRST_ORDER
+    o.iteration_limit = iteration_limit
+    o.verbosity = 0
+    s = sg.SkSolution()
+    rc = orc.orc_solve_first_order(model.ptr(), C.byref(o), C.byref(s))
+    return rc, s
+
+
+def simplex_ref(model):
+    """Reference revised simplex through sk_solve's default engine."""
+    ref = load_ref()
+    if ref is None:
+        return None
+    o = sg.SkOptions()
+    ref.sk_options_default(C.byref(o))
+    o.verbosity = 0
+    s = sg.SkSolution()
+    rc = ref.sk_solve(model.ptr(), C.byref(o), C.byref(s))
+    return rc, s
+
+
+def verify_orc(model, x, y):
+    orc = load()
+    s = sg.SkSolution()
+    s.result = 6
+    xs = np.ascontiguousarray(x, np.float64)
+    ys = np.ascontiguousarray(y, np.float64) if y is not None else None
+    s.x = sg.dptr(xs)
+    s.y = sg.dptr(ys) if ys is not None and ys.size else None
+    s.ncol, s.nrow = model.ncol, model.nrow
+    rc = orc.orc_verify(model.ptr(), C.byref(s))
+    return rc, s.objective, s.primal_infeasibility, s.dual_infeasibility, s.complementarity
+
+
+def relative_kkt(model, x, y):
+    orc = load()
+    A = Csr.from_model(model)
+    Q = Csr.from_q(model)
+    out = np.zeros(5)
+    c, rl, ru, l, u = (model.vec(k) for k in ("c", "rlow", "rupp", "clow", "cupp"))
+    xs = np.ascontiguousarray(x, np.float64)
+    ys = np.ascontiguousarray(y, np.float64) if len(y) else np.zeros(1)
+    orc.orc_relative_kkt(A.ref(), Q.ref() if Q else None, sg.dptr(c), sg.dptr(rl), sg.dptr(ru),
+                         sg.dptr(l), sg.dptr(u), sg.dptr(xs), sg.dptr(ys), sg.dptr(out))
+    return out
+
+
+def first_order_pdlp(model, params):
+    """Scaling + PDLP + unscaling on the CPU, FP64 (orc_first_order)."""
+    orc = load()
+    A = Csr.from_model(model)
+    c, rl, ru, l, u = (model.vec(k) for k in ("c", "rlow", "rupp", "clow", "cupp"))
+    x = np.zeros(max(model.ncol, 1))
+    y = np.zeros(max(model.nrow, 1))
+    st = sg.PdlpStats()
+    rc = orc.orc_first_order(A.ref(), sg.dptr(c), sg.dptr(l), sg.dptr(u), sg.dptr(rl), sg.dptr(ru),
+                             C.byref(params), sg.dptr(x), sg.dptr(y), C.byref(st))
+    return rc, x[:model.ncol], y[:model.nrow], st

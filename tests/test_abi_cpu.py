@@ -1,0 +1,89 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports
+every entry point include/sankhya_gpu.h declares; argument validation happens
+before any device call; the workload generators follow their laws."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+
+import sankhya_gpu as sg
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    text = (ROOT / "include" / "sankhya_gpu.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(sk_gpu_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(str(sg.GPU_LIB))
+    names = declared_functions()
+    assert len(names) >= 29
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_csr_create_validates_before_touching_the_device():
+    lib = sg.load_gpu()
+    h = sg.GpuCsr()
+    off = np.array([0, 3, 2], np.int32)
+    col = np.array([0, 2, 1, 2], np.int32)
+    val = np.array([1.0, 2.0, 3.0, 4.0])
+    assert lib.sk_gpu_csr_create(C.byref(h), 2, 3, 4, sg.iptr(off), sg.iptr(col), sg.dptr(val)) == -1
+    assert "row-offset" in sg.last_error()
+    off = np.array([0, 2, 4], np.int32)
+    bad = np.array([0, 3, 1, 2], np.int32)
+    assert lib.sk_gpu_csr_create(C.byref(h), 2, 3, 4, sg.iptr(off), sg.iptr(bad), sg.dptr(val)) == -1
+    assert "column index" in sg.last_error()
+
+
+def test_params_defaults_and_names():
+    lib = sg.load_gpu()
+    p = sg.PdlpParams()
+    lib.sk_gpu_pdlp_params_default(C.byref(p))
+    assert p.eps_optimal == 1e-6 and p.restart_period == 64 and p.power_iterations == 30
+    assert p.adaptive_step == 1 and p.eta_min_factor == 1e-6 and p.eta_max_factor == 1e4
+    assert lib.sk_gpu_result_name(0) == b"optimal" and lib.sk_gpu_status_name(3) == b"invalid_structure"
+    o = sg.SkOptions()
+    lib.sk_gpu_options_default(C.byref(o))
+    assert o.primal_tol == 1e-7 and o.random_seed == 20260825 and o.threads == 1
+
+
+def test_model_validate_matches_reference_rules():
+    lib = sg.load_gpu()
+    good = sg.Model.from_arrays(1, 1, [0, 1], [0], [1.0], [1.0], [1.0], [2.0], [0.0], [1.0])
+    assert lib.sk_gpu_model_validate(good.ptr()) == 0
+    bad = sg.Model.from_arrays(1, 1, [0, 1], [0], [np.inf], [1.0], [1.0], [2.0], [0.0], [1.0])
+    assert lib.sk_gpu_model_validate(bad.ptr()) == sg.SK_ERR_STRUCTURE
+
+
+def test_generators_are_deterministic_and_follow_their_laws():
+    a = sg.Model.generated("skw_bounded_lp", 100, 200, 5, 42)
+    b = sg.Model.generated("skw_bounded_lp", 100, 200, 5, 42)
+    for u, v in zip(a.csc(), b.csc()):
+        assert np.array_equal(u, v)
+    p, i, x = a.csc()
+    assert np.all(np.diff(p) == 5)
+    for j in range(200):
+        col = i[p[j]:p[j + 1]]
+        assert np.all(np.diff(col) > 0)
+    assert np.all(a.vec("clow") == 0) and np.all(a.vec("cupp") == 10)
+    q = C.c_double(0)
+    w = sg.load_workloads()
+    z = sg.Model()
+    assert w.skw_zipf_lp(C.byref(z.m), 20000, 40000, 1.2, 8.0, 2000, 0.3, 7, C.byref(q)) == 0
+    z._generated = True
+    deg = np.diff(z.csc()[0])
+    assert -1.0 < q.value < 0.0
+    assert abs(deg.mean() - 8.0) < 0.8 and deg.max() <= 2000 and deg.min() >= 1
+    eq = z.vec("rlow") == z.vec("rupp")
+    assert abs(eq.mean() - 0.3) < 0.02
+    qp = sg.Model.generated("skw_sparse_qp", 60, 120, 3, 40, 3, 5)
+    qp_, qi, qx = qp.q_csc()
+    import scipy.sparse as sp
+    Q = sp.csc_matrix((qx, qi, qp_), shape=(120, 120)).toarray()
+    assert np.allclose(Q, Q.T) and np.linalg.eigvalsh(Q).min() > 0.0

@@ -1,0 +1,169 @@
+/* Thin C ABI between the C host driver and the HIP/CDNA4 kernels.
+ *
+ * The host driver (skg_host.c) never includes a HIP header: every device
+ * allocation, copy, stream/graph operation and kernel launch goes through the
+ * skk_* functions below, implemented in skg_kernels.hip.  All work is queued
+ * on one library-owned stream (default priority).  Return codes: 0 ok,
+ * otherwise a HIP error number (the text is in skk_error_text()).
+ */
+#ifndef SKG_KERNELS_H
+#define SKG_KERNELS_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ runtime */
+int  skk_device_count(int *count);
+int  skk_malloc(void **ptr, size_t bytes);
+void skk_free(void *ptr);
+int  skk_h2d(void *dst, const void *src, size_t bytes);        /* stream-ordered, waits */
+int  skk_d2h(void *dst, const void *src, size_t bytes);        /* stream-ordered, waits */
+int  skk_d2h_async(void *dst, const void *src, size_t bytes);  /* pinned dst only       */
+int  skk_d2d(void *dst, const void *src, size_t bytes);        /* async                 */
+int  skk_host_alloc(void **ptr, size_t bytes);                 /* pinned host memory    */
+void skk_host_free(void *ptr);
+int  skk_sync(void);
+int  skk_last_launch_error(void);
+const char *skk_error_text(int code);
+/* Capture everything queued between begin and end into a graph. */
+int  skk_graph_begin(void);
+int  skk_graph_end(void **exec);
+int  skk_graph_launch(void *exec);
+void skk_graph_destroy(void *exec);
+/* Event timing on the library stream (milliseconds between two marks). */
+int    skk_mark(int slot);
+double skk_elapsed_ms(int slot_a, int slot_b);
+void  *skk_stream(void);
+
+/* ------------------------------------------------------------- sparse */
+#define SKK_MAX_BINS 12
+#define SKK_BLOCK 256
+
+/* Rows [row_begin,row_end) of a bin are reduced by 2^log2L lanes each
+ * (log2L 0..6 inside one wavefront; 8 = one 256-thread workgroup per row with
+ * an LDS combine across its four wavefronts).  Bins own consecutive
+ * workgroups starting at block_begin. */
+typedef struct skk_bin { int row_begin, row_end, log2L, block_begin; } skk_bin;
+typedef struct skk_bins {
+    int nbins, total_blocks;
+    skk_bin b[SKK_MAX_BINS];
+    const int *row_list;     /* NULL: bin slot k is row k; else row_list[k] */
+} skk_bins;
+
+typedef struct skk_csr {
+    int rows, cols, nnz;
+    const int *rp, *ci;
+    const float *vf;         /* FP32 values (may be NULL) */
+    const double *vd;        /* FP64 values               */
+    skk_bins bins;
+} skk_csr;
+
+/* y = A x in FP64 (plain product, public ABI). */
+int skk_spmv_f64(const skk_csr *A, const double *x, double *y);
+int skk_axpy_f64(int n, double alpha, const double *x, double *y);
+
+/* --------------------------------------------------- PDHG device state */
+
+/* Device-resident scalars of the restarted PDHG loop (FP64).  The step
+ * accept/reject decision, the buffer rotation that replaces a commit copy,
+ * and the accepted-step counters all live here, so a whole restart window
+ * runs without the host. */
+typedef struct skk_state {
+    double eta, omega, eta_min, eta_max;
+    double updates, accept_count, accepted_at_restart;
+    double dx2, dy2, inter, eta_bar;
+    double eta_fixed;        /* fixed-step path: eta = 0.9/||A||           */
+    int    accepted;         /* last step accepted                          */
+    int    pending;          /* accepted iterate not yet added to the sums  */
+    int    cur;              /* X[cur], Y[cur] hold the iterate             */
+    int    a0, a1, a2;       /* AX[a0] = A x, AX[a1] = A x_prev, AX[a2] cand */
+    int    pad[2];
+} skk_state;
+
+/* Iterate storage in one precision (float for MIXED, double for FP64). */
+typedef struct skk_iter {
+    void *X[2], *Y[2], *AX[3];
+    void *xsum, *ysum, *xr, *yr, *xavg, *yavg, *xbar;
+} skk_iter;
+
+typedef struct skk_pdhg {
+    int m, n;
+    skk_csr A, AT;                           /* AT is CSR(A^T)              */
+    const float *cf, *lf, *uf, *rlf, *ruf;   /* FP32 copies of the data     */
+    const double *cd, *ld, *ud, *rld, *rud;  /* FP64 data                   */
+    skk_iter it[2];                          /* [0] FP64, [1] FP32          */
+    double *ax_d, *aty_d;                    /* FP64 products for checks    */
+    skk_state *st;
+    double *part;                            /* block partials              */
+    unsigned *counters;                      /* self-resetting tickets      */
+    double *eval_out;                        /* [2][8] residual sums        */
+    int part_capacity;                       /* blocks per partial slot     */
+    int elem_grid;                           /* fixed grid for reductions   */
+} skk_pdhg;
+
+enum { SKK_F64 = 0, SKK_F32 = 1 };
+
+int skk_pdhg_init(const skk_pdhg *P, int prec, double eta0, double omega, double eta_min,
+                  double eta_max, int warm_x, int warm_y);
+int skk_pdhg_adaptive_iter(const skk_pdhg *P, int prec);
+int skk_pdhg_fixed_iter(const skk_pdhg *P, int prec);
+int skk_pdhg_flush(const skk_pdhg *P, int prec);
+/* which 0: current iterate, 1: running average (formed here). */
+int skk_pdhg_eval(const skk_pdhg *P, int prec, int which, int adaptive, double fixed_terms);
+int skk_pdhg_restart(const skk_pdhg *P, int prec, int use_average, int adaptive);
+int skk_pdhg_set_omega(const skk_pdhg *P, double omega);
+/* Copy the FP32 iterate into the FP64 set and continue from it. */
+int skk_pdhg_promote(const skk_pdhg *P);
+/* Store the final iterate (or the average) as FP64 into x_out/y_out. */
+int skk_pdhg_export(const skk_pdhg *P, int prec, int use_average, double *x_out, double *y_out);
+/* ||A||_2 by power iteration on A'A in FP64; v, v2 length n, w length m.
+ * Leaves ||A'A v||^2 of the last step in *out (device). */
+int skk_power_norm(const skk_pdhg *P, double *v, double *w, double *v2, int iters, double *out);
+/* Reset the adaptive-step state to a new eta (FP64 tail start). */
+int skk_pdhg_set_eta(const skk_pdhg *P, double eta);
+
+/* ------------------------------------------------------ certificates */
+
+/* sk_verify (c/src/sk_core.c:394-510) on the device in FP64 over the
+ * original model: out[0] objective (without objshift), out[1] primal
+ * infeasibility, out[2] dual infeasibility, out[3] complementarity max
+ * product, out[4] dual objective contribution.  act[m], rc[n] are written. */
+typedef struct skk_verify_in {
+    int m, n;
+    skk_csr A, AT;
+    const skk_csr *Q;       /* may be NULL (full symmetric CSR) */
+    const double *c, *rl, *ru, *l, *u, *x, *y;
+    double *act, *rc, *qx;
+    double *part; unsigned *counters; double *out; int grid;
+} skk_verify_in;
+int skk_verify(const skk_verify_in *V);
+
+/* PDLP relative KKT sums in FP64 (same slots as the loop's evaluation):
+ * out[0] ||Ax-proj Ax||^2, [1] dual violation^2, [2] c'x (+x'Qx), [3] dual
+ * objective, [4] x'Qx.  Bounds with |b| >= 1e30 are absent. */
+int skk_relative_kkt(const skk_verify_in *V);
+
+
+/* ------------------------------------------- plain PDHG (legacy ABI) */
+typedef struct skk_legacy {
+    int m, n;
+    const skk_csr *A, *AT, *Q;             /* Q: full symmetric CSR or NULL */
+    double *x, *xbar, *y, *act, *grad, *qg, *prev;
+    const double *c, *l, *u, *rl, *ru, *tau_vec, *sigma_vec, *qd;
+    double tau, sigma, theta;
+    double *part; unsigned *counters; double *out; int grid;
+} skk_legacy;
+int skk_legacy_init(double *x, double *xbar, const double *l, const double *u, int n, double *y, int m);
+int skk_legacy_iter(const skk_legacy *L);
+/* out[0] objective, [1] non-finite count, [2] step, [3] scale, [4] row
+ * violation, [5] projected KKT, [6] dual residual, [7] complementarity */
+int skk_legacy_check(const skk_legacy *L, int have_prev);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,1372 @@
+// HIP/CDNA4 (gfx950) kernels and the thin C ABI the host driver calls.
+//
+// Design (MI355X-first, not a translation of the reference's CUDA):
+//  * Sparse products are CSR gathers on two stored operators, A and A^T, so no
+//    atomics are needed (the reference's transpose scatters with atomicAdd,
+//    cuda/src/sankhya_cuda.cu:45-53).  Rows are grouped into length bins and
+//    each bin reduces a row with 2^k lanes of a 64-wide wavefront (k = 0..6),
+//    or with a whole 256-thread workgroup combined through LDS for the long
+//    power-law rows.  One launch covers every bin.  Matrix streams use
+//    non-temporal loads so L2 keeps the gathered vector.
+//  * The PDHG updates are epilogues of those products: the dual prox is
+//    fused into A*x, the primal prox + extrapolation + running sums into
+//    A^T*y (src/cuda/PdlpKernels.cu:71-97 are separate kernels there).
+//  * Reductions are two-stage: wavefront butterfly + LDS inside a workgroup,
+//    then the last workgroup to retire combines the per-block partials in
+//    ascending block order, so every sum is bit-reproducible for a fixed
+//    grid (the property PdlpKernels.cu:10-25 asks for).
+//  * The adaptive-step accept/reject decision (PdlpKernels.cu:405-518) is
+//    taken on the device and committed by rotating buffer indices in a
+//    device-resident state struct instead of copying vectors, so a restart
+//    window is a fixed sequence of launches that can be captured in a graph.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <type_traits>
+
+#include "skg_kernels.h"
+
+namespace {
+
+constexpr int kBlock = SKK_BLOCK;
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxDevices = 16;
+
+hipStream_t g_stream[kMaxDevices] = {};
+hipEvent_t g_marks[kMaxDevices][8] = {};
+thread_local int g_last = 0;
+
+hipStream_t stream()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = 0;
+    if (!g_stream[dev]) {
+        // default priority only (no priority streams on these boxes)
+        if (hipStreamCreateWithFlags(&g_stream[dev], hipStreamNonBlocking) != hipSuccess)
+            g_stream[dev] = nullptr;
+    }
+    return g_stream[dev];
+}
+
+int check(hipError_t e)
+{
+    g_last = static_cast<int>(e);
+    return static_cast<int>(e);
+}
+
+int launch_status()
+{
+    return check(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ helpers
+
+template <typename T> __device__ __forceinline__ T clampv(T v, T lo, T hi)
+{
+    // fmin/fmax: an infinite bound never binds
+    return fmin(fmax(v, lo), hi);
+}
+
+__device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
+__device__ __forceinline__ float shfl_xor_d(float v, int m) { return __shfl_xor(v, m, 64); }
+
+template <typename T> __device__ __forceinline__ T ntload(const T *p)
+{
+    return __builtin_nontemporal_load(p);
+}
+
+// Sum over lanes of one wavefront (butterfly, fixed order).
+template <typename T> __device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += shfl_xor_d(v, off);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Reduce S doubles over the workgroup; thread s < S gets total s.
+template <int S, bool kMax = false>
+__device__ __forceinline__ void block_reduce(double (&v)[S], double (&out)[S])
+{
+    __shared__ double sh[S > 0 ? S : 1][kWaves];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const double t = kMax ? wave_max(v[s]) : wave_sum(v[s]);
+        if (lane == 0) sh[s][wave] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        double t = sh[s][0];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) t = kMax ? fmax(t, sh[s][w]) : t + sh[s][w];
+        out[s] = t;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void store_coherent(double *p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double load_coherent(const double *p)
+{
+    return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// True in exactly one workgroup: the last to retire.  The ticket counter
+// wraps back to zero, so it needs no reset launch (graph-safe).
+__device__ __forceinline__ bool last_block(unsigned *counter)
+{
+    __shared__ int is_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned prev =
+            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = (prev == gridDim.x - 1);
+        if (is_last) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (is_last) __threadfence();
+    return is_last != 0;
+}
+
+// Block partials of S slots -> part[s*stride + blockIdx.x]; if this is the
+// last block, totals[s] (ascending block order) are returned in every thread
+// of that block.
+template <int S, bool kMax = false>
+__device__ bool partials_and_total(double (&v)[S], double *part, int stride, unsigned *counter,
+                                   double (&totals)[S])
+{
+    double blk[S];
+    block_reduce<S, kMax>(v, blk);
+    if (threadIdx.x < S) {
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            if (s == (int)threadIdx.x) store_coherent(part + (size_t)s * stride + blockIdx.x, blk[s]);
+    }
+    if (!last_block(counter)) return false;
+    double acc[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        acc[s] = kMax ? -INFINITY : 0.0;
+        for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock) {
+            const double t = load_coherent(part + (size_t)s * stride + b);
+            acc[s] = kMax ? fmax(acc[s], t) : acc[s] + t;
+        }
+    }
+    block_reduce<S, kMax>(acc, totals);
+    return true;
+}
+
+// Reduce `count` partials of one slot written by an earlier launch.
+__device__ double total_of(const double *part, int count)
+{
+    double acc[1] = {0.0};
+    for (int b = threadIdx.x; b < count; b += kBlock) acc[0] += load_coherent(part + b);
+    double out[1];
+    block_reduce<1>(acc, out);
+    return out[0];
+}
+
+// ----------------------------------------------------------- binned SpMV
+
+// Lane-strided partial dot product of one row; two accumulators and
+// non-temporal matrix loads.
+template <typename VT, typename XT, typename AT>
+__device__ __forceinline__ AT row_part(const int *__restrict__ ci, const VT *__restrict__ val,
+                                       const XT *__restrict__ x, int p, int e, int L)
+{
+    AT s0 = 0, s1 = 0;
+    for (; p + L < e; p += 2 * L) {
+        const int c0 = ntload(ci + p), c1 = ntload(ci + p + L);
+        const VT v0 = ntload(val + p), v1 = ntload(val + p + L);
+        s0 += static_cast<AT>(v0) * static_cast<AT>(x[c0]);
+        s1 += static_cast<AT>(v1) * static_cast<AT>(x[c1]);
+    }
+    if (p < e) s0 += static_cast<AT>(ntload(val + p)) * static_cast<AT>(x[ntload(ci + p)]);
+    return s0 + s1;
+}
+
+// The gathered vector is x0, or -- when `sel` is given -- the candidate
+// buffer of the rotating pair (x0, x1) named by the device state (cur ^ 1).
+template <typename VT, typename XT, typename AT, class Epi>
+__global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
+                                                 const int *__restrict__ ci,
+                                                 const VT *__restrict__ val, const XT *x0,
+                                                 const XT *x1, const skk_state *sel,
+                                                 skk_bins bins, Epi epi)
+{
+    Epi e = epi;
+    e.begin();
+    const XT *__restrict__ x = (sel && (sel->cur ^ 1)) ? x1 : x0;
+    const int blk = blockIdx.x;
+    int b = 0;
+    while (b + 1 < bins.nbins && blk >= bins.b[b + 1].block_begin) ++b;
+    const skk_bin d = bins.b[b];
+    double part[Epi::kSlots > 0 ? Epi::kSlots : 1] = {};
+    if (d.log2L <= 6) {
+        const int log2L = d.log2L, L = 1 << log2L;
+        const int lane = threadIdx.x & (L - 1);
+        const int k = d.row_begin + ((blk - d.block_begin) << (8 - log2L)) + (threadIdx.x >> log2L);
+        const bool valid = k < d.row_end;
+        int row = 0;
+        AT sum = 0;
+        if (valid) {
+            row = bins.row_list ? bins.row_list[k] : k;
+            sum = row_part<VT, XT, AT>(ci, val, x, rp[row] + lane, rp[row + 1], L);
+        }
+        for (int off = L >> 1; off > 0; off >>= 1) sum += shfl_xor_d(sum, off);
+        if (valid && lane == 0) e.row(row, sum, part);
+    } else {
+        __shared__ AT sh[kWaves];
+        const int k = d.row_begin + (blk - d.block_begin);
+        const int row = bins.row_list ? bins.row_list[k] : k;
+        AT sum = row_part<VT, XT, AT>(ci, val, x, rp[row] + threadIdx.x, rp[row + 1], kBlock);
+        sum = wave_sum(sum);
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = sum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            AT t = sh[0];
+            for (int w = 1; w < kWaves; ++w) t += sh[w];
+            e.row(row, t, part);
+        }
+    }
+    e.end(part);
+}
+
+template <typename YT, typename AT> struct EpiStore {
+    static constexpr int kSlots = 0;
+    YT *y;
+    __device__ void begin() {}
+    __device__ void row(int r, AT s, double *) const { y[r] = static_cast<YT>(s); }
+    __device__ void end(double *) const {}
+};
+
+template <typename VT, typename XT, typename AT, class Epi>
+int launch_spmv(const skk_csr &A, const VT *vals, const XT *x0, const XT *x1,
+                const skk_state *sel, const Epi &epi)
+{
+    if (A.rows == 0 || A.bins.total_blocks == 0) return 0;
+    hipLaunchKernelGGL((k_spmv<VT, XT, AT, Epi>), dim3(A.bins.total_blocks), dim3(kBlock), 0,
+                       stream(), A.rp, A.ci, vals, x0, x1, sel, A.bins, epi);
+    return launch_status();
+}
+
+// ------------------------------------------------------------ PDHG state
+
+template <typename T> struct Iter {
+    T *X[2], *Y[2], *AX[3], *xsum, *ysum, *xr, *yr, *xavg, *yavg, *xbar;
+    __host__ __device__ Iter() {}
+    explicit Iter(const skk_iter &it)
+    {
+        for (int k = 0; k < 2; ++k) X[k] = static_cast<T *>(it.X[k]), Y[k] = static_cast<T *>(it.Y[k]);
+        for (int k = 0; k < 3; ++k) AX[k] = static_cast<T *>(it.AX[k]);
+        xsum = static_cast<T *>(it.xsum); ysum = static_cast<T *>(it.ysum);
+        xr = static_cast<T *>(it.xr); yr = static_cast<T *>(it.yr);
+        xavg = static_cast<T *>(it.xavg); yavg = static_cast<T *>(it.yavg);
+        xbar = static_cast<T *>(it.xbar);
+    }
+};
+
+template <typename T> struct Data {
+    const T *c, *l, *u, *rl, *ru;
+};
+
+template <typename T> Data<T> data_of(const skk_pdhg &P);
+template <> Data<float> data_of<float>(const skk_pdhg &P) { return {P.cf, P.lf, P.uf, P.rlf, P.ruf}; }
+template <> Data<double> data_of<double>(const skk_pdhg &P) { return {P.cd, P.ld, P.ud, P.rld, P.rud}; }
+
+template <typename T> const T *vals_of(const skk_csr &A);
+template <> const float *vals_of<float>(const skk_csr &A) { return A.vf; }
+template <> const double *vals_of<double>(const skk_csr &A) { return A.vd; }
+
+// Partial-slot regions (each part_capacity doubles).
+enum { R_DX = 0, R_DY = 1, R_INTER = 2, R_EVAL = 3, R_POWER = 11, R_VERIFY = 12 };
+enum { C_PROBE = 0, C_EVAL = 1, C_POWER = 2, C_VERIFY = 3, C_KKT = 4 };
+
+__global__ void k_init_state(skk_state *st, double eta0, double omega, double eta_min,
+                             double eta_max)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    st->eta = eta0; st->omega = omega; st->eta_min = eta_min; st->eta_max = eta_max;
+    st->updates = 0.0; st->accept_count = 0.0; st->accepted_at_restart = 0.0;
+    st->dx2 = st->dy2 = st->inter = 0.0; st->eta_bar = eta0; st->eta_fixed = eta0;
+    st->accepted = 1; st->pending = 0; st->cur = 0; st->a0 = 0; st->a1 = 1; st->a2 = 2;
+}
+
+__global__ void k_set_omega(skk_state *st, double omega)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    st->omega = omega;
+}
+
+__global__ void k_set_eta(skk_state *st, double eta)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    st->eta = eta;
+    st->eta_fixed = eta;
+}
+
+__global__ void k_clear_pending(skk_state *st)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    st->pending = 0;
+}
+
+// Cold/warm start: X[0] = x0 (already in place when warm) or 0, Y likewise,
+// sums 0, restart points = start point, xbar = x, AX[0] = AX[1] = 0 unless
+// warm (then filled by a following product).
+template <typename T>
+__global__ void k_init_vectors(Iter<T> I, int n, int m, int warm_x, int warm_y)
+{
+    const int stride = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        const T x = warm_x ? I.X[0][j] : T(0);
+        I.X[0][j] = x; I.X[1][j] = x; I.xsum[j] = T(0); I.xr[j] = x; I.xbar[j] = x; I.xavg[j] = x;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        const T y = warm_y ? I.Y[0][i] : T(0);
+        I.Y[0][i] = y; I.Y[1][i] = y; I.ysum[i] = T(0); I.yr[i] = y; I.yavg[i] = y;
+        I.AX[0][i] = T(0); I.AX[1][i] = T(0); I.AX[2][i] = T(0);
+    }
+}
+
+template <typename T> __global__ void k_copy(T *dst, const T *src, int n)
+{
+    const int stride = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) dst[j] = src[j];
+}
+
+// ------------------------------------------------------ adaptive PDHG step
+
+// y_cand = prox(y + sigma (2 A x - A x_prev)), sigma = eta * omega
+// (PdlpKernels.cu:377-389); adds a pending accepted y into the running sum.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dual_adaptive(Iter<T> I, Data<T> D,
+                                                          const skk_state *__restrict__ st, int m)
+{
+    const int cur = st->cur, pending = st->pending;
+    const T sigma = static_cast<T>(st->eta * st->omega);
+    const T *__restrict__ y = I.Y[cur];
+    const T *__restrict__ ax = I.AX[st->a0];
+    const T *__restrict__ axp = I.AX[st->a1];
+    T *__restrict__ yc = I.Y[cur ^ 1];
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        const T yi = y[i];
+        const T v = yi + sigma * (T(2) * ax[i] - axp[i]);
+        yc[i] = v - sigma * clampv(v / sigma, D.rl[i], D.ru[i]);
+        if (pending) I.ysum[i] += yi;
+    }
+}
+
+// A^T y_cand epilogue: x_cand = proj(x - tau (c + A^T y_cand)),
+// tau = eta / omega (PdlpKernels.cu:391-400); partial ||x_cand - x||^2.
+template <typename T, typename AT> struct EpiPrimalAdaptive {
+    static constexpr int kSlots = 1;
+    Iter<T> I;
+    Data<T> D;
+    const skk_state *st;
+    double *part;
+    int stride;
+    // per-launch values
+    const T *x;
+    T *xc;
+    T tau;
+    int pending;
+    __device__ void begin()
+    {
+        const int cur = st->cur;
+        x = I.X[cur]; xc = I.X[cur ^ 1];
+        tau = static_cast<T>(st->eta / st->omega);
+        pending = st->pending;
+    }
+    __device__ void row(int j, AT s, double *p) const
+    {
+        const T xj = x[j];
+        const T xn = clampv(xj - tau * (D.c[j] + static_cast<T>(s)), D.l[j], D.u[j]);
+        xc[j] = xn;
+        if (pending) I.xsum[j] += xj;
+        const double d = static_cast<double>(xn) - static_cast<double>(xj);
+        p[0] += d * d;
+    }
+    __device__ void end(double *p) const
+    {
+        double v[1] = {p[0]}, out[1];
+        block_reduce<1>(v, out);
+        if (threadIdx.x == 0) store_coherent(part + blockIdx.x, out[0]);
+    }
+};
+
+// A x_cand epilogue: AX[a2] = A x_cand; partial ||dy||^2 and dy'(A dx).  The
+// last workgroup takes the accept/reject decision and the next step size
+// (PdlpKernels.cu:454-493) and commits by rotating buffer indices.
+template <typename T, typename AT> struct EpiProbe {
+    static constexpr int kSlots = 2;
+    Iter<T> I;
+    skk_state *st;
+    double *part;       // region base
+    int stride;         // part_capacity
+    unsigned *counter;
+    int dx_blocks;      // workgroups of the A^T launch
+    const T *y, *yc, *ax;
+    T *axc;
+    __device__ void begin()
+    {
+        const int cur = st->cur;
+        y = I.Y[cur]; yc = I.Y[cur ^ 1]; ax = I.AX[st->a0]; axc = I.AX[st->a2];
+    }
+    __device__ void row(int i, AT s, double *p) const
+    {
+        const T a = static_cast<T>(s);
+        axc[i] = a;
+        const double dy = static_cast<double>(yc[i]) - static_cast<double>(y[i]);
+        p[0] += dy * dy;
+        p[1] += dy * (static_cast<double>(a) - static_cast<double>(ax[i]));
+    }
+    __device__ void end(double *p) const
+    {
+        double v[2] = {p[0], p[1]}, tot[2];
+        if (!partials_and_total<2>(v, part + (size_t)R_DY * stride, stride, counter, tot)) return;
+        const double dx2 = total_of(part + (size_t)R_DX * stride, dx_blocks);
+        if (threadIdx.x != 0) return;
+        const double dy2 = tot[0];
+        const double inter = fabs(tot[1]);
+        const double omega = st->omega, eta = st->eta;
+        const double movement = 0.5 * (omega * dx2 + dy2 / omega);
+        const double eta_bar = (inter > 0.0 && movement > 0.0) ? movement / inter : st->eta_max;
+        const bool accept = eta <= eta_bar;
+        const double k = st->updates + 1.0;
+        const double shrink = 1.0 - pow(k + 1.0, -0.3);
+        const double grow = 1.0 + pow(k + 1.0, -0.6);
+        double next = fmin(shrink * eta_bar, grow * eta);
+        if (!(next > 0.0) || !isfinite(next)) next = eta;
+        next = fmin(fmax(next, st->eta_min), st->eta_max);
+        st->eta = next;
+        st->eta_bar = eta_bar;
+        st->dx2 = dx2; st->dy2 = dy2; st->inter = inter;
+        st->updates = k;
+        st->accepted = accept ? 1 : 0;
+        st->pending = accept ? 1 : 0;
+        if (accept) {
+            st->accept_count += 1.0;
+            st->cur ^= 1;
+            const int a0 = st->a0, a1 = st->a1, a2 = st->a2;
+            st->a0 = a2; st->a1 = a0; st->a2 = a1;   // x_prev <- x, x <- cand
+        }
+    }
+};
+
+// ---------------------------------------------------------- fixed PDHG step
+
+// A xbar epilogue: y = prox(y + sigma A xbar), y_sum += y (PdlpKernels.cu:71-83).
+template <typename T, typename AT> struct EpiDualFixed {
+    static constexpr int kSlots = 0;
+    Iter<T> I;
+    Data<T> D;
+    const skk_state *st;
+    T *y;
+    T sigma;
+    __device__ void begin()
+    {
+        y = I.Y[st->cur];
+        sigma = static_cast<T>(st->eta_fixed * st->omega);
+    }
+    __device__ void row(int i, AT s, double *) const
+    {
+        const T v = y[i] + sigma * static_cast<T>(s);
+        const T yi = v - sigma * clampv(v / sigma, D.rl[i], D.ru[i]);
+        y[i] = yi;
+        I.ysum[i] += yi;
+    }
+    __device__ void end(double *) const {}
+};
+
+// A^T y epilogue: x+ = proj(x - tau (c + A^T y)), xbar = 2 x+ - x, x_sum += x+
+// (PdlpKernels.cu:85-97).
+template <typename T, typename AT> struct EpiPrimalFixed {
+    static constexpr int kSlots = 0;
+    Iter<T> I;
+    Data<T> D;
+    const skk_state *st;
+    T *x;
+    T tau;
+    __device__ void begin()
+    {
+        x = I.X[st->cur];
+        tau = static_cast<T>(st->eta_fixed / st->omega);
+    }
+    __device__ void row(int j, AT s, double *) const
+    {
+        const T xj = x[j];
+        const T xn = clampv(xj - tau * (D.c[j] + static_cast<T>(s)), D.l[j], D.u[j]);
+        I.xbar[j] = T(2) * xn - xj;
+        x[j] = xn;
+        I.xsum[j] += xn;
+    }
+    __device__ void end(double *) const {}
+};
+
+// ------------------------------------------------------------ evaluation
+
+template <typename T>
+__global__ void k_flush(Iter<T> I, const skk_state *__restrict__ st, int n, int m)
+{
+    if (!st->pending) return;
+    const int cur = st->cur;
+    const int stride = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) I.xsum[j] += I.X[cur][j];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) I.ysum[i] += I.Y[cur][i];
+}
+
+template <typename T>
+__global__ void k_average(Iter<T> I, const skk_state *__restrict__ st, int n, int m, int adaptive,
+                          double fixed_terms)
+{
+    double terms = fixed_terms;
+    if (adaptive) {
+        terms = st->accept_count - st->accepted_at_restart;
+        if (terms < 1.0) terms = 1.0;
+    }
+    const T scale = static_cast<T>(1.0 / terms);
+    const int stride = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) I.xavg[j] = scale * I.xsum[j];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) I.yavg[i] = scale * I.ysum[i];
+}
+
+// The eight sums of PdlpKernels.cu:101-209 in FP64: ||Ax - proj Ax||^2,
+// dual violation^2, c'x, dual objective, ||x||^2, ||y||^2, ||x - x_r||^2,
+// ||y - y_r||^2.  Column terms use the FP64 data and FP64 products.
+// `which` 0 evaluates X[cur], Y[cur]; 1 the running average.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_residuals_sel(Iter<T> I, int which,
+                                                          const double *__restrict__ ax,
+                                                          const double *__restrict__ aty,
+                                                          Data<double> D, int n, int m,
+                                                          double *part, int stride,
+                                                          unsigned *counter, double *out,
+                                                          const skk_state *__restrict__ st)
+{
+    const int cur = st->cur;
+    const T *__restrict__ x = which ? I.xavg : I.X[cur];
+    const T *__restrict__ y = which ? I.yavg : I.Y[cur];
+    const T *__restrict__ xr = I.xr;
+    const T *__restrict__ yr = I.yr;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int gs = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gs) {
+        const double xj = x[j], lo = D.l[j], hi = D.u[j], r = D.c[j] + aty[j];
+        const double dx = xj - static_cast<double>(xr[j]);
+        acc[2] += D.c[j] * xj;
+        acc[4] += xj * xj;
+        acc[6] += dx * dx;
+        double viol = 0.0, term = 0.0;
+        const bool lf = isfinite(lo), hf = isfinite(hi);
+        if (lf && hf) term = fmin(r * lo, r * hi);
+        else if (lf) { viol = fmax(0.0, -r); term = fmax(0.0, r) * lo; }
+        else if (hf) { viol = fmax(0.0, r); term = fmin(0.0, r) * hi; }
+        else viol = fabs(r);
+        acc[1] += viol * viol;
+        acc[3] += term;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gs) {
+        const double a = ax[i], rl = D.rl[i], ru = D.ru[i];
+        const double pr = a - fmin(fmax(a, rl), ru);
+        const double yi = y[i], dy = yi - static_cast<double>(yr[i]);
+        acc[0] += pr * pr;
+        acc[5] += yi * yi;
+        acc[7] += dy * dy;
+        double support = 0.0;
+        if (yi > 0.0) support = isfinite(ru) ? yi * ru : 0.0;
+        else if (yi < 0.0) support = isfinite(rl) ? yi * rl : 0.0;
+        acc[3] -= support;
+    }
+    double tot[8];
+    if (!partials_and_total<8>(acc, part, stride, counter, tot)) return;
+    if (threadIdx.x == 0)
+        for (int s = 0; s < 8; ++s) out[s] = tot[s];
+}
+
+// AX[a0] <- A x from the FP64 product of the evaluation.
+template <typename T>
+__global__ void k_to_t_sel(Iter<T> I, const double *__restrict__ src, int m,
+                           const skk_state *__restrict__ st)
+{
+    T *dst = I.AX[st->a0];
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) dst[i] = static_cast<T>(src[i]);
+}
+
+// Restart (GpuPdlp.cpp:453-498 minus the primal-weight update, which the
+// host applies through k_set_omega).
+template <typename T>
+__global__ void k_restart(Iter<T> I, skk_state *st, const double *__restrict__ ax_avg, int n, int m,
+                          int use_avg, int adaptive)
+{
+    const int cur = st->cur, a0 = st->a0, a1 = st->a1;
+    const int stride = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        T x = I.X[cur][j];
+        if (use_avg) { x = I.xavg[j]; I.X[cur][j] = x; }
+        I.xbar[j] = x;
+        I.xsum[j] = T(0);
+        I.xr[j] = x;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        T y = I.Y[cur][i];
+        if (use_avg) { y = I.yavg[i]; I.Y[cur][i] = y; }
+        I.ysum[i] = T(0);
+        I.yr[i] = y;
+        if (adaptive) {
+            const T a = use_avg ? static_cast<T>(ax_avg[i]) : I.AX[a0][i];
+            I.AX[a0][i] = a;
+            I.AX[a1][i] = a;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->accepted_at_restart = st->accept_count;
+}
+
+// FP32 -> FP64 copy of every iterate buffer (the state indices are shared).
+__global__ void k_promote(Iter<double> D, Iter<float> F, int n, int m)
+{
+    const int stride = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        D.X[0][j] = F.X[0][j]; D.X[1][j] = F.X[1][j];
+        D.xsum[j] = F.xsum[j]; D.xr[j] = F.xr[j]; D.xbar[j] = F.xbar[j]; D.xavg[j] = F.xavg[j];
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        D.Y[0][i] = F.Y[0][i]; D.Y[1][i] = F.Y[1][i];
+        D.AX[0][i] = F.AX[0][i]; D.AX[1][i] = F.AX[1][i]; D.AX[2][i] = F.AX[2][i];
+        D.ysum[i] = F.ysum[i]; D.yr[i] = F.yr[i]; D.yavg[i] = F.yavg[i];
+    }
+}
+
+template <typename T>
+__global__ void k_export(Iter<T> I, const skk_state *__restrict__ st, int n, int m, int use_avg,
+                         double *xo, double *yo)
+{
+    const int cur = st->cur;
+    const int stride = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride)
+        xo[j] = static_cast<double>(use_avg ? I.xavg[j] : I.X[cur][j]);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride)
+        yo[i] = static_cast<double>(use_avg ? I.yavg[i] : I.Y[cur][i]);
+}
+
+// ------------------------------------------------------- power iteration
+
+struct EpiNormSq {
+    static constexpr int kSlots = 1;
+    double *y, *part, *out;
+    int stride;
+    unsigned *counter;
+    __device__ void begin() {}
+    __device__ void row(int r, double s, double *p) const { y[r] = s; p[0] += s * s; }
+    __device__ void end(double *p) const
+    {
+        double v[1] = {p[0]}, tot[1];
+        if (!partials_and_total<1>(v, part, stride, counter, tot)) return;
+        if (threadIdx.x == 0) *out = tot[0];
+    }
+};
+
+__global__ void k_fill(double *v, double value, int n)
+{
+    const int stride = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) v[j] = value;
+}
+
+__global__ void k_normalize_from(double *v, const double *src, const double *norm_sq, int n)
+{
+    const double nrm = sqrt(*norm_sq);
+    const int stride = gridDim.x * blockDim.x;
+    if (!(nrm > 0.0)) {
+        for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) v[j] = src[j];
+        return;
+    }
+    const double inv = 1.0 / nrm;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) v[j] = src[j] * inv;
+}
+
+__global__ void k_axpy(int n, double alpha, const double *x, double *y)
+{
+    const int stride = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) y[j] = fma(alpha, x[j], y[j]);
+}
+
+// ---------------------------------------------------------- certificates
+
+__device__ __forceinline__ bool sk_inf(double b) { return b >= 1.0e30; }
+__device__ __forceinline__ bool sk_ninf(double b) { return b <= -1.0e30; }
+
+// rc = c + Qx + A^T y epilogue for the certificate.
+struct EpiRc {
+    static constexpr int kSlots = 0;
+    double *rc;
+    const double *c, *qx;
+    __device__ void begin() {}
+    __device__ void row(int j, double s, double *) const { rc[j] = c[j] + (qx ? qx[j] : 0.0) + s; }
+    __device__ void end(double *) const {}
+};
+
+// sk_verify (c/src/sk_core.c:394-510) residuals: sums pobj, dobj and
+// x'Qx/2; maxima pinf, dinf, comp.
+__global__ __launch_bounds__(kBlock) void k_verify(skk_verify_in V, int have_y)
+{
+    double s[3] = {0, 0, 0};   // c'x, dual objective terms, x'Qx
+    double mx[3] = {0, 0, 0};  // pinf, dinf, comp
+    bool bad_x = false;
+    const int gs = gridDim.x * blockDim.x;
+    const double bt = 1e-9;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < V.n; j += gs) {
+        const double xj = V.x[j], lo = V.l[j], hi = V.u[j];
+        s[0] += V.c[j] * xj;
+        if (V.qx) s[2] += xj * V.qx[j];
+        if (!isfinite(xj)) bad_x = true;
+        if (!sk_ninf(lo) && lo - xj > mx[0]) mx[0] = lo - xj;
+        if (!sk_inf(hi) && xj - hi > mx[0]) mx[0] = xj - hi;
+        if (!have_y) continue;
+        const double r = V.rc[j];
+        const bool atlo = !sk_ninf(lo) && fabs(xj - lo) <= bt;
+        const bool athi = !sk_inf(hi) && fabs(xj - hi) <= bt;
+        if (atlo && athi) {
+        } else if (atlo) { if (-r > mx[1]) mx[1] = -r; }
+        else if (athi) { if (r > mx[1]) mx[1] = r; }
+        else { if (fabs(r) > mx[1]) mx[1] = fabs(r); }
+        if (r > 0.0) {
+            if (sk_ninf(lo)) { if (r > mx[1]) mx[1] = r; } else s[1] += r * lo;
+        } else if (r < 0.0) {
+            if (sk_inf(hi)) { if (-r > mx[1]) mx[1] = -r; } else s[1] += r * hi;
+        }
+        if (r > 0.0 && !sk_ninf(lo)) { const double t = r * fabs(xj - lo); if (t > mx[2]) mx[2] = t; }
+        else if (r < 0.0 && !sk_inf(hi)) { const double t = -r * fabs(hi - xj); if (t > mx[2]) mx[2] = t; }
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V.m; i += gs) {
+        const double a = V.act[i], rl = V.rl[i], ru = V.ru[i];
+        if (!sk_ninf(rl) && rl - a > mx[0]) mx[0] = rl - a;
+        if (!sk_inf(ru) && a - ru > mx[0]) mx[0] = a - ru;
+        if (!have_y) continue;
+        const double yv = V.y[i];
+        const bool atlo = !sk_ninf(rl) && fabs(a - rl) <= bt;
+        const bool athi = !sk_inf(ru) && fabs(a - ru) <= bt;
+        if (atlo && athi) {
+        } else if (atlo) { if (yv > mx[1]) mx[1] = yv; }
+        else if (athi) { if (-yv > mx[1]) mx[1] = -yv; }
+        else { if (fabs(yv) > mx[1]) mx[1] = fabs(yv); }
+        if (yv < 0.0) {
+            if (sk_ninf(rl)) { if (-yv > mx[1]) mx[1] = -yv; } else s[1] -= yv * rl;
+            if (!sk_ninf(rl)) { const double t = -yv * fabs(a - rl); if (t > mx[2]) mx[2] = t; }
+        } else if (yv > 0.0) {
+            if (sk_inf(ru)) { if (yv > mx[1]) mx[1] = yv; } else s[1] -= yv * ru;
+            if (!sk_inf(ru)) { const double t = yv * fabs(ru - a); if (t > mx[2]) mx[2] = t; }
+        }
+    }
+    if (bad_x) mx[0] = INFINITY;
+    double ts[3], tm[3];
+    const int stride = V.grid;
+    const bool last_s = partials_and_total<3>(s, V.part, stride, V.counters + 0, ts);
+    // second reduction over the same grid with its own ticket
+    const bool last_m = partials_and_total<3, true>(mx, V.part + 3 * (size_t)stride, stride,
+                                                    V.counters + 1, tm);
+    if (last_s && threadIdx.x == 0) { V.out[0] = ts[0]; V.out[4] = ts[1]; V.out[5] = ts[2]; }
+    if (last_m && threadIdx.x == 0) { V.out[1] = tm[0]; V.out[2] = tm[1]; V.out[3] = tm[2]; }
+}
+
+// Relative-KKT sums in the original space (|b| >= 1e30 absent), with
+// r = c + Qx + A^T y already in V.rc.
+__global__ __launch_bounds__(kBlock) void k_relkkt(skk_verify_in V)
+{
+    double acc[5] = {0, 0, 0, 0, 0};
+    const int gs = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < V.n; j += gs) {
+        const double xj = V.x[j], r = V.rc[j];
+        const double lo = sk_ninf(V.l[j]) ? -INFINITY : V.l[j];
+        const double hi = sk_inf(V.u[j]) ? INFINITY : V.u[j];
+        const double q = V.qx ? xj * V.qx[j] : 0.0;
+        acc[2] += V.c[j] * xj;
+        acc[4] += q;
+        double viol = 0.0, term = 0.0;
+        const bool lf = isfinite(lo), hf = isfinite(hi);
+        if (lf && hf) term = fmin(r * lo, r * hi);
+        else if (lf) { viol = fmax(0.0, -r); term = fmax(0.0, r) * lo; }
+        else if (hf) { viol = fmax(0.0, r); term = fmin(0.0, r) * hi; }
+        else viol = fabs(r);
+        acc[1] += viol * viol;
+        acc[3] += term;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V.m; i += gs) {
+        const double a = V.act[i];
+        const double rl = sk_ninf(V.rl[i]) ? -INFINITY : V.rl[i];
+        const double ru = sk_inf(V.ru[i]) ? INFINITY : V.ru[i];
+        const double pr = a - fmin(fmax(a, rl), ru);
+        acc[0] += pr * pr;
+        const double yi = V.y[i];
+        double support = 0.0;
+        if (yi > 0.0) support = isfinite(ru) ? yi * ru : 0.0;
+        else if (yi < 0.0) support = isfinite(rl) ? yi * rl : 0.0;
+        acc[3] -= support;
+    }
+    double tot[5];
+    if (!partials_and_total<5>(acc, V.part, V.grid, V.counters + 2, tot)) return;
+    if (threadIdx.x == 0)
+        for (int s = 0; s < 5; ++s) V.out[s] = tot[s];
+}
+
+// ------------------------------------------------ plain PDHG (legacy ABI)
+//
+// cuda/src/sankhya_cuda_lp.cu:26-52 semantics: per-row sigma and per-column
+// tau (vectors or scalars), theta extrapolation, diagonal-Q implicit prox or
+// an explicit Q x gradient.  FP64 throughout, like the reference.
+
+__device__ __forceinline__ double project_box(double v, double lo, double hi)
+{
+    if (!isinf(lo) && v < lo) v = lo;
+    if (!isinf(hi) && v > hi) v = hi;
+    return v;
+}
+
+// A xbar epilogue: y <- trial - sigma proj(trial / sigma), in place.
+struct EpiLegacyDual {
+    static constexpr int kSlots = 0;
+    double *y;
+    const double *rl, *ru, *sigma_vec;
+    double sigma;
+    __device__ void begin() {}
+    __device__ void row(int i, double s, double *) const
+    {
+        const double sg = sigma_vec ? sigma_vec[i] : sigma;
+        const double trial = y[i] + sg * s;
+        y[i] = trial - sg * project_box(trial / sg, rl[i], ru[i]);
+    }
+    __device__ void end(double *) const {}
+};
+
+// A^T y epilogue: x+ = proj((x - tau (c + A^T y + Qx)) / (1 + tau q)),
+// xbar = x+ + theta (x+ - x), in place.
+struct EpiLegacyPrimal {
+    static constexpr int kSlots = 0;
+    double *x, *xbar;
+    const double *c, *l, *u, *tau_vec, *qg, *qd;
+    double tau, theta;
+    __device__ void begin() {}
+    __device__ void row(int j, double s, double *) const
+    {
+        const double t = tau_vec ? tau_vec[j] : tau;
+        const double curv = qd ? qd[j] : 0.0;
+        const double qgr = qg ? qg[j] : 0.0;
+        const double xj = x[j];
+        const double trial = (xj - t * (c[j] + s + qgr)) / (1.0 + t * curv);
+        const double next = project_box(trial, l[j], u[j]);
+        x[j] = next;
+        xbar[j] = next + theta * (next - xj);
+    }
+    __device__ void end(double *) const {}
+};
+
+// Checkpoint residuals of sankhya_cuda_lp.cu:258-367 on the device.
+// Sums: objective, count of non-finite entries.  Maxima: step, scale,
+// row violation, projected KKT, dual residual, complementarity.
+struct LegacyCheck {
+    int m, n, have_prev;
+    const double *x, *y, *act, *grad, *qg, *qd, *c, *l, *u, *rl, *ru;
+    double *prev;
+    double *part; int stride; unsigned *counters; double *out;
+};
+
+__device__ __forceinline__ bool has_lo(double v) { return isfinite(v) && v > -5.0e29; }
+__device__ __forceinline__ bool has_hi(double v) { return isfinite(v) && v < 5.0e29; }
+
+__global__ __launch_bounds__(kBlock) void k_legacy_check(LegacyCheck K)
+{
+    double s[2] = {0.0, 0.0};
+    double mx[6] = {0.0, 1.0, 0.0, 0.0, 0.0, 0.0};
+    const int gs = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < K.n; j += gs) {
+        const double xj = K.x[j], g = K.grad[j];
+        const double qgr = K.qd ? K.qd[j] * xj : (K.qg ? K.qg[j] : 0.0);
+        if (!isfinite(xj) || !isfinite(g) || (K.qg && !isfinite(K.qg[j]))) s[1] += 1.0;
+        s[0] += K.c[j] * xj;
+        if (K.qd) s[0] += 0.5 * K.qd[j] * xj * xj;
+        else if (K.qg) s[0] += 0.5 * xj * K.qg[j];
+        if (K.have_prev) mx[0] = fmax(mx[0], fabs(xj - K.prev[j]));
+        K.prev[j] = xj;
+        mx[1] = fmax(mx[1], fabs(xj));
+        const double r = K.c[j] + g + qgr;
+        const double lo = K.l[j], hi = K.u[j];
+        mx[3] = fmax(mx[3], fabs(xj - project_box(xj - r, lo, hi)));
+        const bool atlo = has_lo(lo) && fabs(xj - lo) <= 1e-9;
+        const bool athi = has_hi(hi) && fabs(xj - hi) <= 1e-9;
+        if (atlo && athi) {
+        } else if (atlo) mx[4] = fmax(mx[4], -r);
+        else if (athi) mx[4] = fmax(mx[4], r);
+        else mx[4] = fmax(mx[4], fabs(r));
+        if (r > 0.0 && has_lo(lo)) mx[5] = fmax(mx[5], r * fabs(xj - lo));
+        else if (r < 0.0 && has_hi(hi)) mx[5] = fmax(mx[5], -r * fabs(hi - xj));
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < K.m; i += gs) {
+        const double a = K.act[i], mu = K.y[i], rl = K.rl[i], ru = K.ru[i];
+        if (!isfinite(a) || !isfinite(mu)) s[1] += 1.0;
+        if (isfinite(rl)) mx[2] = fmax(mx[2], rl - a);
+        if (isfinite(ru)) mx[2] = fmax(mx[2], a - ru);
+        mx[3] = fmax(mx[3], fabs(a - project_box(a + mu, rl, ru)));
+        const bool atlo = has_lo(rl) && fabs(a - rl) <= 1e-9;
+        const bool athi = has_hi(ru) && fabs(a - ru) <= 1e-9;
+        if (atlo && athi) {
+        } else if (atlo) mx[4] = fmax(mx[4], mu);
+        else if (athi) mx[4] = fmax(mx[4], -mu);
+        else mx[4] = fmax(mx[4], fabs(mu));
+        if (mu < 0.0 && has_lo(rl)) mx[5] = fmax(mx[5], -mu * fabs(a - rl));
+        else if (mu > 0.0 && has_hi(ru)) mx[5] = fmax(mx[5], mu * fabs(ru - a));
+    }
+    double ts[2], tm[6];
+    const bool ls = partials_and_total<2>(s, K.part, K.stride, K.counters + 0, ts);
+    const bool lm = partials_and_total<6, true>(mx, K.part + 2 * (size_t)K.stride, K.stride,
+                                                K.counters + 1, tm);
+    if (ls && threadIdx.x == 0) { K.out[0] = ts[0]; K.out[1] = ts[1]; }
+    if (lm && threadIdx.x == 0)
+        for (int k = 0; k < 6; ++k) K.out[2 + k] = tm[k];
+}
+
+__global__ void k_legacy_init(double *x, double *xbar, const double *l, const double *u, int n,
+                              double *y, int m)
+{
+    const int stride = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        const double v = project_box(0.0, l[j], u[j]);
+        x[j] = v; xbar[j] = v;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) y[i] = 0.0;
+}
+
+int grid_for(int n, int cap)
+{
+    int g = (n + kBlock - 1) / kBlock;
+    if (g < 1) g = 1;
+    return g < cap ? g : cap;
+}
+
+
+template <typename T> constexpr int prec_index() { return std::is_same<T, float>::value ? SKK_F32 : SKK_F64; }
+
+template <typename T> int adaptive_iter(const skk_pdhg &P)
+{
+    const Iter<T> I(P.it[prec_index<T>()]);
+    const Data<T> D = data_of<T>(P);
+    int rc;
+    hipLaunchKernelGGL(k_dual_adaptive<T>, dim3(P.elem_grid), dim3(kBlock), 0, stream(), I, D,
+                       P.st, P.m);
+    if ((rc = launch_status())) return rc;
+    EpiPrimalAdaptive<T, T> ep{};
+    ep.I = I; ep.D = D; ep.st = P.st;
+    ep.part = P.part + (size_t)R_DX * P.part_capacity;
+    ep.stride = P.part_capacity;
+    // A^T y_cand: y_cand is Y[cur ^ 1], picked on the device
+    if ((rc = launch_spmv<T, T, T>(P.AT, vals_of<T>(P.AT), I.Y[0], I.Y[1], P.st, ep))) return rc;
+    EpiProbe<T, T> pr{};
+    pr.I = I; pr.st = P.st; pr.part = P.part; pr.stride = P.part_capacity;
+    pr.counter = P.counters + C_PROBE; pr.dx_blocks = P.AT.bins.total_blocks;
+    // A x_cand: x_cand is X[cur ^ 1]
+    return launch_spmv<T, T, T>(P.A, vals_of<T>(P.A), I.X[0], I.X[1], P.st, pr);
+}
+
+template <typename T> int fixed_iter(const skk_pdhg &P)
+{
+    const Iter<T> I(P.it[prec_index<T>()]);
+    const Data<T> D = data_of<T>(P);
+    int rc;
+    EpiDualFixed<T, T> ed{};
+    ed.I = I; ed.D = D; ed.st = P.st;
+    if ((rc = launch_spmv<T, T, T>(P.A, vals_of<T>(P.A), I.xbar, I.xbar, nullptr, ed))) return rc;
+    EpiPrimalFixed<T, T> ep{};
+    ep.I = I; ep.D = D; ep.st = P.st;
+    // the fixed path never rotates: cur stays 0
+    return launch_spmv<T, T, T>(P.AT, vals_of<T>(P.AT), I.Y[0], I.Y[0], nullptr, ep);
+}
+
+template <typename T> int flush(const skk_pdhg &P)
+{
+    const Iter<T> I(P.it[prec_index<T>()]);
+    hipLaunchKernelGGL(k_flush<T>, dim3(P.elem_grid), dim3(kBlock), 0, stream(), I, P.st, P.n, P.m);
+    int rc = launch_status();
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_clear_pending, dim3(1), dim3(64), 0, stream(), P.st);
+    return launch_status();
+}
+
+// Products in FP64 of the FP64 data with the current iterate (which 0) or
+// the running average (which 1), then the eight residual sums.
+template <typename T> int eval(const skk_pdhg &P, int which, int adaptive, double fixed_terms)
+{
+    const Iter<T> I(P.it[prec_index<T>()]);
+    int rc;
+    const T *x, *y, *x1 = nullptr, *y1 = nullptr;
+    const skk_state *sel = nullptr;
+    if (which) {
+        hipLaunchKernelGGL(k_average<T>, dim3(P.elem_grid), dim3(kBlock), 0, stream(), I, P.st, P.n,
+                           P.m, adaptive, fixed_terms);
+        if ((rc = launch_status())) return rc;
+        x = I.xavg; y = I.yavg;
+        x1 = x; y1 = y;
+    } else {
+        // current iterate X[cur]: with sel the kernel reads X[cur ^ 1] of
+        // (X[1], X[0]), i.e. X[cur]
+        x = I.X[1]; x1 = I.X[0]; y = I.Y[1]; y1 = I.Y[0];
+        sel = P.st;
+    }
+    EpiStore<double, double> sa{P.ax_d};
+    if ((rc = launch_spmv<double, T, double>(P.A, P.A.vd, x, x1, sel, sa))) return rc;
+    EpiStore<double, double> st{P.aty_d};
+    if ((rc = launch_spmv<double, T, double>(P.AT, P.AT.vd, y, y1, sel, st))) return rc;
+    const Data<double> D = data_of<double>(P);
+    hipLaunchKernelGGL(k_residuals_sel<T>, dim3(P.elem_grid), dim3(kBlock), 0, stream(), I, which,
+                       P.ax_d, P.aty_d, D, P.n, P.m, P.part + (size_t)R_EVAL * P.part_capacity,
+                       P.part_capacity, P.counters + C_EVAL, P.eval_out + 8 * which, P.st);
+    if ((rc = launch_status())) return rc;
+    if (!which && adaptive) {
+        // the live A x is refreshed from the FP64 product (GpuPdlp.cpp:367)
+        hipLaunchKernelGGL(k_to_t_sel<T>, dim3(P.elem_grid), dim3(kBlock), 0, stream(), I, P.ax_d,
+                           P.m, P.st);
+        rc = launch_status();
+    }
+    return rc;
+}
+
+template <typename T> int restart(const skk_pdhg &P, int use_avg, int adaptive)
+{
+    const Iter<T> I(P.it[prec_index<T>()]);
+    hipLaunchKernelGGL(k_restart<T>, dim3(P.elem_grid), dim3(kBlock), 0, stream(), I, P.st, P.ax_d,
+                       P.n, P.m, use_avg, adaptive);
+    return launch_status();
+}
+
+}  // namespace
+
+// ======================================================== extern "C" layer
+
+extern "C" {
+
+int skk_device_count(int *count)
+{
+    *count = 0;
+    return check(hipGetDeviceCount(count));
+}
+
+int skk_malloc(void **ptr, size_t bytes)
+{
+    *ptr = nullptr;
+    if (bytes == 0) bytes = 8;
+    return check(hipMalloc(ptr, bytes));
+}
+
+void skk_free(void *ptr)
+{
+    if (ptr) (void)hipFree(ptr);
+}
+
+int skk_h2d(void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return 0;
+    int rc = check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
+    return rc ? rc : check(hipStreamSynchronize(stream()));
+}
+
+int skk_d2h(void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return 0;
+    int rc = check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
+    return rc ? rc : check(hipStreamSynchronize(stream()));
+}
+
+int skk_d2h_async(void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return 0;
+    return check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
+}
+
+int skk_d2d(void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return 0;
+    return check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream()));
+}
+
+int skk_host_alloc(void **ptr, size_t bytes)
+{
+    *ptr = nullptr;
+    return check(hipHostMalloc(ptr, bytes ? bytes : 8, hipHostMallocDefault));
+}
+
+void skk_host_free(void *ptr)
+{
+    if (ptr) (void)hipHostFree(ptr);
+}
+
+int skk_sync(void) { return check(hipStreamSynchronize(stream())); }
+
+int skk_last_launch_error(void) { return g_last; }
+
+const char *skk_error_text(int code) { return hipGetErrorString(static_cast<hipError_t>(code)); }
+
+void *skk_stream(void) { return static_cast<void *>(stream()); }
+
+int skk_graph_begin(void)
+{
+    return check(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+}
+
+int skk_graph_end(void **exec)
+{
+    hipGraph_t g = nullptr;
+    *exec = nullptr;
+    int rc = check(hipStreamEndCapture(stream(), &g));
+    if (rc) return rc;
+    hipGraphExec_t e = nullptr;
+    rc = check(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    if (rc) return rc;
+    *exec = static_cast<void *>(e);
+    return 0;
+}
+
+int skk_graph_launch(void *exec)
+{
+    return check(hipGraphLaunch(static_cast<hipGraphExec_t>(exec), stream()));
+}
+
+void skk_graph_destroy(void *exec)
+{
+    if (exec) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(exec));
+}
+
+int skk_mark(int slot)
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev < 0 || dev >= kMaxDevices || slot < 0 || slot >= 8) return -1;
+    if (!g_marks[dev][slot] && check(hipEventCreate(&g_marks[dev][slot]))) return g_last;
+    return check(hipEventRecord(g_marks[dev][slot], stream()));
+}
+
+double skk_elapsed_ms(int a, int b)
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    float ms = 0.0f;
+    if (dev < 0 || dev >= kMaxDevices || !g_marks[dev][a] || !g_marks[dev][b]) return -1.0;
+    if (hipEventSynchronize(g_marks[dev][b]) != hipSuccess) return -1.0;
+    if (hipEventElapsedTime(&ms, g_marks[dev][a], g_marks[dev][b]) != hipSuccess) return -1.0;
+    return static_cast<double>(ms);
+}
+
+int skk_spmv_f64(const skk_csr *A, const double *x, double *y)
+{
+    EpiStore<double, double> e{y};
+    return launch_spmv<double, double, double>(*A, A->vd, x, x, nullptr, e);
+}
+
+int skk_axpy_f64(int n, double alpha, const double *x, double *y)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_axpy, dim3(grid_for(n, 2048)), dim3(kBlock), 0, stream(), n, alpha, x, y);
+    return launch_status();
+}
+
+int skk_pdhg_init(const skk_pdhg *P, int prec, double eta0, double omega, double eta_min,
+                  double eta_max, int warm_x, int warm_y)
+{
+    hipLaunchKernelGGL(k_init_state, dim3(1), dim3(64), 0, stream(), P->st, eta0, omega, eta_min,
+                       eta_max);
+    int rc = launch_status();
+    if (rc) return rc;
+    if (prec == SKK_F32) {
+        Iter<float> I(P->it[SKK_F32]);
+        hipLaunchKernelGGL(k_init_vectors<float>, dim3(P->elem_grid), dim3(kBlock), 0, stream(), I,
+                           P->n, P->m, warm_x, warm_y);
+    } else {
+        Iter<double> I(P->it[SKK_F64]);
+        hipLaunchKernelGGL(k_init_vectors<double>, dim3(P->elem_grid), dim3(kBlock), 0, stream(), I,
+                           P->n, P->m, warm_x, warm_y);
+    }
+    if ((rc = launch_status())) return rc;
+    if (warm_x) {
+        // A x0 into AX[0] and AX[1] (extrapolation starts as the identity)
+        if (prec == SKK_F32) {
+            Iter<float> I(P->it[SKK_F32]);
+            EpiStore<float, float> e{I.AX[0]};
+            if ((rc = launch_spmv<float, float, float>(P->A, P->A.vf, I.X[0], I.X[0], nullptr, e))) return rc;
+            hipLaunchKernelGGL(k_copy<float>, dim3(P->elem_grid), dim3(kBlock), 0, stream(), I.AX[1], I.AX[0], P->m);
+        } else {
+            Iter<double> I(P->it[SKK_F64]);
+            EpiStore<double, double> e{I.AX[0]};
+            if ((rc = launch_spmv<double, double, double>(P->A, P->A.vd, I.X[0], I.X[0], nullptr, e))) return rc;
+            hipLaunchKernelGGL(k_copy<double>, dim3(P->elem_grid), dim3(kBlock), 0, stream(), I.AX[1], I.AX[0], P->m);
+        }
+        rc = launch_status();
+    }
+    return rc;
+}
+
+int skk_pdhg_adaptive_iter(const skk_pdhg *P, int prec)
+{
+    return prec == SKK_F32 ? adaptive_iter<float>(*P) : adaptive_iter<double>(*P);
+}
+
+int skk_pdhg_fixed_iter(const skk_pdhg *P, int prec)
+{
+    return prec == SKK_F32 ? fixed_iter<float>(*P) : fixed_iter<double>(*P);
+}
+
+int skk_pdhg_flush(const skk_pdhg *P, int prec)
+{
+    return prec == SKK_F32 ? flush<float>(*P) : flush<double>(*P);
+}
+
+int skk_pdhg_eval(const skk_pdhg *P, int prec, int which, int adaptive, double fixed_terms)
+{
+    return prec == SKK_F32 ? eval<float>(*P, which, adaptive, fixed_terms)
+                           : eval<double>(*P, which, adaptive, fixed_terms);
+}
+
+int skk_pdhg_restart(const skk_pdhg *P, int prec, int use_average, int adaptive)
+{
+    return prec == SKK_F32 ? restart<float>(*P, use_average, adaptive)
+                           : restart<double>(*P, use_average, adaptive);
+}
+
+int skk_pdhg_set_omega(const skk_pdhg *P, double omega)
+{
+    hipLaunchKernelGGL(k_set_omega, dim3(1), dim3(64), 0, stream(), P->st, omega);
+    return launch_status();
+}
+
+int skk_pdhg_set_eta(const skk_pdhg *P, double eta)
+{
+    hipLaunchKernelGGL(k_set_eta, dim3(1), dim3(64), 0, stream(), P->st, eta);
+    return launch_status();
+}
+
+int skk_pdhg_promote(const skk_pdhg *P)
+{
+    Iter<double> D(P->it[SKK_F64]);
+    Iter<float> F(P->it[SKK_F32]);
+    hipLaunchKernelGGL(k_promote, dim3(P->elem_grid), dim3(kBlock), 0, stream(), D, F, P->n, P->m);
+    return launch_status();
+}
+
+int skk_pdhg_export(const skk_pdhg *P, int prec, int use_average, double *x_out, double *y_out)
+{
+    if (prec == SKK_F32) {
+        Iter<float> I(P->it[SKK_F32]);
+        hipLaunchKernelGGL(k_export<float>, dim3(P->elem_grid), dim3(kBlock), 0, stream(), I, P->st,
+                           P->n, P->m, use_average, x_out, y_out);
+    } else {
+        Iter<double> I(P->it[SKK_F64]);
+        hipLaunchKernelGGL(k_export<double>, dim3(P->elem_grid), dim3(kBlock), 0, stream(), I, P->st,
+                           P->n, P->m, use_average, x_out, y_out);
+    }
+    return launch_status();
+}
+
+int skk_power_norm(const skk_pdhg *P, double *v, double *w, double *v2, int iters, double *out)
+{
+    int rc;
+    const int n = P->n;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, P->elem_grid)), dim3(kBlock), 0, stream(), v,
+                       1.0 * (1.0 / sqrt(static_cast<double>(n > 0 ? n : 1))), n);
+    if ((rc = launch_status())) return rc;
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream(), out, 0.0, 1);
+    if ((rc = launch_status())) return rc;
+    for (int k = 0; k < iters; ++k) {
+        EpiStore<double, double> e{w};
+        if ((rc = launch_spmv<double, double, double>(P->A, P->A.vd, v, v, nullptr, e))) return rc;
+        EpiNormSq ns{};
+        ns.y = v2; ns.part = P->part + (size_t)R_POWER * P->part_capacity; ns.out = out;
+        ns.stride = P->part_capacity; ns.counter = P->counters + C_POWER;
+        if ((rc = launch_spmv<double, double, double>(P->AT, P->AT.vd, w, w, nullptr, ns))) return rc;
+        hipLaunchKernelGGL(k_normalize_from, dim3(grid_for(n, P->elem_grid)), dim3(kBlock), 0,
+                           stream(), v, v2, out, n);
+        if ((rc = launch_status())) return rc;
+    }
+    return 0;
+}
+
+int skk_verify(const skk_verify_in *V)
+{
+    int rc;
+    EpiStore<double, double> ea{V->act};
+    if ((rc = launch_spmv<double, double, double>(V->A, V->A.vd, V->x, V->x, nullptr, ea))) return rc;
+    if (V->Q) {
+        EpiStore<double, double> eq{V->qx};
+        if ((rc = launch_spmv<double, double, double>(*V->Q, V->Q->vd, V->x, V->x, nullptr, eq))) return rc;
+    }
+    if (V->y) {
+        EpiRc er{V->rc, V->c, V->Q ? V->qx : nullptr};
+        if (V->AT.rows > 0) {
+            if ((rc = launch_spmv<double, double, double>(V->AT, V->AT.vd, V->y, V->y, nullptr, er))) return rc;
+        }
+    }
+    skk_verify_in v = *V;
+    if (!V->Q) v.qx = nullptr;
+    hipLaunchKernelGGL(k_verify, dim3(V->grid), dim3(kBlock), 0, stream(), v, V->y ? 1 : 0);
+    return launch_status();
+}
+
+int skk_relative_kkt(const skk_verify_in *V)
+{
+    skk_verify_in v = *V;
+    if (!V->Q) v.qx = nullptr;
+    hipLaunchKernelGGL(k_relkkt, dim3(V->grid), dim3(kBlock), 0, stream(), v);
+    return launch_status();
+}
+
+
+int skk_legacy_init(double *x, double *xbar, const double *l, const double *u, int n, double *y, int m)
+{
+    const int g = grid_for(n > m ? n : m, 2048);
+    hipLaunchKernelGGL(k_legacy_init, dim3(g), dim3(kBlock), 0, stream(), x, xbar, l, u, n, y, m);
+    return launch_status();
+}
+
+int skk_legacy_iter(const skk_legacy *L)
+{
+    int rc;
+    EpiLegacyDual ed{L->y, L->rl, L->ru, L->sigma_vec, L->sigma};
+    if ((rc = launch_spmv<double, double, double>(*L->A, L->A->vd, L->xbar, L->xbar, nullptr, ed))) return rc;
+    if (L->Q) {
+        EpiStore<double, double> eq{L->qg};
+        if ((rc = launch_spmv<double, double, double>(*L->Q, L->Q->vd, L->x, L->x, nullptr, eq))) return rc;
+    }
+    EpiLegacyPrimal ep{L->x, L->xbar, L->c, L->l, L->u, L->tau_vec, L->Q ? L->qg : nullptr, L->qd,
+                       L->tau, L->theta};
+    return launch_spmv<double, double, double>(*L->AT, L->AT->vd, L->y, L->y, nullptr, ep);
+}
+
+int skk_legacy_check(const skk_legacy *L, int have_prev)
+{
+    int rc;
+    EpiStore<double, double> ea{L->act};
+    if ((rc = launch_spmv<double, double, double>(*L->A, L->A->vd, L->x, L->x, nullptr, ea))) return rc;
+    EpiStore<double, double> eg{L->grad};
+    if ((rc = launch_spmv<double, double, double>(*L->AT, L->AT->vd, L->y, L->y, nullptr, eg))) return rc;
+    if (L->Q) {
+        EpiStore<double, double> eq{L->qg};
+        if ((rc = launch_spmv<double, double, double>(*L->Q, L->Q->vd, L->x, L->x, nullptr, eq))) return rc;
+    }
+    LegacyCheck K{L->m, L->n, have_prev, L->x, L->y, L->act, L->grad, L->Q ? L->qg : nullptr, L->qd,
+                  L->c, L->l, L->u, L->rl, L->ru, L->prev, L->part, L->grid, L->counters, L->out};
+    hipLaunchKernelGGL(k_legacy_check, dim3(L->grid), dim3(kBlock), 0, stream(), K);
+    return launch_status();
+}
+
+}  // extern "C"

@@ -1,0 +1,624 @@
+/* Restarted-average PDHG (PDLP) control loop on the host, C11.
+ *
+ * Algorithm spec: src/cuda/GpuPdlp.cpp:252-534 (windows of restart_period
+ * iterations with no host synchronisation inside, evaluation of the current
+ * and the averaged iterate, restart criteria 0.2 / 0.8 / 0.36, primal-weight
+ * log-smoothing) and src/lp/LpSolver.cpp:62-99 (Ruiz then Pock-Chambolle
+ * scaling, unscaling x = C x').  Additions for MI355X: the problem is
+ * permuted so that every row-length bin of A and of A^T is a contiguous row
+ * range; FP32 iterates with FP64 checks and an FP64 tail; optional hipGraph
+ * capture of a whole restart window; warm start. */
+#define _POSIX_C_SOURCE 200809L
+#include "skg_internal.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+skg_tuning g_skg_tuning = {4, 512};
+
+void sk_gpu_pdlp_params_default(sk_gpu_pdlp_params *p)
+{
+    memset(p, 0, sizeof(*p));
+    p->eps_optimal = 1e-6;
+    p->max_iterations = 500000;
+    p->restart_period = 64;
+    p->time_limit_seconds = 0.0;
+    p->power_iterations = 30;
+    p->adaptive_step = 1;
+    p->eta_min_factor = 1e-6;
+    p->eta_max_factor = 1e4;
+    p->precision = SK_GPU_PREC_FP64;
+    p->tail_switch = 1e-5;
+    p->tail_stall_windows = 8;
+    p->scaling = 1;
+    p->ruiz_iterations = 100;
+    p->ruiz_tol = 1e-2;
+    p->use_graph = 0;
+    p->verbosity = 0;
+}
+
+static double to_ieee(double b)
+{
+    if (b >= SK_INFINITY) return INFINITY;
+    if (b <= -SK_INFINITY) return -INFINITY;
+    return b;
+}
+
+static double finite_norm(const double *v, int n)       /* GpuPdlp.cpp:84-90 */
+{
+    double acc = 0.0;
+    int k;
+    for (k = 0; k < n; ++k) if (isfinite(v[k])) acc += v[k] * v[k];
+    return sqrt(acc);
+}
+
+/* ---------------------------------------------------------------- scaling */
+
+/* Ruiz equilibration, src/lp/Scaling.cpp:11-92: simultaneous row/column
+ * division by sqrt of the current inf-norms until every nonzero norm is
+ * within tol of 1; identity if that does not happen within max_iters. */
+static void ruiz(const hcsr *a, const hcsr *at, int max_iters, double tol, double *R, double *C)
+{
+    const int m = a->rows, n = a->cols;
+    int i, j, k, iter, converged = 0;
+    double *rn, *cn;
+    for (i = 0; i < m; ++i) R[i] = 1.0;
+    for (j = 0; j < n; ++j) C[j] = 1.0;
+    if (a->nnz == 0 || m == 0 || n == 0) return;
+    rn = (double *)malloc(sizeof(double) * (size_t)m);
+    cn = (double *)malloc(sizeof(double) * (size_t)n);
+    if (!rn || !cn) { free(rn); free(cn); return; }
+    for (iter = 0; iter < max_iters; ++iter) {
+        double dev = 0.0;
+        for (i = 0; i < m; ++i) {
+            double best = 0.0;
+            for (k = a->p[i]; k < a->p[i + 1]; ++k) {
+                const double v = fabs(a->v[k]) * C[a->j[k]];
+                if (v > best) best = v;
+            }
+            rn[i] = R[i] * best;
+        }
+        for (j = 0; j < n; ++j) {
+            double best = 0.0;
+            for (k = at->p[j]; k < at->p[j + 1]; ++k) {
+                const double v = fabs(at->v[k]) * R[at->j[k]];
+                if (v > best) best = v;
+            }
+            cn[j] = C[j] * best;
+        }
+        for (i = 0; i < m; ++i) if (rn[i] > 0.0 && fabs(rn[i] - 1.0) > dev) dev = fabs(rn[i] - 1.0);
+        for (j = 0; j < n; ++j) if (cn[j] > 0.0 && fabs(cn[j] - 1.0) > dev) dev = fabs(cn[j] - 1.0);
+        if (dev < tol) { converged = 1; break; }
+        for (i = 0; i < m; ++i) if (rn[i] > 0.0) R[i] /= sqrt(rn[i]);
+        for (j = 0; j < n; ++j) if (cn[j] > 0.0) C[j] /= sqrt(cn[j]);
+    }
+    if (!converged) {
+        for (i = 0; i < m; ++i) R[i] = 1.0;
+        for (j = 0; j < n; ++j) C[j] = 1.0;
+    }
+    free(rn); free(cn);
+}
+
+/* Pock-Chambolle with alpha = 1 on the matrix a scaled by (R0, C0),
+ * src/lp/Scaling.cpp:111-145: R_i = 1/sqrt(sum_j |a'_ij|),
+ * C_j = 1/sqrt(sum_i |a'_ij|); empty rows/columns keep 1.  Multiplies the
+ * result into R0, C0 (compose_scaling, Scaling.cpp:147-152). */
+static void pock_chambolle_into(const hcsr *a, double *R, double *C)
+{
+    const int m = a->rows, n = a->cols;
+    int i, j, k;
+    double *ra = (double *)calloc((size_t)m + 1, sizeof(double));
+    double *ca = (double *)calloc((size_t)n + 1, sizeof(double));
+    if (!ra || !ca) { free(ra); free(ca); return; }
+    for (i = 0; i < m; ++i)
+        for (k = a->p[i]; k < a->p[i + 1]; ++k) {
+            const double v = fabs(a->v[k] * R[i] * C[a->j[k]]);
+            if (v == 0.0) continue;
+            ra[i] += pow(v, 1.0);
+            ca[a->j[k]] += pow(v, 1.0);
+        }
+    for (i = 0; i < m; ++i) if (ra[i] > 0.0) R[i] *= 1.0 / sqrt(ra[i]);
+    for (j = 0; j < n; ++j) if (ca[j] > 0.0) C[j] *= 1.0 / sqrt(ca[j]);
+    free(ra); free(ca);
+}
+
+/* ----------------------------------------------------------------- engine */
+
+typedef struct engine {
+    int m, n, mixed;
+    dcsr A, AT;
+    int *perm_r, *perm_c;        /* slot -> original index */
+    double *R, *C;               /* scale factors, original indexing */
+    double cost_norm, bound_norm;
+    void *dev[64];               /* every device allocation, freed together */
+    int ndev;
+    skk_pdhg P;
+    double *pv, *pw, *pv2, *pnorm;
+    double *h_eval;              /* pinned: 16 doubles + state */
+    skk_state *h_state;
+    void *graph[2];
+    int graph_period[2];
+    long long launches;
+} engine;
+
+static int dalloc(engine *E, void **p, size_t bytes)
+{
+    int rc = skk_malloc(p, bytes);
+    if (rc) return skg_hip_fail("hipMalloc", rc);
+    if (E->ndev < 64) E->dev[E->ndev++] = *p;
+    return 0;
+}
+
+static void engine_free(engine *E)
+{
+    int k;
+    for (k = 0; k < 2; ++k) skk_graph_destroy(E->graph[k]);
+    for (k = 0; k < E->ndev; ++k) skk_free(E->dev[k]);
+    dcsr_free(&E->A); dcsr_free(&E->AT);
+    skk_host_free(E->h_eval);
+    free(E->perm_r); free(E->perm_c); free(E->R); free(E->C);
+    memset(E, 0, sizeof(*E));
+}
+
+static int upload_vec_d(engine *E, double **dst, const double *src, int n)
+{
+    int rc;
+    if ((rc = dalloc(E, (void **)dst, sizeof(double) * (size_t)(n ? n : 1)))) return rc;
+    if ((rc = skk_h2d(*dst, src, sizeof(double) * (size_t)n))) return skg_hip_fail("upload", rc);
+    return 0;
+}
+
+static int upload_vec_f(engine *E, float **dst, const double *src, int n)
+{
+    int rc, k;
+    float *tmp = (float *)malloc(sizeof(float) * (size_t)(n ? n : 1));
+    if (!tmp) return skg_fail("out of host memory");
+    for (k = 0; k < n; ++k) tmp[k] = (float)src[k];
+    if ((rc = dalloc(E, (void **)dst, sizeof(float) * (size_t)(n ? n : 1)))) { free(tmp); return rc; }
+    rc = skk_h2d(*dst, tmp, sizeof(float) * (size_t)n);
+    free(tmp);
+    return rc ? skg_hip_fail("upload", rc) : 0;
+}
+
+static int alloc_iter(engine *E, skk_iter *it, size_t es)
+{
+    const size_t n = (size_t)(E->n ? E->n : 1), m = (size_t)(E->m ? E->m : 1);
+    int k, rc;
+    for (k = 0; k < 2; ++k) {
+        if ((rc = dalloc(E, &it->X[k], es * n))) return rc;
+        if ((rc = dalloc(E, &it->Y[k], es * m))) return rc;
+    }
+    for (k = 0; k < 3; ++k) if ((rc = dalloc(E, &it->AX[k], es * m))) return rc;
+    if ((rc = dalloc(E, &it->xsum, es * n)) || (rc = dalloc(E, &it->ysum, es * m)) ||
+        (rc = dalloc(E, &it->xr, es * n)) || (rc = dalloc(E, &it->yr, es * m)) ||
+        (rc = dalloc(E, &it->xavg, es * n)) || (rc = dalloc(E, &it->yavg, es * m)) ||
+        (rc = dalloc(E, &it->xbar, es * n)))
+        return rc;
+    return 0;
+}
+
+/* Build the scaled, permuted, binned problem on the device. */
+static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, const double *v,
+                        const double *c, const double *rl, const double *ru, const double *l,
+                        const double *u, const sk_gpu_pdlp_params *P)
+{
+    hcsr A0, AT0, Ap, ATp;
+    int *inv_r = NULL, *inv_c = NULL, i, j, k, q, rc = -1;
+    double *tc = NULL, *tl = NULL, *tu = NULL, *trl = NULL, *tru = NULL;
+    skk_bins bA, bAT;
+    memset(&Ap, 0, sizeof(Ap)); memset(&ATp, 0, sizeof(ATp)); memset(&AT0, 0, sizeof(AT0));
+    E->m = m; E->n = n;
+    A0.rows = m; A0.cols = n; A0.nnz = rp[m];
+    A0.p = (int *)rp; A0.j = (int *)ci; A0.v = (double *)v;
+    if (hcsr_transpose(&A0, &AT0)) return skg_fail("out of host memory");
+    E->R = (double *)malloc(sizeof(double) * (size_t)(m + 1));
+    E->C = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    E->perm_r = (int *)malloc(sizeof(int) * (size_t)(m + 1));
+    E->perm_c = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    inv_r = (int *)malloc(sizeof(int) * (size_t)(m + 1));
+    inv_c = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    tc = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    tl = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    tu = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    trl = (double *)malloc(sizeof(double) * (size_t)(m + 1));
+    tru = (double *)malloc(sizeof(double) * (size_t)(m + 1));
+    if (!E->R || !E->C || !E->perm_r || !E->perm_c || !inv_r || !inv_c || !tc || !tl || !tu || !trl || !tru) {
+        skg_fail("out of host memory"); goto done;
+    }
+    if (P->scaling) {
+        ruiz(&A0, &AT0, P->ruiz_iterations, P->ruiz_tol, E->R, E->C);
+        pock_chambolle_into(&A0, E->R, E->C);
+    } else {
+        for (i = 0; i < m; ++i) E->R[i] = 1.0;
+        for (j = 0; j < n; ++j) E->C[j] = 1.0;
+    }
+    /* bin classes decide the permutations */
+    if (skg_plan_bins(A0.p, m, &g_skg_tuning, E->perm_r, &bA) ||
+        skg_plan_bins(AT0.p, n, &g_skg_tuning, E->perm_c, &bAT)) { skg_fail("out of host memory"); goto done; }
+    for (k = 0; k < m; ++k) inv_r[E->perm_r[k]] = k;
+    for (k = 0; k < n; ++k) inv_c[E->perm_c[k]] = k;
+    if (hcsr_alloc(&Ap, m, n, A0.nnz) || hcsr_alloc(&ATp, n, m, A0.nnz)) { skg_fail("out of host memory"); goto done; }
+    q = 0;
+    for (k = 0; k < m; ++k) {
+        const int r = E->perm_r[k];
+        Ap.p[k] = q;
+        for (i = A0.p[r]; i < A0.p[r + 1]; ++i, ++q) {
+            Ap.j[q] = inv_c[A0.j[i]];
+            Ap.v[q] = A0.v[i] * E->R[r] * E->C[A0.j[i]];
+        }
+    }
+    Ap.p[m] = q;
+    q = 0;
+    for (k = 0; k < n; ++k) {
+        const int col = E->perm_c[k];
+        ATp.p[k] = q;
+        for (i = AT0.p[col]; i < AT0.p[col + 1]; ++i, ++q) {
+            ATp.j[q] = inv_r[AT0.j[i]];
+            ATp.v[q] = AT0.v[i] * E->R[AT0.j[i]] * E->C[col];
+        }
+    }
+    ATp.p[n] = q;
+    /* rows are now sorted by class: the bins are contiguous slot ranges */
+    if (skg_plan_bins(Ap.p, m, &g_skg_tuning, NULL, &bA) ||
+        skg_plan_bins(ATp.p, n, &g_skg_tuning, NULL, &bAT)) { skg_fail("out of host memory"); goto done; }
+    if ((rc = dcsr_upload(&E->A, &Ap, &bA, NULL, E->mixed)) ||
+        (rc = dcsr_upload(&E->AT, &ATp, &bAT, NULL, E->mixed))) goto done;
+    for (k = 0; k < n; ++k) {
+        const int col = E->perm_c[k];
+        tc[k] = c[col] * E->C[col];
+        tl[k] = to_ieee(l[col]) / E->C[col];
+        tu[k] = to_ieee(u[col]) / E->C[col];
+    }
+    for (k = 0; k < m; ++k) {
+        const int r = E->perm_r[k];
+        trl[k] = E->R[r] * to_ieee(rl[r]);
+        tru[k] = E->R[r] * to_ieee(ru[r]);
+    }
+    E->cost_norm = finite_norm(tc, n);
+    E->bound_norm = sqrt(finite_norm(trl, m) * finite_norm(trl, m) + finite_norm(tru, m) * finite_norm(tru, m));
+    {
+        skk_pdhg *Q = &E->P;
+        double *cd, *ld, *ud, *rld, *rud;
+        float *cf = NULL, *lf = NULL, *uf = NULL, *rlf = NULL, *ruf = NULL;
+        int cap, eg;
+        memset(Q, 0, sizeof(*Q));
+        Q->m = m; Q->n = n;
+        Q->A = E->A.k; Q->AT = E->AT.k;
+        if ((rc = upload_vec_d(E, &cd, tc, n)) || (rc = upload_vec_d(E, &ld, tl, n)) ||
+            (rc = upload_vec_d(E, &ud, tu, n)) || (rc = upload_vec_d(E, &rld, trl, m)) ||
+            (rc = upload_vec_d(E, &rud, tru, m))) goto done;
+        if (E->mixed) {
+            if ((rc = upload_vec_f(E, &cf, tc, n)) || (rc = upload_vec_f(E, &lf, tl, n)) ||
+                (rc = upload_vec_f(E, &uf, tu, n)) || (rc = upload_vec_f(E, &rlf, trl, m)) ||
+                (rc = upload_vec_f(E, &ruf, tru, m))) goto done;
+        }
+        Q->cd = cd; Q->ld = ld; Q->ud = ud; Q->rld = rld; Q->rud = rud;
+        Q->cf = cf; Q->lf = lf; Q->uf = uf; Q->rlf = rlf; Q->ruf = ruf;
+        if ((rc = alloc_iter(E, &Q->it[SKK_F64], sizeof(double)))) goto done;
+        if (E->mixed && (rc = alloc_iter(E, &Q->it[SKK_F32], sizeof(float)))) goto done;
+        eg = ((m > n ? m : n) + SKK_BLOCK - 1) / SKK_BLOCK;
+        if (eg < 1) eg = 1;
+        if (eg > 2048) eg = 2048;
+        Q->elem_grid = eg;
+        cap = eg;
+        if (bA.total_blocks > cap) cap = bA.total_blocks;
+        if (bAT.total_blocks > cap) cap = bAT.total_blocks;
+        Q->part_capacity = cap;
+        if ((rc = dalloc(E, (void **)&Q->ax_d, sizeof(double) * (size_t)(m ? m : 1))) ||
+            (rc = dalloc(E, (void **)&Q->aty_d, sizeof(double) * (size_t)(n ? n : 1))) ||
+            (rc = dalloc(E, (void **)&Q->st, sizeof(skk_state))) ||
+            (rc = dalloc(E, (void **)&Q->part, sizeof(double) * 16 * (size_t)cap)) ||
+            (rc = dalloc(E, (void **)&Q->counters, sizeof(unsigned) * 16)) ||
+            (rc = dalloc(E, (void **)&Q->eval_out, sizeof(double) * 16)) ||
+            (rc = dalloc(E, (void **)&E->pv, sizeof(double) * (size_t)(n ? n : 1))) ||
+            (rc = dalloc(E, (void **)&E->pw, sizeof(double) * (size_t)(m ? m : 1))) ||
+            (rc = dalloc(E, (void **)&E->pv2, sizeof(double) * (size_t)(n ? n : 1))) ||
+            (rc = dalloc(E, (void **)&E->pnorm, sizeof(double))))
+            goto done;
+        {
+            unsigned zeros[16] = {0};
+            if ((rc = skk_h2d(Q->counters, zeros, sizeof(zeros)))) { rc = skg_hip_fail("upload", rc); goto done; }
+        }
+        if ((rc = skk_host_alloc((void **)&E->h_eval, sizeof(double) * 16 + sizeof(skk_state)))) {
+            rc = skg_hip_fail("hipHostMalloc", rc); goto done;
+        }
+        E->h_state = (skk_state *)(E->h_eval + 16);
+    }
+    rc = 0;
+done:
+    hcsr_free(&AT0); hcsr_free(&Ap); hcsr_free(&ATp);
+    free(inv_r); free(inv_c); free(tc); free(tl); free(tu); free(trl); free(tru);
+    return rc;
+}
+
+/* ------------------------------------------------------------------ loop */
+
+typedef struct kkt {
+    double primal, dual, gap, pobj, dobj, pmove, dmove;
+} kkt;
+
+static double kkt_worst(const kkt *k)
+{
+    const double a = k->primal > k->dual ? k->primal : k->dual;
+    return a > k->gap ? a : k->gap;
+}
+
+/* GpuPdlp.cpp:226-249 */
+static kkt kkt_from(const double *s, double cost_norm, double bound_norm)
+{
+    kkt k;
+    k.primal = sqrt(fmax(0.0, s[0])) / (1.0 + bound_norm);
+    k.dual = sqrt(fmax(0.0, s[1])) / (1.0 + cost_norm);
+    k.pobj = s[2]; k.dobj = s[3];
+    k.gap = fabs(s[2] - s[3]) / (1.0 + fabs(s[2]) + fabs(s[3]));
+    k.pmove = sqrt(fmax(0.0, s[6]));
+    k.dmove = sqrt(fmax(0.0, s[7]));
+    if (!isfinite(k.primal) || !isfinite(k.dual) || !isfinite(k.gap))
+        k.primal = k.dual = k.gap = INFINITY;
+    return k;
+}
+
+/* Queue one restart window: `period` iterations, the pending-sum flush and
+ * both evaluations.  Launch count is returned through E->launches. */
+static int queue_window(engine *E, int prec, int period, int adaptive, double fixed_terms)
+{
+    int k, rc;
+    for (k = 0; k < period; ++k) {
+        rc = adaptive ? skk_pdhg_adaptive_iter(&E->P, prec) : skk_pdhg_fixed_iter(&E->P, prec);
+        if (rc) return skg_hip_fail("PDHG iteration launch", rc);
+    }
+    E->launches += (long long)period * (adaptive ? 3 : 2);
+    if (adaptive && (rc = skk_pdhg_flush(&E->P, prec))) return skg_hip_fail("flush launch", rc);
+    if ((rc = skk_pdhg_eval(&E->P, prec, 0, adaptive, fixed_terms))) return skg_hip_fail("evaluation launch", rc);
+    if ((rc = skk_pdhg_eval(&E->P, prec, 1, adaptive, fixed_terms))) return skg_hip_fail("evaluation launch", rc);
+    E->launches += (adaptive ? 2 : 0) + 3 + (adaptive ? 1 : 0) + 4;
+    return 0;
+}
+
+static int run_window(engine *E, int prec, int period, int adaptive, double fixed_terms, int use_graph,
+                      int restart_period)
+{
+    int rc;
+    const int gi = prec;
+    if (use_graph && adaptive && period == restart_period) {
+        if (!E->graph[gi] || E->graph_period[gi] != period) {
+            skk_graph_destroy(E->graph[gi]);
+            E->graph[gi] = NULL;
+            if ((rc = skk_graph_begin())) return skg_hip_fail("graph capture", rc);
+            rc = queue_window(E, prec, period, adaptive, fixed_terms);
+            {
+                void *exec = NULL;
+                const int rc2 = skk_graph_end(&exec);
+                if (rc) { skk_graph_destroy(exec); return rc; }
+                if (rc2) return skg_hip_fail("graph instantiate", rc2);
+                E->graph[gi] = exec;
+                E->graph_period[gi] = period;
+            }
+        }
+        if ((rc = skk_graph_launch(E->graph[gi]))) return skg_hip_fail("graph launch", rc);
+        E->launches += 1;
+    } else {
+        if ((rc = queue_window(E, prec, period, adaptive, fixed_terms))) return rc;
+    }
+    if ((rc = skk_d2h_async(E->h_eval, E->P.eval_out, sizeof(double) * 16)) ||
+        (rc = skk_d2h_async(E->h_state, E->P.st, sizeof(skk_state))) || (rc = skk_sync()))
+        return skg_hip_fail("window read-back", rc);
+    return 0;
+}
+
+int sk_gpu_pdlp_solve(int rows, int cols, int nnz, const int *row_offsets, const int *column_indices,
+                      const double *values, const int *q_row_offsets, const int *q_column_indices,
+                      const double *q_values, int q_nnz, const double *c, const double *row_lower,
+                      const double *row_upper, const double *col_lower, const double *col_upper,
+                      const double *x0, const double *y0, const sk_gpu_pdlp_params *params,
+                      double *x_out, double *y_out, sk_gpu_pdlp_stats *stats)
+{
+    engine E;
+    sk_gpu_pdlp_params P;
+    const double t0 = skg_now();
+    double norm, eta_global, omega = 1.0, s_norm;
+    double mu_at_restart = INFINITY, mu_previous = INFINITY, best_mu_seen = INFINITY;
+    int iteration = 0, windows_since_restart = 0, its_since_restart = 0, prec, rc = 0, k;
+    int stall = 0, tail = 0, diverged = 0, use_avg_final = 0, final_prec;
+    kkt best;
+    memset(&E, 0, sizeof(E));
+    memset(&best, 0, sizeof(best));
+    best.primal = best.dual = best.gap = INFINITY;
+    skg_clear_error();
+    if (!stats) return skg_fail("null stats");
+    memset(stats, 0, sizeof(*stats));
+    if (params) P = *params; else sk_gpu_pdlp_params_default(&P);
+    if (!c || !row_lower || !row_upper || !col_lower || !col_upper || !x_out || (rows > 0 && !y_out))
+        return skg_fail("null model or output pointer");
+    if (q_row_offsets || q_column_indices || q_values || q_nnz)
+        return skg_fail("PDLP engine: quadratic objective not supported by this entry point");
+    if (P.eps_optimal <= 0.0 || P.max_iterations <= 0 || P.restart_period <= 0 ||
+        P.power_iterations < 0 || P.eta_min_factor <= 0.0 || P.eta_max_factor < P.eta_min_factor ||
+        (P.precision != SK_GPU_PREC_FP64 && P.precision != SK_GPU_PREC_MIXED))
+        return skg_fail("invalid PDLP parameters");
+    if (hcsr_validate(rows, cols, nnz, row_offsets, column_indices, values)) return -1;
+    for (k = 0; k < cols; ++k)
+        if (!isfinite(c[k]) || isnan(col_lower[k]) || isnan(col_upper[k]) || col_lower[k] > col_upper[k] ||
+            to_ieee(col_lower[k]) == INFINITY || to_ieee(col_upper[k]) == -INFINITY)
+            return skg_fail("invalid cost or column bounds at %d", k);
+    for (k = 0; k < rows; ++k)
+        if (isnan(row_lower[k]) || isnan(row_upper[k]) || row_lower[k] > row_upper[k] ||
+            to_ieee(row_lower[k]) == INFINITY || to_ieee(row_upper[k]) == -INFINITY)
+            return skg_fail("invalid row bounds at %d", k);
+    if ((rc = skg_have_device())) return -1;
+    E.mixed = P.precision == SK_GPU_PREC_MIXED;
+    if ((rc = engine_build(&E, rows, cols, row_offsets, column_indices, values, c, row_lower, row_upper,
+                           col_lower, col_upper, &P))) { engine_free(&E); return -1; }
+    /* ||A||_2 of the scaled matrix (GpuPdlp.cpp:179-211) */
+    if ((rc = skk_power_norm(&E.P, E.pv, E.pw, E.pv2, P.power_iterations, E.pnorm)) ||
+        (rc = skk_d2h(&s_norm, E.pnorm, sizeof(double)))) {
+        skg_hip_fail("power iteration", rc); engine_free(&E); return -1;
+    }
+    E.launches += 2 + 3LL * P.power_iterations;
+    stats->host_syncs++;
+    norm = sqrt(sqrt(fmax(0.0, s_norm)));
+    if (!(norm > 0.0) || !isfinite(norm)) norm = 1.0;
+    stats->matrix_norm = norm;
+    eta_global = 0.9 / norm;
+    if (E.cost_norm > 0.0 && E.bound_norm > 0.0)
+        omega = fmin(1e6, fmax(1e-6, E.cost_norm / E.bound_norm));
+    prec = E.mixed ? SKK_F32 : SKK_F64;
+    /* warm start: x' = x0 / C, y' = y0 / R, in slot order */
+    if (x0 || y0) {
+        double *tx = (double *)malloc(sizeof(double) * (size_t)(cols + 1));
+        double *ty = (double *)malloc(sizeof(double) * (size_t)(rows + 1));
+        if (!tx || !ty) { free(tx); free(ty); engine_free(&E); return skg_fail("out of host memory"); }
+        for (k = 0; k < cols; ++k) tx[k] = x0 ? x0[E.perm_c[k]] / E.C[E.perm_c[k]] : 0.0;
+        for (k = 0; k < rows; ++k) ty[k] = y0 ? y0[E.perm_r[k]] / E.R[E.perm_r[k]] : 0.0;
+        if (prec == SKK_F32) {
+            float *fx = (float *)malloc(sizeof(float) * (size_t)(cols + 1));
+            float *fy = (float *)malloc(sizeof(float) * (size_t)(rows + 1));
+            if (!fx || !fy) { free(fx); free(fy); free(tx); free(ty); engine_free(&E); return skg_fail("out of host memory"); }
+            for (k = 0; k < cols; ++k) fx[k] = (float)tx[k];
+            for (k = 0; k < rows; ++k) fy[k] = (float)ty[k];
+            rc = skk_h2d(E.P.it[prec].X[0], fx, sizeof(float) * (size_t)cols);
+            if (!rc) rc = skk_h2d(E.P.it[prec].Y[0], fy, sizeof(float) * (size_t)rows);
+            free(fx); free(fy);
+        } else {
+            rc = skk_h2d(E.P.it[prec].X[0], tx, sizeof(double) * (size_t)cols);
+            if (!rc) rc = skk_h2d(E.P.it[prec].Y[0], ty, sizeof(double) * (size_t)rows);
+        }
+        free(tx); free(ty);
+        if (rc) { skg_hip_fail("warm start upload", rc); engine_free(&E); return -1; }
+    }
+    if ((rc = skk_pdhg_init(&E.P, prec, eta_global, omega, P.eta_min_factor / norm,
+                            P.eta_max_factor / norm, x0 != NULL, y0 != NULL))) {
+        skg_hip_fail("PDHG init", rc); engine_free(&E); return -1;
+    }
+    if ((rc = skk_sync())) { skg_hip_fail("setup", rc); engine_free(&E); return -1; }
+    stats->setup_seconds = skg_now() - t0;
+    {
+        const double tl0 = skg_now();
+        while (iteration < P.max_iterations) {
+            const int period = P.restart_period < P.max_iterations - iteration ? P.restart_period
+                                                                              : P.max_iterations - iteration;
+            kkt cur, avg;
+            int average_better, sufficient, necessary, artificial;
+            double mu;
+            if (period <= 0) break;
+            rc = run_window(&E, prec, period, P.adaptive_step,
+                            (double)(windows_since_restart + 1) * (double)P.restart_period, P.use_graph,
+                            P.restart_period);
+            if (rc) { engine_free(&E); return -1; }
+            stats->host_syncs++;
+            iteration += period;
+            if (prec == SKK_F32) stats->fp32_iterations += period; else stats->fp64_iterations += period;
+            its_since_restart += period;
+            ++windows_since_restart;
+            cur = kkt_from(E.h_eval, E.cost_norm, E.bound_norm);
+            avg = kkt_from(E.h_eval + 8, E.cost_norm, E.bound_norm);
+            average_better = kkt_worst(&avg) < kkt_worst(&cur);
+            best = average_better ? avg : cur;
+            if (P.verbosity > 1)
+                fprintf(stderr, "it %d prec %s cur %.3e avg %.3e (p %.2e d %.2e g %.2e) eta %.3e omega %.3e\n",
+                        iteration, prec == SKK_F32 ? "f32" : "f64", kkt_worst(&cur), kkt_worst(&avg),
+                        best.primal, best.dual, best.gap, E.h_state->eta, E.h_state->omega);
+            if (!isfinite(kkt_worst(&cur)) && !isfinite(kkt_worst(&avg))) { diverged = 1; break; }
+            if (best.primal <= P.eps_optimal && best.dual <= P.eps_optimal && best.gap <= P.eps_optimal) {
+                use_avg_final = average_better;
+                stats->converged = 1;
+                break;
+            }
+            if (P.time_limit_seconds > 0.0 && skg_now() - t0 > P.time_limit_seconds) {
+                use_avg_final = average_better;
+                break;
+            }
+            mu = kkt_worst(&best);
+            /* mixed precision: hand over to FP64 once FP32 is near its floor */
+            if (prec == SKK_F32 && !tail) {
+                if (mu < 0.5 * best_mu_seen) { best_mu_seen = mu; stall = 0; }
+                else ++stall;
+                if (mu <= P.tail_switch || stall >= P.tail_stall_windows) {
+                    if ((rc = skk_pdhg_restart(&E.P, SKK_F32, average_better, P.adaptive_step)) ||
+                        (rc = skk_pdhg_promote(&E.P)) ||
+                        (rc = skk_pdhg_eval(&E.P, SKK_F64, 0, P.adaptive_step, 1.0)) ||
+                        (rc = skk_pdhg_restart(&E.P, SKK_F64, 0, P.adaptive_step))) {
+                        skg_hip_fail("precision switch", rc); engine_free(&E); return -1;
+                    }
+                    E.launches += 8;
+                    prec = SKK_F64;
+                    tail = 1;
+                    windows_since_restart = 0;
+                    its_since_restart = 0;
+                    mu_at_restart = mu;
+                    mu_previous = INFINITY;
+                    continue;
+                }
+            }
+            /* restart only when earned (GpuPdlp.cpp:437-478) */
+            sufficient = mu <= 0.2 * mu_at_restart;
+            necessary = mu <= 0.8 * mu_at_restart && mu > mu_previous;
+            artificial = (double)its_since_restart >= 0.36 * (double)iteration;
+            mu_previous = mu;
+            if (!(sufficient || necessary || artificial)) continue;
+            if ((rc = skk_pdhg_restart(&E.P, prec, average_better, P.adaptive_step))) {
+                skg_hip_fail("restart launch", rc); engine_free(&E); return -1;
+            }
+            E.launches += 1;
+            windows_since_restart = 0;
+            its_since_restart = 0;
+            mu_at_restart = mu;
+            mu_previous = INFINITY;
+            stats->restarts++;
+            /* primal weight (GpuPdlp.cpp:485-493) */
+            if (best.pmove > 1e-14 && best.dmove > 1e-14) {
+                const double target = best.dmove / best.pmove;
+                const double updated = exp(0.5 * log(target) + 0.5 * log(omega));
+                if (isfinite(updated)) {
+                    omega = fmin(1e6, fmax(1e-6, updated));
+                    if ((rc = skk_pdhg_set_omega(&E.P, omega))) {
+                        skg_hip_fail("primal weight update", rc); engine_free(&E); return -1;
+                    }
+                    E.launches += 1;
+                }
+            }
+        }
+        final_prec = prec;
+        if ((rc = skk_pdhg_export(&E.P, final_prec, use_avg_final, E.pv, E.pw)) ||
+            (rc = skk_d2h_async(E.h_state, E.P.st, sizeof(skk_state))) || (rc = skk_sync())) {
+            skg_hip_fail("export", rc); engine_free(&E); return -1;
+        }
+        stats->loop_seconds = skg_now() - tl0;
+    }
+    {
+        double *xs = (double *)malloc(sizeof(double) * (size_t)(cols + 1));
+        double *ys = (double *)malloc(sizeof(double) * (size_t)(rows + 1));
+        if (!xs || !ys) { free(xs); free(ys); engine_free(&E); return skg_fail("out of host memory"); }
+        if ((rc = skk_d2h(xs, E.pv, sizeof(double) * (size_t)cols)) ||
+            (rc = skk_d2h(ys, E.pw, sizeof(double) * (size_t)rows))) {
+            free(xs); free(ys); skg_hip_fail("result copy", rc); engine_free(&E); return -1;
+        }
+        stats->host_syncs += 2;
+        for (k = 0; k < cols; ++k) x_out[E.perm_c[k]] = xs[k] * E.C[E.perm_c[k]];
+        for (k = 0; k < rows; ++k) y_out[E.perm_r[k]] = ys[k] * E.R[E.perm_r[k]];
+        free(xs); free(ys);
+    }
+    stats->iterations = iteration;
+    stats->primal_objective = best.pobj;
+    stats->dual_objective = best.dobj;
+    stats->relative_primal_residual = best.primal;
+    stats->relative_dual_residual = best.dual;
+    stats->relative_gap = best.gap;
+    stats->primal_weight = omega;
+    if (P.adaptive_step) {
+        const double rej = E.h_state->updates - E.h_state->accept_count;
+        stats->final_step_size = E.h_state->eta;
+        stats->step_size_ratio = E.h_state->eta * norm;
+        stats->rejected_steps = (int)(rej > 0.0 ? rej : 0.0);
+    } else {
+        stats->final_step_size = eta_global;
+        stats->step_size_ratio = 0.9;
+    }
+    stats->kernel_launches = E.launches;
+    stats->seconds = skg_now() - t0;
+    engine_free(&E);
+    if (diverged) { skg_fail("PDHG iterate became non-finite"); return -2; }
+    return 0;
+}

@@ -1,0 +1,354 @@
+"""Python bindings (ctypes) for the sankhya_gpu C ABI and its workload library.
+
+The structures mirror ``include/sankhya_gpu.h`` field for field; that header
+mirrors the reference C API (``c/include/sankhya.h:17-161``) and GPU ABI
+(``cuda/include/sankhya_cuda.h``).  Nothing here computes: every solve goes
+through ``libsankhya_gpu.so`` (HIP, gfx950).  Loading fails loudly when the
+library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+GPU_LIB = HERE / "libsankhya_gpu.so"
+WORKLOADS_LIB = HERE / "libsk_workloads.so"
+
+c_int_p = C.POINTER(C.c_int)
+c_double_p = C.POINTER(C.c_double)
+
+# sk_status / sk_result values (c/include/sankhya.h:17-38)
+SK_OK, SK_ERR_ARG, SK_ERR_MEMORY, SK_ERR_STRUCTURE = 0, 1, 2, 3
+SK_ERR_IO, SK_ERR_SINGULAR, SK_ERR_UNSUPPORTED, SK_ERR_NUMERIC = 4, 5, 6, 7
+RESULT_NAMES = ["optimal", "infeasible", "unbounded", "iteration_limit",
+                "time_limit", "numeric_failure", "not_run", "gap_limit"]
+SK_RESULT_OPTIMAL, SK_RESULT_INFEASIBLE, SK_RESULT_UNBOUNDED = 0, 1, 2
+SK_RESULT_ITERATION_LIMIT, SK_RESULT_TIME_LIMIT, SK_RESULT_NUMERIC_FAILURE = 3, 4, 5
+SK_RESULT_NOT_RUN = 6
+SK_LP_AUTO, SK_LP_SIMPLEX, SK_LP_FIRST_ORDER = 0, 1, 2
+SK_INFINITY = 1.0e30
+PREC_FP64, PREC_MIXED = 0, 1
+
+
+class SkCsc(C.Structure):
+    _fields_ = [("nrow", C.c_int), ("ncol", C.c_int), ("p", c_int_p),
+                ("i", c_int_p), ("x", c_double_p), ("nzmax", C.c_int)]
+
+
+class SkModel(C.Structure):
+    _fields_ = [("nrow", C.c_int), ("ncol", C.c_int), ("A", SkCsc),
+                ("Q", C.POINTER(SkCsc)), ("c", c_double_p), ("rlow", c_double_p),
+                ("rupp", c_double_p), ("clow", c_double_p), ("cupp", c_double_p),
+                ("vartype", C.POINTER(C.c_ubyte)), ("objshift", C.c_double),
+                ("objsense", C.c_double), ("rowname", C.POINTER(C.c_char_p)),
+                ("colname", C.POINTER(C.c_char_p)), ("name", C.c_char_p)]
+
+
+class SkSolution(C.Structure):
+    _fields_ = [("result", C.c_int), ("objective", C.c_double), ("x", c_double_p),
+                ("y", c_double_p), ("rc", c_double_p), ("rowact", c_double_p),
+                ("ncol", C.c_int), ("nrow", C.c_int), ("iterations", C.c_longlong),
+                ("nodes", C.c_longlong), ("solve_seconds", C.c_double),
+                ("primal_infeasibility", C.c_double), ("dual_infeasibility", C.c_double),
+                ("complementarity", C.c_double), ("mip_gap", C.c_double),
+                ("dual_bound", C.c_double)]
+
+
+class SkOptions(C.Structure):
+    _fields_ = [("primal_tol", C.c_double), ("dual_tol", C.c_double),
+                ("pivot_tol", C.c_double), ("time_limit", C.c_double),
+                ("iteration_limit", C.c_longlong), ("node_limit", C.c_longlong),
+                ("presolve", C.c_int), ("scaling", C.c_int),
+                ("refactor_interval", C.c_int), ("verbosity", C.c_int),
+                ("threads", C.c_int), ("mip_gap_abs", C.c_double),
+                ("mip_gap_rel", C.c_double), ("mip_cuts", C.c_int),
+                ("mip_heuristics", C.c_int), ("dual_simplex", C.c_int),
+                ("lp_engine", C.c_int), ("random_seed", C.c_ulonglong)]
+
+
+class GpuCsr(C.Structure):
+    _fields_ = [("rows", C.c_int), ("cols", C.c_int), ("nnz", C.c_int),
+                ("device_row_offsets", C.c_void_p), ("device_column_indices", C.c_void_p),
+                ("device_values", C.c_void_p), ("plan", C.c_void_p)]
+
+
+class LpSettings(C.Structure):
+    _fields_ = [("max_iterations", C.c_int), ("check_every", C.c_int),
+                ("tau", C.c_double), ("sigma", C.c_double), ("theta", C.c_double),
+                ("tolerance", C.c_double), ("time_limit", C.c_double)]
+
+
+class LpResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("objective", C.c_double),
+                ("maximum_row_violation", C.c_double), ("maximum_step", C.c_double),
+                ("maximum_kkt_residual", C.c_double), ("maximum_dual_residual", C.c_double),
+                ("maximum_complementarity", C.c_double)]
+
+
+class PdlpParams(C.Structure):
+    _fields_ = [("eps_optimal", C.c_double), ("max_iterations", C.c_int),
+                ("restart_period", C.c_int), ("time_limit_seconds", C.c_double),
+                ("power_iterations", C.c_int), ("adaptive_step", C.c_int),
+                ("eta_min_factor", C.c_double), ("eta_max_factor", C.c_double),
+                ("precision", C.c_int), ("tail_switch", C.c_double),
+                ("tail_stall_windows", C.c_int), ("scaling", C.c_int),
+                ("ruiz_iterations", C.c_int), ("ruiz_tol", C.c_double),
+                ("use_graph", C.c_int), ("verbosity", C.c_int)]
+
+
+class PdlpStats(C.Structure):
+    _fields_ = [("converged", C.c_int), ("cancelled", C.c_int), ("iterations", C.c_int),
+                ("restarts", C.c_int), ("primal_objective", C.c_double),
+                ("dual_objective", C.c_double), ("relative_primal_residual", C.c_double),
+                ("relative_dual_residual", C.c_double), ("relative_gap", C.c_double),
+                ("matrix_norm", C.c_double), ("seconds", C.c_double),
+                ("rejected_steps", C.c_int), ("final_step_size", C.c_double),
+                ("step_size_ratio", C.c_double), ("host_syncs", C.c_int),
+                ("fp32_iterations", C.c_int), ("fp64_iterations", C.c_int),
+                ("kernel_launches", C.c_longlong), ("loop_seconds", C.c_double),
+                ("setup_seconds", C.c_double), ("primal_weight", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class GpuOptions(C.Structure):
+    _fields_ = [("pdlp", PdlpParams), ("x0", c_double_p), ("y0", c_double_p),
+                ("stats", PdlpStats)]
+
+
+def default_pdlp_params(**overrides) -> PdlpParams:
+    """Defaults of PdlpParams (src/cuda/GpuPdlp.hpp:19-67) except eps 1e-6;
+    the same values the library's sk_gpu_pdlp_params_default sets."""
+    p = PdlpParams(eps_optimal=1e-6, max_iterations=500000, restart_period=64,
+                   time_limit_seconds=0.0, power_iterations=30, adaptive_step=1,
+                   eta_min_factor=1e-6, eta_max_factor=1e4, precision=PREC_FP64,
+                   tail_switch=1e-5, tail_stall_windows=8, scaling=1,
+                   ruiz_iterations=100, ruiz_tol=1e-2, use_graph=0, verbosity=0)
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
+
+
+def dptr(a):
+    return _ptr(a, C.c_double)
+
+
+def iptr(a):
+    return _ptr(a, C.c_int)
+
+
+# ---------------------------------------------------------------- loading
+
+_gpu = None
+_work = None
+
+
+def _declare(lib, name, restype, argtypes):
+    f = getattr(lib, name)
+    f.restype = restype
+    f.argtypes = argtypes
+    return f
+
+
+def load_workloads():
+    global _work
+    if _work is None:
+        if not WORKLOADS_LIB.exists():
+            raise RuntimeError(f"{WORKLOADS_LIB} is not built; run __graft_entry__.build()")
+        lib = C.CDLL(str(WORKLOADS_LIB))
+        M = C.POINTER(SkModel)
+        _declare(lib, "skw_model_free", None, [M])
+        _declare(lib, "skw_bounded_lp", C.c_int, [M, C.c_int, C.c_int, C.c_int, C.c_ulonglong])
+        _declare(lib, "skw_zipf_lp", C.c_int, [M, C.c_int, C.c_int, C.c_double, C.c_double,
+                                               C.c_int, C.c_double, C.c_ulonglong, c_double_p])
+        _declare(lib, "skw_sparse_qp", C.c_int, [M, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                 C.c_int, C.c_ulonglong])
+        _declare(lib, "skw_batch_sizes", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_ulonglong, c_int_p])
+        _declare(lib, "skw_parametric_family", C.c_int,
+                 [M, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong,
+                  c_double_p, c_double_p, c_double_p, c_double_p])
+        _declare(lib, "skw_parametric_instance", None,
+                 [C.c_int, C.c_int, C.c_int, c_double_p, c_double_p, c_double_p, c_double_p,
+                  c_double_p, c_double_p, c_double_p, c_double_p])
+        _declare(lib, "skw_normal_matrix", None, [C.c_int, C.c_int, C.c_ulonglong, c_double_p])
+        _work = lib
+    return _work
+
+
+def load_gpu():
+    """Load libsankhya_gpu.so.  Raises if it is missing: there is no fallback."""
+    global _gpu
+    if _gpu is None:
+        if not GPU_LIB.exists():
+            raise RuntimeError(f"{GPU_LIB} is not built; run __graft_entry__.build()")
+        lib = C.CDLL(str(GPU_LIB))
+        M = C.POINTER(SkModel)
+        S = C.POINTER(SkSolution)
+        H = C.POINTER(GpuCsr)
+        _declare(lib, "sk_gpu_last_error", C.c_char_p, [])
+        _declare(lib, "sk_gpu_csr_create", C.c_int, [H, C.c_int, C.c_int, C.c_int, c_int_p, c_int_p, c_double_p])
+        _declare(lib, "sk_gpu_csr_destroy", None, [H])
+        _declare(lib, "sk_gpu_spmv_device_f64", C.c_int, [H, C.c_void_p, C.c_void_p])
+        _declare(lib, "sk_gpu_spmv_transpose_device_f64", C.c_int, [H, C.c_void_p, C.c_void_p])
+        _declare(lib, "sk_gpu_axpy_device_f64", C.c_int, [C.c_int, C.c_double, C.c_void_p, C.c_void_p])
+        _declare(lib, "sk_gpu_synchronize", C.c_int, [])
+        _declare(lib, "sk_gpu_spmv_f64", C.c_int, [C.c_int, C.c_int, C.c_int, c_int_p, c_int_p,
+                                                   c_double_p, c_double_p, c_double_p])
+        _declare(lib, "sk_gpu_axpy_f64", C.c_int, [C.c_int, C.c_double, c_double_p, c_double_p])
+        lp_args = [H, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p, LpSettings,
+                   c_double_p, C.POINTER(LpResult)]
+        _declare(lib, "sk_gpu_lp_pdhg", C.c_int, lp_args)
+        _declare(lib, "sk_gpu_diagonal_qp_pdhg", C.c_int,
+                 [H, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                  LpSettings, c_double_p, C.POINTER(LpResult)])
+        _declare(lib, "sk_gpu_sparse_qp_pdhg", C.c_int,
+                 [H, H, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                  LpSettings, c_double_p, C.POINTER(LpResult)])
+        _declare(lib, "sk_gpu_qp_pdhg_preconditioned", C.c_int,
+                 [H, H, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                  c_double_p, c_double_p, LpSettings, c_double_p, C.POINTER(LpResult)])
+        _declare(lib, "sk_gpu_qp_pdhg_preconditioned_with_dual", C.c_int,
+                 [H, H, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                  c_double_p, c_double_p, LpSettings, c_double_p, c_double_p, C.POINTER(LpResult)])
+        _declare(lib, "sk_gpu_pdlp_params_default", None, [C.POINTER(PdlpParams)])
+        _declare(lib, "sk_gpu_pdlp_solve", C.c_int,
+                 [C.c_int, C.c_int, C.c_int, c_int_p, c_int_p, c_double_p,
+                  c_int_p, c_int_p, c_double_p, C.c_int,
+                  c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                  c_double_p, c_double_p, C.POINTER(PdlpParams),
+                  c_double_p, c_double_p, C.POINTER(PdlpStats)])
+        _declare(lib, "sk_gpu_solver_options_default", None, [C.POINTER(GpuOptions)])
+        _declare(lib, "sk_gpu_solve", C.c_int, [M, C.POINTER(SkOptions), C.POINTER(GpuOptions), S])
+        _declare(lib, "sk_gpu_verify", C.c_int, [M, S])
+        _declare(lib, "sk_gpu_relative_kkt", C.c_int,
+                 [C.c_int, C.c_int, C.c_int, c_int_p, c_int_p, c_double_p, c_double_p,
+                  c_double_p, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                  c_double_p])
+        _declare(lib, "sk_gpu_model_validate", C.c_int, [M])
+        _declare(lib, "sk_gpu_solution_init", None, [S])
+        _declare(lib, "sk_gpu_solution_free", None, [S])
+        _declare(lib, "sk_gpu_options_default", None, [C.POINTER(SkOptions)])
+        _declare(lib, "sk_gpu_result_name", C.c_char_p, [C.c_int])
+        _declare(lib, "sk_gpu_status_name", C.c_char_p, [C.c_int])
+        _gpu = lib
+    return _gpu
+
+
+def last_error() -> str:
+    return load_gpu().sk_gpu_last_error().decode()
+
+
+# ---------------------------------------------------------------- models
+
+class Model:
+    """An sk_model in CSC form, owned either by numpy arrays or by the
+    workload generator (freed with skw_model_free)."""
+
+    def __init__(self):
+        self.m = SkModel()
+        self.m.objsense = 1.0
+        self._keep = []
+        self._generated = False
+        self._qcsc = None
+
+    @classmethod
+    def from_arrays(cls, nrow, ncol, p, i, x, c, rlow, rupp, clow, cupp, q=None):
+        """CSC arrays (int32 p[ncol+1], i[nnz], float64 x[nnz]); q is an
+        optional (p, i, x) triple for the full symmetric Q."""
+        self = cls()
+        arrs = [np.ascontiguousarray(p, np.int32), np.ascontiguousarray(i, np.int32),
+                np.ascontiguousarray(x, np.float64), np.ascontiguousarray(c, np.float64),
+                np.ascontiguousarray(rlow, np.float64), np.ascontiguousarray(rupp, np.float64),
+                np.ascontiguousarray(clow, np.float64), np.ascontiguousarray(cupp, np.float64),
+                np.zeros(max(ncol, 1), np.uint8)]
+        # never hand out zero-length buffers (NULL-looking pointers)
+        arrs = [a if a.size else np.zeros(1, a.dtype) for a in arrs]
+        self._keep = arrs
+        m = self.m
+        m.nrow, m.ncol = nrow, ncol
+        m.A = SkCsc(nrow, ncol, iptr(arrs[0]), iptr(arrs[1]), dptr(arrs[2]), int(arrs[1].size))
+        m.c, m.rlow, m.rupp, m.clow, m.cupp = (dptr(a) for a in arrs[3:8])
+        m.vartype = arrs[8].ctypes.data_as(C.POINTER(C.c_ubyte))
+        if q is not None:
+            qp, qi, qx = (np.ascontiguousarray(q[0], np.int32), np.ascontiguousarray(q[1], np.int32),
+                          np.ascontiguousarray(q[2], np.float64))
+            qi = qi if qi.size else np.zeros(1, np.int32)
+            qx = qx if qx.size else np.zeros(1, np.float64)
+            self._keep += [qp, qi, qx]
+            self._qcsc = SkCsc(ncol, ncol, iptr(qp), iptr(qi), dptr(qx), int(qi.size))
+            m.Q = C.pointer(self._qcsc)
+        return self
+
+    @classmethod
+    def generated(cls, kind, *args):
+        self = cls()
+        w = load_workloads()
+        rc = getattr(w, kind)(C.byref(self.m), *args)
+        if rc != 0:
+            raise RuntimeError(f"{kind}{args} failed")
+        self._generated = True
+        return self
+
+    def __del__(self):
+        if self._generated and _work is not None:
+            _work.skw_model_free(C.byref(self.m))
+            self._generated = False
+
+    # numpy views ---------------------------------------------------
+    @property
+    def nrow(self):
+        return self.m.nrow
+
+    @property
+    def ncol(self):
+        return self.m.ncol
+
+    @property
+    def nnz(self):
+        return int(self.m.A.p[self.m.ncol])
+
+    def _view(self, ptr, n, dtype=np.float64):
+        if n == 0:
+            return np.zeros(0, dtype)
+        return np.ctypeslib.as_array(ptr, shape=(n,)).view(dtype)
+
+    def csc(self):
+        n = self.ncol
+        p = self._view(self.m.A.p, n + 1, np.int32).copy()
+        nnz = int(p[-1])
+        return p, self._view(self.m.A.i, nnz, np.int32).copy(), self._view(self.m.A.x, nnz).copy()
+
+    def q_csc(self):
+        if not self.m.Q:
+            return None
+        q = self.m.Q.contents
+        p = self._view(q.p, self.ncol + 1, np.int32).copy()
+        nnz = int(p[-1])
+        return p, self._view(q.i, nnz, np.int32).copy(), self._view(q.x, nnz).copy()
+
+    def vec(self, name):
+        n = self.ncol if name in ("c", "clow", "cupp") else self.nrow
+        return self._view(getattr(self.m, name), n).copy()
+
+    def ptr(self):
+        return C.byref(self.m)
+
+
+def csc_to_csr(nrow, ncol, p, i, x):
+    """Counting-sort transpose: CSC(A) -> CSR(A) (apps/sankhya_qp_cuda.cu:15-33)."""
+    nnz = int(p[-1])
+    counts = np.bincount(i[:nnz], minlength=nrow)
+    rp = np.zeros(nrow + 1, np.int32)
+    np.cumsum(counts, out=rp[1:])
+    cols = np.repeat(np.arange(ncol, dtype=np.int32), np.diff(p))
+    order = np.argsort(i[:nnz], kind="stable")
+    return rp, cols[order].astype(np.int32), x[:nnz][order].astype(np.float64)
