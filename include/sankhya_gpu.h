@@ -8,10 +8,11 @@
  * Each declaration cites the reference interface it replaces
  * (Jayaragul/The-solver, paths relative to that repository).
  *
- * Threading: the library owns one HIP stream per device at the default
- * priority; calls from several host threads on one device are serialised by
- * an internal mutex.  The caller selects the device (hipSetDevice) as with
- * the reference's cudaSetDevice (apps/sankhya_pdhg.cu:79).
+ * Threading: the library queues all device work on one stream per device
+ * (default priority), starts no threads and takes no locks; a CSR handle or
+ * a PDLP session must not be used from two host threads at once.  The
+ * caller selects the device (hipSetDevice) as with the reference's
+ * cudaSetDevice (apps/sankhya_pdhg.cu:79).
  */
 #ifndef SANKHYA_GPU_H
 #define SANKHYA_GPU_H
@@ -312,7 +313,8 @@ void sk_gpu_pdlp_params_default(sk_gpu_pdlp_params *p);
 
 /* Solve min c'x (+ 1/2 x'Qx) s.t. row_lower <= A x <= row_upper,
  * col_lower <= x <= col_upper.  A is CSR in host memory (FP64); Q, if given,
- * is a full symmetric CSR (n x n).  Bounds use IEEE +-inf or the +-1e30
+ * is a full symmetric positive semidefinite CSR (n x n) and needs the
+ * adaptive step.  Bounds use IEEE +-inf or the +-1e30
  * sentinel.  x0/y0 are an optional warm start (either may be NULL).  Outputs
  * x[n], y[m] in host memory, in the original (unscaled) space, with the
  * sign convention of sk_verify: rc = c + Qx + A'y.  Returns 0 (read
@@ -327,6 +329,77 @@ int sk_gpu_pdlp_solve(int rows, int cols, int nnz, const int *row_offsets,
                       const double *col_upper, const double *x0,
                       const double *y0, const sk_gpu_pdlp_params *params,
                       double *x, double *y, sk_gpu_pdlp_stats *stats);
+
+/* Resident problem: the scaled, permuted, binned operator and its ||A||_2
+ * stay on the device across solves (the role of a GpuPdlp object,
+ * src/cuda/GpuPdlp.hpp:128-196).  `update` replaces c and the bounds (same
+ * A) -- the parametric-family case; `iterate` runs raw iterations from a
+ * cold start with no checks (benchmarking; with kernel_ms it also returns
+ * the mean event-timed milliseconds of the three launches of an iteration
+ * and of the whole iteration, kernel_ms[0..3]). */
+typedef struct sk_gpu_pdlp_session sk_gpu_pdlp_session;
+int  sk_gpu_pdlp_session_create(int rows, int cols, int nnz, const int *row_offsets,
+                                const int *column_indices, const double *values,
+                                const double *c, const double *row_lower,
+                                const double *row_upper, const double *col_lower,
+                                const double *col_upper, const sk_gpu_pdlp_params *params,
+                                sk_gpu_pdlp_session **session);
+/* Convex QP: Q is the full symmetric Hessian as CSR (n x n), PSD. */
+int  sk_gpu_pdlp_session_create_qp(int rows, int cols, int nnz, const int *row_offsets,
+                                   const int *column_indices, const double *values,
+                                   const int *q_row_offsets, const int *q_column_indices,
+                                   const double *q_values, int q_nnz, const double *c,
+                                   const double *row_lower, const double *row_upper,
+                                   const double *col_lower, const double *col_upper,
+                                   const sk_gpu_pdlp_params *params,
+                                   sk_gpu_pdlp_session **session);
+int  sk_gpu_pdlp_session_update(sk_gpu_pdlp_session *session, const double *c,
+                                const double *row_lower, const double *row_upper,
+                                const double *col_lower, const double *col_upper);
+int  sk_gpu_pdlp_session_solve(sk_gpu_pdlp_session *session, const double *x0,
+                               const double *y0, double *x, double *y,
+                               sk_gpu_pdlp_stats *stats);
+int  sk_gpu_pdlp_session_iterate(sk_gpu_pdlp_session *session, int iterations,
+                                 double *kernel_ms);
+void sk_gpu_pdlp_session_destroy(sk_gpu_pdlp_session *session);
+
+/* ------------------------------------------------------------------------
+ * Batched PDLP: many small LPs in one persistent launch.  No reference
+ * counterpart (the reference solves one model per call, sk_solve,
+ * c/src/sk_solve.c:1009); each instance follows the same algorithm as
+ * sk_gpu_pdlp_solve with FP32 iterates and FP64 KKT evaluation.
+ * ---------------------------------------------------------------------- */
+typedef struct sk_gpu_batch_options {
+    sk_gpu_pdlp_params pdlp;  /* eps, iteration limit, restart period, ... */
+    int block_threads;        /* threads per instance workgroup; 0 = autotune */
+    int use_graph;            /* replay the launch from a captured hipGraph  */
+} sk_gpu_batch_options;
+
+typedef struct sk_gpu_batch_result {
+    int    status;            /* 0 relative KKT <= eps, 1 iteration limit  */
+    int    iterations;
+    int    restarts;
+    double relative_primal_residual;
+    double relative_dual_residual;
+    double relative_gap;
+    double objective;         /* original space, including objshift        */
+} sk_gpu_batch_result;
+
+typedef struct sk_gpu_batch_stats {
+    double setup_seconds;     /* scaling, packing, upload                  */
+    double device_seconds;    /* the persistent launch (event-timed)       */
+    double seconds;
+    int    block_threads, grid, lds_bytes, converged;
+    long long total_iterations;
+    double tune_ms[3];        /* autotuner timings for 64/128/256 threads  */
+} sk_gpu_batch_stats;
+
+void sk_gpu_batch_options_default(sk_gpu_batch_options *o);
+/* models[count]: continuous LPs without Q.  x_out holds the instances'
+ * columns back to back (sum of ncol), y_out their rows (sum of nrow). */
+int  sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_options *opt,
+                        double *x_out, double *y_out, sk_gpu_batch_result *results,
+                        sk_gpu_batch_stats *stats);
 
 /* ------------------------------------------------------------------------
  * sk_solve-style entry point.  Mirrors sk_solve (c/include/sankhya.h:173,

@@ -1,0 +1,68 @@
+"""Batched PDLP (one persistent launch, one workgroup per instance): each
+instance agrees with the CPU oracle's PDLP and with an independent LP
+solver, and its result does not depend on the batch it was solved in."""
+import numpy as np
+import pytest
+
+import sankhya_gpu as sg
+import sk_oracle as so
+
+pytestmark = pytest.mark.gpu
+
+
+def _models(count, lo, hi, seed):
+    w = sg.load_workloads()
+    sizes = (sg.C.c_int * count)()
+    assert w.skw_batch_sizes(count, lo, hi, seed, sizes) == 0
+    return [sg.Model.generated("skw_bounded_lp", sizes[i], 2 * sizes[i], 5, seed * count + i) for i in range(count)]
+
+
+def test_batch_matches_oracle_and_highs():
+    models = _models(48, 20, 120, 11)
+    rc, xs, ys, res, st = sg.batch_solve(models, eps_optimal=1e-7, max_iterations=200000)
+    assert rc == 0, sg.last_error()
+    assert st.converged == len(models)
+    from scipy.optimize import linprog
+    import scipy.sparse as sp
+    for k, md in enumerate(models):
+        c = md.vec("c")
+        p, i, x = md.csc()
+        A = sp.csc_matrix((x, i, p), shape=(md.nrow, md.ncol))
+        r = linprog(c, A_ub=sp.vstack([A, -A]), b_ub=np.concatenate([md.vec("rupp"), -md.vec("rlow")]),
+                    bounds=list(zip(md.vec("clow"), md.vec("cupp"))), method="highs")
+        assert r.status == 0
+        assert abs(c @ xs[k] - r.fun) <= 1e-6 * (1 + abs(r.fun)), (k, c @ xs[k], r.fun)
+        assert abs(res[k].objective - c @ xs[k]) <= 1e-9 * (1 + abs(r.fun))
+        kkt = so.relative_kkt(md, xs[k], ys[k])
+        assert max(kkt[:3]) <= 1e-5, (k, kkt)
+    # the CPU oracle's PDLP reaches the same optimum
+    rc_o, xo, yo, sto = so.first_order_pdlp(models[0], sg.default_pdlp_params(eps_optimal=1e-8))
+    assert abs(models[0].vec("c") @ xo - models[0].vec("c") @ xs[0]) <= 1e-6 * (1 + abs(models[0].vec("c") @ xo))
+
+
+@pytest.mark.parametrize("threads", [64, 128, 256])
+def test_instance_result_independent_of_batch(threads):
+    models = _models(24, 20, 80, 5)
+    rc, xs, ys, res, st = sg.batch_solve(models, block_threads=threads, eps_optimal=1e-6)
+    assert rc == 0 and st.block_threads == threads
+    for k in (0, 7, 23):
+        rc1, x1, y1, r1, _ = sg.batch_solve([models[k]], block_threads=threads, eps_optimal=1e-6)
+        assert rc1 == 0
+        assert r1[0].iterations == res[k].iterations
+        assert np.array_equal(x1[0], xs[k]) and np.array_equal(y1[0], ys[k])
+
+
+def test_graph_and_direct_launch_agree():
+    models = _models(16, 20, 60, 3)
+    a = sg.batch_solve(models, block_threads=128, use_graph=0, eps_optimal=1e-6)
+    b = sg.batch_solve(models, block_threads=128, use_graph=1, eps_optimal=1e-6)
+    assert a[0] == 0 and b[0] == 0
+    for k in range(len(models)):
+        assert np.array_equal(a[1][k], b[1][k])
+
+
+def test_iteration_limit_reported():
+    models = _models(8, 50, 60, 9)
+    rc, xs, ys, res, st = sg.batch_solve(models, eps_optimal=1e-9, max_iterations=128)
+    assert rc == 0
+    assert all(r.iterations == 128 and r.status == 1 for r in res)

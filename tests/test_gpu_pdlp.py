@@ -117,3 +117,67 @@ def test_iteration_limit_and_rejection():
     arr = dense_lp([[1]], [np.nan], [1], [2], [0], [1])
     rc, *_ = gpu_pdlp(arrays=arr)
     assert rc == -1
+
+
+def _qp_arrays(A, c, rl, ru, l, u, Q):
+    arr = dense_lp(A, c, rl, ru, l, u)
+    Qd = np.asarray(Q, float)
+    rp, cj, v = [0], [], []
+    for i in range(Qd.shape[0]):
+        for j in range(Qd.shape[1]):
+            if Qd[i, j] != 0:
+                cj.append(j); v.append(Qd[i, j])
+        rp.append(len(cj))
+    return arr, (np.asarray(rp, np.int32), np.asarray(cj, np.int32), np.asarray(v, float))
+
+
+def _gpu_qp(arr, q, **params):
+    import ctypes as C
+    lib = sg.load_gpu()
+    m, n, rp, cj, v, vec = arr
+    P = sg.default_pdlp_params(**params)
+    x = np.zeros(n); y = np.zeros(max(m, 1)); st = sg.PdlpStats()
+    rc = lib.sk_gpu_pdlp_solve(m, n, int(rp[-1]), sg.iptr(rp), sg.iptr(cj), sg.dptr(v),
+                               sg.iptr(q[0]), sg.iptr(q[1]), sg.dptr(q[2]), int(q[0][-1]),
+                               sg.dptr(vec["c"]), sg.dptr(vec["rlow"]), sg.dptr(vec["rupp"]),
+                               sg.dptr(vec["clow"]), sg.dptr(vec["cupp"]), None, None, C.byref(P),
+                               sg.dptr(x), sg.dptr(y), C.byref(st))
+    return rc, x, y[:m], st
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_qp_closed_forms(prec):
+    # general_qp_active_smoke.c: min x'Qx/2 - 3x1 - 3x2, Q=[[2,1],[1,2]], x1+x2<=1 -> (1/2,1/2), -2.25
+    arr, q = _qp_arrays([[1, 1]], [-3, -3], [-INF], [1], [0, 0], [10, 10], [[2, 1], [1, 2]])
+    rc, x, y, st = _gpu_qp(arr, q, precision=prec, eps_optimal=1e-9)
+    assert rc == 0 and st.converged, sg.last_error()
+    assert np.allclose(x, [0.5, 0.5], atol=1e-6)
+    # diagonal_qp_smoke.c (cuda): min x^2 - 4x st x >= 1 -> x = 2, objective -4, y = 0
+    arr, q = _qp_arrays([[1]], [-4], [1], [INF], [0], [INF], [[2]])
+    rc, x, y, st = _gpu_qp(arr, q, precision=prec, eps_optimal=1e-9)
+    assert rc == 0 and st.converged
+    assert abs(x[0] - 2.0) < 1e-6 and abs(y[0]) < 1e-6
+
+
+def test_qp_reference_first_order_fixture():
+    g = golden("qp_small.npz")
+    model = sg.Model.generated("skw_sparse_qp", 300, 600, 3, 200, 3, 777)
+    import ctypes as C
+    lib = sg.load_gpu()
+    o = sg.SkOptions(); lib.sk_gpu_options_default(C.byref(o))
+    go = sg.GpuOptions(); lib.sk_gpu_solver_options_default(C.byref(go))
+    go.pdlp.eps_optimal = 1e-7
+    s = sg.SkSolution()
+    assert lib.sk_gpu_solve(model.ptr(), C.byref(o), C.byref(go), C.byref(s)) == 0
+    assert s.result == sg.SK_RESULT_OPTIMAL, sg.last_error()
+    x = np.ctypeslib.as_array(s.x, shape=(model.ncol,)).copy()
+    y = np.ctypeslib.as_array(s.y, shape=(model.nrow,)).copy()
+    # the reference's CPU QP PDHG stops near 1e-5 KKT: agree with it to that level
+    ref = float(g["pdhg_objective"])
+    assert abs(s.objective - ref) <= 1e-4 * (1 + abs(ref)), (s.objective, ref)
+    # and certify this point independently (convex QP: KKT residuals)
+    rc, obj, pinf, dinf, comp = so.verify_orc(model, x, y)
+    kkt = so.relative_kkt(model, x, y)
+    assert max(kkt[:3]) <= 1e-6, kkt
+    assert pinf <= 1e-5
+    lib.sk_gpu_solution_free(C.byref(s))

@@ -18,13 +18,11 @@ VAL = np.array([1.0, 2.0, 3.0, 4.0])
 
 
 def _dev(arr):
-    import torch
-    return torch.from_numpy(np.ascontiguousarray(arr, np.float64)).to("cuda")
+    return sg.DeviceArray(arr)
 
 
 def _sync():
-    import torch
-    torch.cuda.synchronize()
+    sg.device_synchronize()
     assert sg.load_gpu().sk_gpu_synchronize() == 0
 
 
@@ -55,15 +53,15 @@ def test_device_chain_known_answers():
     dx = _dev([2.0, 5.0, 7.0])
     dy = _dev(np.zeros(2))
     _sync()
-    assert lib.sk_gpu_spmv_device_f64(C.byref(h), dx.data_ptr(), dy.data_ptr()) == 0
+    assert lib.sk_gpu_spmv_device_f64(C.byref(h), dx.value, dy.value) == 0
     _sync()
-    assert np.array_equal(dy.cpu().numpy(), [16.0, 43.0])
+    assert np.array_equal(dy.numpy(), [16.0, 43.0])
     # A^T [16, 43] = [16, 129, 204]; A [16, 129, 204] = [424, 1203]
-    assert lib.sk_gpu_spmv_transpose_device_f64(C.byref(h), dy.data_ptr(), dx.data_ptr()) == 0
-    assert lib.sk_gpu_spmv_device_f64(C.byref(h), dx.data_ptr(), dy.data_ptr()) == 0
+    assert lib.sk_gpu_spmv_transpose_device_f64(C.byref(h), dy.value, dx.value) == 0
+    assert lib.sk_gpu_spmv_device_f64(C.byref(h), dx.value, dy.value) == 0
     _sync()
-    assert np.array_equal(dx.cpu().numpy(), [16.0, 129.0, 204.0])
-    assert np.array_equal(dy.cpu().numpy(), [424.0, 1203.0])
+    assert np.array_equal(dx.numpy(), [16.0, 129.0, 204.0])
+    assert np.array_equal(dy.numpy(), [424.0, 1203.0])
     lib.sk_gpu_csr_destroy(C.byref(h))
     lib.sk_gpu_csr_destroy(C.byref(h))       # idempotent
     assert h.rows == 0 and h.plan is None
@@ -108,17 +106,17 @@ def test_random_family_matches_cpu(seed, m, n, kind):
     dx = _dev(x)
     dy = _dev(np.full(m, np.nan))
     _sync()
-    assert lib.sk_gpu_spmv_device_f64(C.byref(h), dx.data_ptr(), dy.data_ptr()) == 0
+    assert lib.sk_gpu_spmv_device_f64(C.byref(h), dx.value, dy.value) == 0
     _sync()
-    y = dy.cpu().numpy()
+    y = dy.numpy()
     scale = np.abs(A).sum(axis=1).A1 * np.abs(x).max() + 1e-300
     assert np.all(np.abs(y - ref) <= 1e-12 * scale + 1e-300), np.max(np.abs(y - ref) / scale)
     dyin = _dev(ref / (1.0 + np.abs(ref).max()))
     dxo = _dev(np.full(n, np.nan))
     _sync()
-    assert lib.sk_gpu_spmv_transpose_device_f64(C.byref(h), dyin.data_ptr(), dxo.data_ptr()) == 0
+    assert lib.sk_gpu_spmv_transpose_device_f64(C.byref(h), dyin.value, dxo.value) == 0
     _sync()
-    xt = dxo.cpu().numpy()
+    xt = dxo.numpy()
     scale_t = np.abs(A).sum(axis=0).A1 + 1e-300
     assert np.all(np.abs(xt - reft) <= 1e-12 * scale_t + 1e-300)
     lib.sk_gpu_csr_destroy(C.byref(h))

@@ -570,7 +570,7 @@ sk_status sk_gpu_solve(const sk_model *m, const sk_options *o, sk_gpu_options *g
 {
     sk_options od;
     sk_gpu_options gd;
-    hcsr A;
+    hcsr A, Qh;
     double *x, *y, kkt[5], t0 = skg_now(), eps;
     int j, rc, round;
     long long iters = 0;
@@ -587,9 +587,10 @@ sk_status sk_gpu_solve(const sk_model *m, const sk_options *o, sk_gpu_options *g
         return SK_OK;
     }
     for (j = 0; j < m->ncol; ++j) if (m->vartype[j] == SK_INTEGER) return SK_ERR_UNSUPPORTED;
-    if (m->Q) return SK_ERR_UNSUPPORTED;
     if (o->lp_engine == SK_LP_SIMPLEX) return SK_ERR_UNSUPPORTED;
     if (hcsr_from_csc(&m->A, &A)) return SK_ERR_MEMORY;
+    memset(&Qh, 0, sizeof(Qh));
+    if (m->Q) { Qh.rows = Qh.cols = m->ncol; Qh.nnz = m->Q->p[m->ncol]; Qh.p = m->Q->p; Qh.j = m->Q->i; Qh.v = m->Q->x; }
     x = (double *)calloc((size_t)m->ncol + 1, sizeof(double));
     y = (double *)calloc((size_t)m->nrow + 1, sizeof(double));
     if (!x || !y) { free(x); free(y); hcsr_free(&A); return SK_ERR_MEMORY; }
@@ -603,8 +604,11 @@ sk_status sk_gpu_solve(const sk_model *m, const sk_options *o, sk_gpu_options *g
         kkt[0] = kkt[1] = kkt[2] = INFINITY;
         for (round = 0; round < 4; ++round) {
             sk_gpu_pdlp_stats st;
-            rc = sk_gpu_pdlp_solve(m->nrow, m->ncol, A.nnz, A.p, A.j, A.v, NULL, NULL, NULL, 0, m->c,
-                                   m->rlow, m->rupp, m->clow, m->cupp, wx, wy, &P, x, y, &st);
+            /* the full symmetric Q's CSC arrays read as CSR are Q itself */
+            rc = sk_gpu_pdlp_solve(m->nrow, m->ncol, A.nnz, A.p, A.j, A.v, m->Q ? m->Q->p : NULL,
+                                   m->Q ? m->Q->i : NULL, m->Q ? m->Q->x : NULL,
+                                   m->Q ? m->Q->p[m->ncol] : 0, m->c, m->rlow, m->rupp, m->clow,
+                                   m->cupp, wx, wy, &P, x, y, &st);
             if (rc == -1) { free(x); free(y); hcsr_free(&A); return SK_ERR_NUMERIC; }
             iters += st.iterations;
             if (round == 0) g->stats = st;
@@ -620,7 +624,7 @@ sk_status sk_gpu_solve(const sk_model *m, const sk_options *o, sk_gpu_options *g
                 g->stats.converged = st.converged;
             }
             if (rc == -2) break;
-            if (relkkt_hcsr(&A, NULL, m->c, m->rlow, m->rupp, m->clow, m->cupp, x, y, kkt)) {
+            if (relkkt_hcsr(&A, m->Q ? &Qh : NULL, m->c, m->rlow, m->rupp, m->clow, m->cupp, x, y, kkt)) {
                 free(x); free(y); hcsr_free(&A); return SK_ERR_NUMERIC;
             }
             if (!st.converged) break;

@@ -1,0 +1,377 @@
+// Batched restarted PDHG: thousands of small LPs in one persistent launch.
+//
+// One workgroup owns one instance at a time and takes the next one from an
+// atomic work queue (instances ordered largest first, so the long ones start
+// early and the tail is made of short ones).  The iterates of an instance
+// live in LDS in FP32; the operators A and A^T (CSR, FP32 values for the
+// iteration, FP64 values for the checks) stay in global memory, where a
+// workgroup's few tens of KB remain L2-resident.  The whole PDLP control
+// loop of src/cuda/GpuPdlp.cpp:252-534 -- ||A|| by power iteration, the
+// adaptive step of PdlpKernels.cu:405-518, FP64 evaluation of the current
+// and averaged iterate every `period` iterations, restarts (0.2 / 0.8 /
+// 0.36) and the primal-weight update -- runs inside the workgroup with no
+// host involvement.  Every reduction is a fixed-shape workgroup reduction,
+// so an instance's result does not depend on which workgroup solved it.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "skg_batch.h"
+
+namespace {
+
+hipStream_t bstream()
+{
+    return static_cast<hipStream_t>(skk_stream());
+}
+
+template <int BT> __device__ __forceinline__ double wsum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Workgroup sum of S doubles; every thread receives the totals.
+template <int BT, int S>
+__device__ __forceinline__ void wg_sum(double (&v)[S], double *sh /* S * BT/64 */)
+{
+    constexpr int W = BT / 64;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const double t = wsum<BT>(v[s]);
+        if (lane == 0) sh[s * W + wave] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        double t = sh[s * W];
+#pragma unroll
+        for (int w = 1; w < W; ++w) t += sh[s * W + w];
+        v[s] = t;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+
+struct Inst {
+    int m, n;
+    const int *arp, *aci, *trp, *tci;      // CSR(A) and CSR(A^T), local indices
+    const float *av, *tv;
+    const double *avd, *tvd;
+    const float *c, *l, *u, *rl, *ru;
+    const double *cd, *ld, *ud, *rld, *rud;
+    double cost_norm, bound_norm;
+};
+
+__device__ Inst load_inst(const skb_batch &B, int k)
+{
+    Inst I;
+    const skb_inst &d = B.inst[k];
+    I.m = d.m; I.n = d.n;
+    I.arp = B.a_rp + d.a_rp; I.aci = B.a_ci + d.nz; I.av = B.a_v + d.nz; I.avd = B.a_vd + d.nz;
+    I.trp = B.t_rp + d.t_rp; I.tci = B.t_ci + d.nz; I.tv = B.t_v + d.nz; I.tvd = B.t_vd + d.nz;
+    I.c = B.c + d.col; I.l = B.l + d.col; I.u = B.u + d.col;
+    I.cd = B.cd + d.col; I.ld = B.ld + d.col; I.ud = B.ud + d.col;
+    I.rl = B.rl + d.row; I.ru = B.ru + d.row; I.rld = B.rld + d.row; I.rud = B.rud + d.row;
+    I.cost_norm = d.cost_norm; I.bound_norm = d.bound_norm;
+    return I;
+}
+
+struct Kkt { double primal, dual, gap, pmove, dmove, worst; };
+
+__device__ Kkt make_kkt(const double *s, double cost_norm, double bound_norm)
+{
+    Kkt k;
+    k.primal = sqrt(fmax(0.0, s[0])) / (1.0 + bound_norm);
+    k.dual = sqrt(fmax(0.0, s[1])) / (1.0 + cost_norm);
+    k.gap = fabs(s[2] - s[3]) / (1.0 + fabs(s[2]) + fabs(s[3]));
+    k.pmove = sqrt(fmax(0.0, s[6]));
+    k.dmove = sqrt(fmax(0.0, s[7]));
+    if (!isfinite(k.primal) || !isfinite(k.dual) || !isfinite(k.gap)) k.primal = k.dual = k.gap = INFINITY;
+    k.worst = fmax(fmax(k.primal, k.dual), k.gap);
+    return k;
+}
+
+// Eight residual sums of PdlpKernels.cu:101-209 in FP64 for the point
+// (x * xs, y * ys) -- xs = ys = 1 for the iterate, 1/terms for the average.
+template <int BT>
+__device__ void residual_sums(const Inst &I, const float *x, const float *y, const float *xr,
+                              const float *yr, double xs, double ys, double *sh, double (&out)[8])
+{
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int j = threadIdx.x; j < I.n; j += BT) {
+        double aty = 0.0;
+        for (int p = I.trp[j]; p < I.trp[j + 1]; ++p) aty += I.tvd[p] * (ys * (double)y[I.tci[p]]);
+        const double xj = xs * (double)x[j], lo = I.ld[j], hi = I.ud[j], r = I.cd[j] + aty;
+        const double dx = xj - (double)xr[j];
+        acc[2] += I.cd[j] * xj;
+        acc[4] += xj * xj;
+        acc[6] += dx * dx;
+        double viol = 0.0, term = 0.0;
+        const bool lf = isfinite(lo), hf = isfinite(hi);
+        if (lf && hf) term = fmin(r * lo, r * hi);
+        else if (lf) { viol = fmax(0.0, -r); term = fmax(0.0, r) * lo; }
+        else if (hf) { viol = fmax(0.0, r); term = fmin(0.0, r) * hi; }
+        else viol = fabs(r);
+        acc[1] += viol * viol;
+        acc[3] += term;
+    }
+    for (int i = threadIdx.x; i < I.m; i += BT) {
+        double ax = 0.0;
+        for (int p = I.arp[i]; p < I.arp[i + 1]; ++p) ax += I.avd[p] * (xs * (double)x[I.aci[p]]);
+        const double rl = I.rld[i], ru = I.rud[i];
+        const double pr = ax - fmin(fmax(ax, rl), ru);
+        const double yi = ys * (double)y[i], dy = yi - (double)yr[i];
+        acc[0] += pr * pr;
+        acc[5] += yi * yi;
+        acc[7] += dy * dy;
+        double support = 0.0;
+        if (yi > 0.0) support = isfinite(ru) ? yi * ru : 0.0;
+        else if (yi < 0.0) support = isfinite(rl) ? yi * rl : 0.0;
+        acc[3] -= support;
+    }
+    wg_sum<BT, 8>(acc, sh);
+    for (int s = 0; s < 8; ++s) out[s] = acc[s];
+}
+
+// y = A x (FP32, LDS vectors), one thread per row.
+template <int BT>
+__device__ __forceinline__ void spmv_rows(int rows, const int *rp, const int *ci, const float *v,
+                                          const float *x, float *y)
+{
+    for (int i = threadIdx.x; i < rows; i += BT) {
+        float s = 0.0f;
+        for (int p = rp[i]; p < rp[i + 1]; ++p) s += v[p] * x[ci[p]];
+        y[i] = s;
+    }
+}
+
+template <int BT>
+__device__ void solve_instance(const skb_batch &B, int k, float *lds, double *sh)
+{
+    const Inst I = load_inst(B, k);
+    const int m = I.m, n = I.n;
+    const skb_params &P = B.params;
+    // LDS layout: 7 column vectors, 9 row vectors
+    float *X0 = lds, *X1 = X0 + n, *xsum = X1 + n, *xr = xsum + n, *cl = xr + n, *ll = cl + n, *ul = ll + n;
+    float *Y0 = ul + n, *Y1 = Y0 + m, *AX0 = Y1 + m, *AX1 = AX0 + m, *AX2 = AX1 + m, *ysum = AX2 + m,
+          *yr = ysum + m, *rll = yr + m, *rul = rll + m;
+    for (int j = threadIdx.x; j < n; j += BT) { cl[j] = I.c[j]; ll[j] = I.l[j]; ul[j] = I.u[j]; }
+    for (int i = threadIdx.x; i < m; i += BT) { rll[i] = I.rl[i]; rul[i] = I.ru[i]; }
+
+    // ||A||_2: power iteration on A'A from v = 1/sqrt(n) (GpuPdlp.cpp:179-211)
+    double norm_sq = 0.0;
+    {
+        const float v0 = (float)(1.0 / sqrt((double)(n > 0 ? n : 1)));
+        for (int j = threadIdx.x; j < n; j += BT) X0[j] = v0;
+        __syncthreads();
+        for (int it = 0; it < P.power_iterations; ++it) {
+            spmv_rows<BT>(m, I.arp, I.aci, I.av, X0, Y0);
+            __syncthreads();
+            double acc[1] = {0.0};
+            for (int j = threadIdx.x; j < n; j += BT) {
+                float s = 0.0f;
+                for (int p = I.trp[j]; p < I.trp[j + 1]; ++p) s += I.tv[p] * Y0[I.tci[p]];
+                X1[j] = s;
+                acc[0] += (double)s * (double)s;
+            }
+            wg_sum<BT, 1>(acc, sh);
+            norm_sq = acc[0];
+            const double nrm = sqrt(norm_sq);
+            const float inv = nrm > 0.0 ? (float)(1.0 / nrm) : 1.0f;
+            for (int j = threadIdx.x; j < n; j += BT) X0[j] = X1[j] * inv;
+            __syncthreads();
+        }
+    }
+    double anorm = sqrt(sqrt(fmax(0.0, norm_sq)));
+    if (!(anorm > 0.0) || !isfinite(anorm)) anorm = 1.0;
+    const double eta_min = P.eta_min_factor / anorm, eta_max = P.eta_max_factor / anorm;
+    double eta = 0.9 / anorm, omega = 1.0, updates = 0.0, accepted = 0.0, accepted_at_restart = 0.0;
+    if (I.cost_norm > 0.0 && I.bound_norm > 0.0) omega = fmin(1e6, fmax(1e-6, I.cost_norm / I.bound_norm));
+
+    // cold start (GpuPdlp.cpp:278-291)
+    for (int j = threadIdx.x; j < n; j += BT) { X0[j] = 0.0f; X1[j] = 0.0f; xsum[j] = 0.0f; xr[j] = 0.0f; }
+    for (int i = threadIdx.x; i < m; i += BT) {
+        Y0[i] = 0.0f; Y1[i] = 0.0f; AX0[i] = 0.0f; AX1[i] = 0.0f; AX2[i] = 0.0f; ysum[i] = 0.0f; yr[i] = 0.0f;
+    }
+    __syncthreads();
+    float *x = X0, *xc = X1, *y = Y0, *yc = Y1, *ax = AX0, *axp = AX1, *axc = AX2;
+
+    double mu_at_restart = INFINITY, mu_prev = INFINITY;
+    int iteration = 0, since_restart = 0, restarts = 0, converged = 0, final_avg = 0;
+    Kkt best{INFINITY, INFINITY, INFINITY, 0, 0, INFINITY};
+    while (iteration < P.max_iterations) {
+        const int period = min(P.restart_period, P.max_iterations - iteration);
+        for (int t = 0; t < period; ++t) {
+            const float sigma = (float)(eta * omega), tau = (float)(eta / omega);
+            for (int i = threadIdx.x; i < m; i += BT) {
+                const float v = y[i] + sigma * (2.0f * ax[i] - axp[i]);
+                yc[i] = v - sigma * clampf(v / sigma, rll[i], rul[i]);
+            }
+            __syncthreads();
+            double acc[3] = {0.0, 0.0, 0.0};
+            for (int j = threadIdx.x; j < n; j += BT) {
+                float s = 0.0f;
+                for (int p = I.trp[j]; p < I.trp[j + 1]; ++p) s += I.tv[p] * yc[I.tci[p]];
+                const float xn = clampf(x[j] - tau * (cl[j] + s), ll[j], ul[j]);
+                xc[j] = xn;
+                const double d = (double)xn - (double)x[j];
+                acc[0] += d * d;
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < m; i += BT) {
+                float s = 0.0f;
+                for (int p = I.arp[i]; p < I.arp[i + 1]; ++p) s += I.av[p] * xc[I.aci[p]];
+                axc[i] = s;
+                const double dy = (double)yc[i] - (double)y[i];
+                acc[1] += dy * dy;
+                acc[2] += dy * ((double)s - (double)ax[i]);
+            }
+            wg_sum<BT, 3>(acc, sh);
+            const double inter = fabs(acc[2]);
+            const double movement = 0.5 * (omega * acc[0] + acc[1] / omega);
+            const double eta_bar = (inter > 0.0 && movement > 0.0) ? movement / inter : eta_max;
+            const bool accept = eta <= eta_bar;
+            const double kk = updates + 1.0;
+            double next = fmin((1.0 - pow(kk + 1.0, -0.3)) * eta_bar, (1.0 + pow(kk + 1.0, -0.6)) * eta);
+            if (!(next > 0.0) || !isfinite(next)) next = eta;
+            eta = fmin(fmax(next, eta_min), eta_max);
+            updates = kk;
+            if (accept) {
+                accepted += 1.0;
+                float *t0 = x; x = xc; xc = t0;
+                t0 = y; y = yc; yc = t0;
+                t0 = axp; axp = ax; ax = axc; axc = t0;
+                for (int j = threadIdx.x; j < n; j += BT) xsum[j] += x[j];
+                for (int i = threadIdx.x; i < m; i += BT) ysum[i] += y[i];
+                __syncthreads();
+            }
+        }
+        iteration += period;
+        since_restart += period;
+        // FP64 evaluation of the iterate and of the running average
+        double sc[8], sa[8];
+        residual_sums<BT>(I, x, y, xr, yr, 1.0, 1.0, sh, sc);
+        double terms = accepted - accepted_at_restart;
+        if (terms < 1.0) terms = 1.0;
+        residual_sums<BT>(I, xsum, ysum, xr, yr, 1.0 / terms, 1.0 / terms, sh, sa);
+        const Kkt cur = make_kkt(sc, I.cost_norm, I.bound_norm);
+        const Kkt avg = make_kkt(sa, I.cost_norm, I.bound_norm);
+        const bool avg_better = avg.worst < cur.worst;
+        best = avg_better ? avg : cur;
+        final_avg = avg_better;
+        if (best.primal <= P.eps && best.dual <= P.eps && best.gap <= P.eps) { converged = 1; break; }
+        const double mu = best.worst;
+        const bool sufficient = mu <= 0.2 * mu_at_restart;
+        const bool necessary = mu <= 0.8 * mu_at_restart && mu > mu_prev;
+        const bool artificial = (double)since_restart >= 0.36 * (double)iteration;
+        mu_prev = mu;
+        if (!(sufficient || necessary || artificial)) continue;
+        // restart (GpuPdlp.cpp:453-498)
+        const float inv = (float)(1.0 / terms);
+        if (avg_better) {
+            for (int j = threadIdx.x; j < n; j += BT) x[j] = xsum[j] * inv;
+            for (int i = threadIdx.x; i < m; i += BT) y[i] = ysum[i] * inv;
+            __syncthreads();
+        }
+        spmv_rows<BT>(m, I.arp, I.aci, I.av, x, ax);
+        __syncthreads();
+        for (int i = threadIdx.x; i < m; i += BT) { axp[i] = ax[i]; ysum[i] = 0.0f; yr[i] = y[i]; }
+        for (int j = threadIdx.x; j < n; j += BT) { xsum[j] = 0.0f; xr[j] = x[j]; }
+        __syncthreads();
+        accepted_at_restart = accepted;
+        since_restart = 0;
+        mu_at_restart = mu;
+        mu_prev = INFINITY;
+        ++restarts;
+        final_avg = 0;
+        if (best.pmove > 1e-14 && best.dmove > 1e-14) {
+            const double upd = exp(0.5 * log(best.dmove / best.pmove) + 0.5 * log(omega));
+            if (isfinite(upd)) omega = fmin(1e6, fmax(1e-6, upd));
+        }
+    }
+    // scaled-space result in FP64 (the host unscales)
+    const skb_inst &d = B.inst[k];
+    const double inv = 1.0 / fmax(1.0, accepted - accepted_at_restart);
+    for (int j = threadIdx.x; j < n; j += BT) B.x_out[d.col + j] = final_avg ? (double)xsum[j] * inv : (double)x[j];
+    for (int i = threadIdx.x; i < m; i += BT) B.y_out[d.row + i] = final_avg ? (double)ysum[i] * inv : (double)y[i];
+    if (threadIdx.x == 0) {
+        skb_out &o = B.out[k];
+        o.iterations = iteration;
+        o.restarts = restarts;
+        o.converged = converged;
+        o.rel_primal = best.primal;
+        o.rel_dual = best.dual;
+        o.rel_gap = best.gap;
+        o.matrix_norm = anorm;
+        o.final_step = eta;
+    }
+    __syncthreads();
+}
+
+template <int BT> __global__ __launch_bounds__(BT) void k_batch_pdlp(skb_batch B)
+{
+    extern __shared__ float lds[];
+    __shared__ double sh[8 * (BT / 64)];
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0)
+            next = (int)__hip_atomic_fetch_add(B.queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int q = next;
+        __syncthreads();
+        if (q >= B.count) break;       // every wave of the workgroup leaves together
+        solve_instance<BT>(B, B.order[q], lds, sh);
+    }
+}
+
+__global__ void k_queue_reset(unsigned *queue)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *queue = 0u;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t skb_lds_bytes(int max_m, int max_n)
+{
+    return sizeof(float) * (7 * (size_t)max_n + 9 * (size_t)max_m);
+}
+
+int skb_launch(const skb_batch *B, int block_threads, int grid)
+{
+    const size_t lds = skb_lds_bytes(B->max_m, B->max_n);
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, bstream(), B->queue);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+    switch (block_threads) {
+    case 64:  hipLaunchKernelGGL(k_batch_pdlp<64>, dim3(grid), dim3(64), lds, bstream(), *B); break;
+    case 128: hipLaunchKernelGGL(k_batch_pdlp<128>, dim3(grid), dim3(128), lds, bstream(), *B); break;
+    case 256: hipLaunchKernelGGL(k_batch_pdlp<256>, dim3(grid), dim3(256), lds, bstream(), *B); break;
+    case 512: hipLaunchKernelGGL(k_batch_pdlp<512>, dim3(grid), dim3(512), lds, bstream(), *B); break;
+    default: return -1;
+    }
+    return (int)hipGetLastError();
+}
+
+int skb_max_lds(void)
+{
+    int dev = 0, v = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 65536;
+    return v;
+}
+
+int skb_cu_count(void)
+{
+    int dev = 0, v = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 256;
+    return v;
+}
+
+}  // extern "C"

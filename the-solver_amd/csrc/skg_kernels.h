@@ -87,15 +87,18 @@ typedef struct skk_state {
 typedef struct skk_iter {
     void *X[2], *Y[2], *AX[3];
     void *xsum, *ysum, *xr, *yr, *xavg, *yavg, *xbar;
+    void *QX[2];             /* QX[cur] = Q X[cur] (convex QP only)          */
 } skk_iter;
 
 typedef struct skk_pdhg {
     int m, n;
     skk_csr A, AT;                           /* AT is CSR(A^T)              */
+    skk_csr Q;                               /* full symmetric Hessian      */
+    int has_q;
     const float *cf, *lf, *uf, *rlf, *ruf;   /* FP32 copies of the data     */
     const double *cd, *ld, *ud, *rld, *rud;  /* FP64 data                   */
     skk_iter it[2];                          /* [0] FP64, [1] FP32          */
-    double *ax_d, *aty_d;                    /* FP64 products for checks    */
+    double *ax_d, *aty_d, *qx_d;             /* FP64 products for checks    */
     skk_state *st;
     double *part;                            /* block partials              */
     unsigned *counters;                      /* self-resetting tickets      */
@@ -110,6 +113,14 @@ int skk_pdhg_init(const skk_pdhg *P, int prec, double eta0, double omega, double
                   double eta_max, int warm_x, int warm_y);
 int skk_pdhg_adaptive_iter(const skk_pdhg *P, int prec);
 int skk_pdhg_fixed_iter(const skk_pdhg *P, int prec);
+/* One of the three launches of an adaptive iteration: 0 dual prox,
+ * 1 A^T y_cand + primal prox, 2 A x_cand + step decision. */
+int skk_pdhg_adaptive_part(const skk_pdhg *P, int prec, int part);
+/* Event timeline on the library stream: reserve, mark slot k, read the
+ * milliseconds between two marked slots (waits for the later one). */
+int    skk_timeline_reserve(int count);
+int    skk_timeline_mark(int k);
+double skk_timeline_ms(int a, int b);
 int skk_pdhg_flush(const skk_pdhg *P, int prec);
 /* which 0: current iterate, 1: running average (formed here). */
 int skk_pdhg_eval(const skk_pdhg *P, int prec, int which, int adaptive, double fixed_terms);

@@ -267,7 +267,7 @@ int launch_spmv(const skk_csr &A, const VT *vals, const XT *x0, const XT *x1,
 // ------------------------------------------------------------ PDHG state
 
 template <typename T> struct Iter {
-    T *X[2], *Y[2], *AX[3], *xsum, *ysum, *xr, *yr, *xavg, *yavg, *xbar;
+    T *X[2], *Y[2], *AX[3], *xsum, *ysum, *xr, *yr, *xavg, *yavg, *xbar, *QX[2];
     __host__ __device__ Iter() {}
     explicit Iter(const skk_iter &it)
     {
@@ -277,6 +277,7 @@ template <typename T> struct Iter {
         xr = static_cast<T *>(it.xr); yr = static_cast<T *>(it.yr);
         xavg = static_cast<T *>(it.xavg); yavg = static_cast<T *>(it.yavg);
         xbar = static_cast<T *>(it.xbar);
+        QX[0] = static_cast<T *>(it.QX[0]); QX[1] = static_cast<T *>(it.QX[1]);
     }
 };
 
@@ -293,7 +294,7 @@ template <> const float *vals_of<float>(const skk_csr &A) { return A.vf; }
 template <> const double *vals_of<double>(const skk_csr &A) { return A.vd; }
 
 // Partial-slot regions (each part_capacity doubles).
-enum { R_DX = 0, R_DY = 1, R_INTER = 2, R_EVAL = 3, R_POWER = 11, R_VERIFY = 12 };
+enum { R_DX = 0, R_DY = 1, R_INTER = 2, R_EVAL = 3, R_POWER = 11, R_VERIFY = 12, R_DQ = 13 };
 enum { C_PROBE = 0, C_EVAL = 1, C_POWER = 2, C_VERIFY = 3, C_KKT = 4 };
 
 __global__ void k_init_state(skk_state *st, double eta0, double omega, double eta_min,
@@ -335,6 +336,7 @@ __global__ void k_init_vectors(Iter<T> I, int n, int m, int warm_x, int warm_y)
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
         const T x = warm_x ? I.X[0][j] : T(0);
         I.X[0][j] = x; I.X[1][j] = x; I.xsum[j] = T(0); I.xr[j] = x; I.xbar[j] = x; I.xavg[j] = x;
+        if (I.QX[0]) { I.QX[0][j] = T(0); I.QX[1][j] = T(0); }
     }
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
         const T y = warm_y ? I.Y[0][i] : T(0);
@@ -382,7 +384,7 @@ template <typename T, typename AT> struct EpiPrimalAdaptive {
     double *part;
     int stride;
     // per-launch values
-    const T *x;
+    const T *x, *qx;
     T *xc;
     T tau;
     int pending;
@@ -390,13 +392,15 @@ template <typename T, typename AT> struct EpiPrimalAdaptive {
     {
         const int cur = st->cur;
         x = I.X[cur]; xc = I.X[cur ^ 1];
+        qx = I.QX[0] ? I.QX[cur] : nullptr;
         tau = static_cast<T>(st->eta / st->omega);
         pending = st->pending;
     }
     __device__ void row(int j, AT s, double *p) const
     {
         const T xj = x[j];
-        const T xn = clampv(xj - tau * (D.c[j] + static_cast<T>(s)), D.l[j], D.u[j]);
+        const T g = D.c[j] + static_cast<T>(s) + (qx ? qx[j] : T(0));
+        const T xn = clampv(xj - tau * g, D.l[j], D.u[j]);
         xc[j] = xn;
         if (pending) I.xsum[j] += xj;
         const double d = static_cast<double>(xn) - static_cast<double>(xj);
@@ -421,6 +425,7 @@ template <typename T, typename AT> struct EpiProbe {
     int stride;         // part_capacity
     unsigned *counter;
     int dx_blocks;      // workgroups of the A^T launch
+    int dq_blocks;      // workgroups of the Q launch (0: no Hessian)
     const T *y, *yc, *ax;
     T *axc;
     __device__ void begin()
@@ -441,9 +446,11 @@ template <typename T, typename AT> struct EpiProbe {
         double v[2] = {p[0], p[1]}, tot[2];
         if (!partials_and_total<2>(v, part + (size_t)R_DY * stride, stride, counter, tot)) return;
         const double dx2 = total_of(part + (size_t)R_DX * stride, dx_blocks);
+        const double dq = dq_blocks > 0 ? total_of(part + (size_t)R_DQ * stride, dq_blocks) : 0.0;
         if (threadIdx.x != 0) return;
         const double dy2 = tot[0];
-        const double inter = fabs(tot[1]);
+        // with a Hessian the local bound also covers dx'Q dx (>= 0 for convex Q)
+        const double inter = fabs(tot[1]) + 0.5 * fmax(dq, 0.0);
         const double omega = st->omega, eta = st->eta;
         const double movement = 0.5 * (omega * dx2 + dy2 / omega);
         const double eta_bar = (inter > 0.0 && movement > 0.0) ? movement / inter : st->eta_max;
@@ -466,6 +473,34 @@ template <typename T, typename AT> struct EpiProbe {
             const int a0 = st->a0, a1 = st->a1, a2 = st->a2;
             st->a0 = a2; st->a1 = a0; st->a2 = a1;   // x_prev <- x, x <- cand
         }
+    }
+};
+
+// Q x_cand epilogue: QX[cur^1] = Q x_cand; partial dx'(Q x_cand - Q x).
+template <typename T, typename AT> struct EpiQProbe {
+    static constexpr int kSlots = 1;
+    Iter<T> I;
+    const skk_state *st;
+    double *part;
+    const T *x, *xc, *qx;
+    T *qxc;
+    __device__ void begin()
+    {
+        const int cur = st->cur;
+        x = I.X[cur]; xc = I.X[cur ^ 1]; qx = I.QX[cur]; qxc = I.QX[cur ^ 1];
+    }
+    __device__ void row(int j, AT s, double *p) const
+    {
+        const T q = static_cast<T>(s);
+        qxc[j] = q;
+        p[0] += (static_cast<double>(xc[j]) - static_cast<double>(x[j])) *
+                (static_cast<double>(q) - static_cast<double>(qx[j]));
+    }
+    __device__ void end(double *p) const
+    {
+        double v[1] = {p[0]}, out[1];
+        block_reduce<1>(v, out);
+        if (threadIdx.x == 0) store_coherent(part + blockIdx.x, out[0]);
     }
 };
 
@@ -557,7 +592,8 @@ __global__ __launch_bounds__(kBlock) void k_residuals_sel(Iter<T> I, int which,
                                                           Data<double> D, int n, int m,
                                                           double *part, int stride,
                                                           unsigned *counter, double *out,
-                                                          const skk_state *__restrict__ st)
+                                                          const skk_state *__restrict__ st,
+                                                          const double *__restrict__ qx)
 {
     const int cur = st->cur;
     const T *__restrict__ x = which ? I.xavg : I.X[cur];
@@ -567,9 +603,12 @@ __global__ __launch_bounds__(kBlock) void k_residuals_sel(Iter<T> I, int which,
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const int gs = gridDim.x * blockDim.x;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gs) {
-        const double xj = x[j], lo = D.l[j], hi = D.u[j], r = D.c[j] + aty[j];
+        const double q = qx ? qx[j] : 0.0;
+        const double xj = x[j], lo = D.l[j], hi = D.u[j], r = D.c[j] + aty[j] + q;
         const double dx = xj - static_cast<double>(xr[j]);
-        acc[2] += D.c[j] * xj;
+        // with Q: primal c'x + x'Qx/2, dual terms of r = c + Qx + A'y minus x'Qx/2
+        acc[2] += D.c[j] * xj + 0.5 * xj * q;
+        acc[3] -= 0.5 * xj * q;
         acc[4] += xj * xj;
         acc[6] += dx * dx;
         double viol = 0.0, term = 0.0;
@@ -602,24 +641,28 @@ __global__ __launch_bounds__(kBlock) void k_residuals_sel(Iter<T> I, int which,
 // AX[a0] <- A x from the FP64 product of the evaluation.
 template <typename T>
 __global__ void k_to_t_sel(Iter<T> I, const double *__restrict__ src, int m,
-                           const skk_state *__restrict__ st)
+                           const skk_state *__restrict__ st, const double *__restrict__ qsrc, int n)
 {
     T *dst = I.AX[st->a0];
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) dst[i] = static_cast<T>(src[i]);
+    if (qsrc) {
+        T *q = I.QX[st->cur];
+        for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) q[j] = static_cast<T>(qsrc[j]);
+    }
 }
 
 // Restart (GpuPdlp.cpp:453-498 minus the primal-weight update, which the
 // host applies through k_set_omega).
 template <typename T>
 __global__ void k_restart(Iter<T> I, skk_state *st, const double *__restrict__ ax_avg, int n, int m,
-                          int use_avg, int adaptive)
+                          int use_avg, int adaptive, const double *__restrict__ qx_avg)
 {
     const int cur = st->cur, a0 = st->a0, a1 = st->a1;
     const int stride = gridDim.x * blockDim.x;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
         T x = I.X[cur][j];
-        if (use_avg) { x = I.xavg[j]; I.X[cur][j] = x; }
+        if (use_avg) { x = I.xavg[j]; I.X[cur][j] = x; if (qx_avg) I.QX[cur][j] = static_cast<T>(qx_avg[j]); }
         I.xbar[j] = x;
         I.xsum[j] = T(0);
         I.xr[j] = x;
@@ -645,6 +688,7 @@ __global__ void k_promote(Iter<double> D, Iter<float> F, int n, int m)
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
         D.X[0][j] = F.X[0][j]; D.X[1][j] = F.X[1][j];
         D.xsum[j] = F.xsum[j]; D.xr[j] = F.xr[j]; D.xbar[j] = F.xbar[j]; D.xavg[j] = F.xavg[j];
+        if (D.QX[0]) { D.QX[0][j] = F.QX[0][j]; D.QX[1][j] = F.QX[1][j]; }
     }
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
         D.Y[0][i] = F.Y[0][i]; D.Y[1][i] = F.Y[1][i];
@@ -960,6 +1004,13 @@ int grid_for(int n, int cap)
 
 template <typename T> constexpr int prec_index() { return std::is_same<T, float>::value ? SKK_F32 : SKK_F64; }
 
+template <typename T> int q_probe(const skk_pdhg &P, const Iter<T> &I)
+{
+    EpiQProbe<T, T> eq{};
+    eq.I = I; eq.st = P.st; eq.part = P.part + (size_t)R_DQ * P.part_capacity;
+    return launch_spmv<T, T, T>(P.Q, vals_of<T>(P.Q), I.X[0], I.X[1], P.st, eq);
+}
+
 template <typename T> int adaptive_iter(const skk_pdhg &P)
 {
     const Iter<T> I(P.it[prec_index<T>()]);
@@ -974,10 +1025,37 @@ template <typename T> int adaptive_iter(const skk_pdhg &P)
     ep.stride = P.part_capacity;
     // A^T y_cand: y_cand is Y[cur ^ 1], picked on the device
     if ((rc = launch_spmv<T, T, T>(P.AT, vals_of<T>(P.AT), I.Y[0], I.Y[1], P.st, ep))) return rc;
+    if (P.has_q && (rc = q_probe<T>(P, I))) return rc;
     EpiProbe<T, T> pr{};
     pr.I = I; pr.st = P.st; pr.part = P.part; pr.stride = P.part_capacity;
     pr.counter = P.counters + C_PROBE; pr.dx_blocks = P.AT.bins.total_blocks;
+    pr.dq_blocks = P.has_q ? P.Q.bins.total_blocks : 0;
     // A x_cand: x_cand is X[cur ^ 1]
+    return launch_spmv<T, T, T>(P.A, vals_of<T>(P.A), I.X[0], I.X[1], P.st, pr);
+}
+
+template <typename T> int adaptive_part(const skk_pdhg &P, int part)
+{
+    const Iter<T> I(P.it[prec_index<T>()]);
+    const Data<T> D = data_of<T>(P);
+    if (part == 0) {
+        hipLaunchKernelGGL(k_dual_adaptive<T>, dim3(P.elem_grid), dim3(kBlock), 0, stream(), I, D,
+                           P.st, P.m);
+        return launch_status();
+    }
+    if (part == 1) {
+        EpiPrimalAdaptive<T, T> ep{};
+        ep.I = I; ep.D = D; ep.st = P.st;
+        ep.part = P.part + (size_t)R_DX * P.part_capacity;
+        ep.stride = P.part_capacity;
+        return launch_spmv<T, T, T>(P.AT, vals_of<T>(P.AT), I.Y[0], I.Y[1], P.st, ep);
+    }
+    int rc;
+    if (P.has_q && (rc = q_probe<T>(P, I))) return rc;
+    EpiProbe<T, T> pr{};
+    pr.I = I; pr.st = P.st; pr.part = P.part; pr.stride = P.part_capacity;
+    pr.counter = P.counters + C_PROBE; pr.dx_blocks = P.AT.bins.total_blocks;
+    pr.dq_blocks = P.has_q ? P.Q.bins.total_blocks : 0;
     return launch_spmv<T, T, T>(P.A, vals_of<T>(P.A), I.X[0], I.X[1], P.st, pr);
 }
 
@@ -1029,15 +1107,20 @@ template <typename T> int eval(const skk_pdhg &P, int which, int adaptive, doubl
     if ((rc = launch_spmv<double, T, double>(P.A, P.A.vd, x, x1, sel, sa))) return rc;
     EpiStore<double, double> st{P.aty_d};
     if ((rc = launch_spmv<double, T, double>(P.AT, P.AT.vd, y, y1, sel, st))) return rc;
+    if (P.has_q) {
+        EpiStore<double, double> sq{P.qx_d};
+        if ((rc = launch_spmv<double, T, double>(P.Q, P.Q.vd, x, x1, sel, sq))) return rc;
+    }
     const Data<double> D = data_of<double>(P);
     hipLaunchKernelGGL(k_residuals_sel<T>, dim3(P.elem_grid), dim3(kBlock), 0, stream(), I, which,
                        P.ax_d, P.aty_d, D, P.n, P.m, P.part + (size_t)R_EVAL * P.part_capacity,
-                       P.part_capacity, P.counters + C_EVAL, P.eval_out + 8 * which, P.st);
+                       P.part_capacity, P.counters + C_EVAL, P.eval_out + 8 * which, P.st,
+                       P.has_q ? P.qx_d : nullptr);
     if ((rc = launch_status())) return rc;
     if (!which && adaptive) {
-        // the live A x is refreshed from the FP64 product (GpuPdlp.cpp:367)
+        // the live A x (and Q x) are refreshed from the FP64 products (GpuPdlp.cpp:367)
         hipLaunchKernelGGL(k_to_t_sel<T>, dim3(P.elem_grid), dim3(kBlock), 0, stream(), I, P.ax_d,
-                           P.m, P.st);
+                           P.m, P.st, P.has_q ? P.qx_d : nullptr, P.n);
         rc = launch_status();
     }
     return rc;
@@ -1047,7 +1130,7 @@ template <typename T> int restart(const skk_pdhg &P, int use_avg, int adaptive)
 {
     const Iter<T> I(P.it[prec_index<T>()]);
     hipLaunchKernelGGL(k_restart<T>, dim3(P.elem_grid), dim3(kBlock), 0, stream(), I, P.st, P.ax_d,
-                       P.n, P.m, use_avg, adaptive);
+                       P.n, P.m, use_avg, adaptive, P.has_q ? P.qx_d : nullptr);
     return launch_status();
 }
 
@@ -1206,11 +1289,21 @@ int skk_pdhg_init(const skk_pdhg *P, int prec, double eta0, double omega, double
             EpiStore<float, float> e{I.AX[0]};
             if ((rc = launch_spmv<float, float, float>(P->A, P->A.vf, I.X[0], I.X[0], nullptr, e))) return rc;
             hipLaunchKernelGGL(k_copy<float>, dim3(P->elem_grid), dim3(kBlock), 0, stream(), I.AX[1], I.AX[0], P->m);
+            if (P->has_q) {
+                EpiStore<float, float> q{I.QX[0]};
+                if ((rc = launch_spmv<float, float, float>(P->Q, P->Q.vf, I.X[0], I.X[0], nullptr, q))) return rc;
+                hipLaunchKernelGGL(k_copy<float>, dim3(P->elem_grid), dim3(kBlock), 0, stream(), I.QX[1], I.QX[0], P->n);
+            }
         } else {
             Iter<double> I(P->it[SKK_F64]);
             EpiStore<double, double> e{I.AX[0]};
             if ((rc = launch_spmv<double, double, double>(P->A, P->A.vd, I.X[0], I.X[0], nullptr, e))) return rc;
             hipLaunchKernelGGL(k_copy<double>, dim3(P->elem_grid), dim3(kBlock), 0, stream(), I.AX[1], I.AX[0], P->m);
+            if (P->has_q) {
+                EpiStore<double, double> q{I.QX[0]};
+                if ((rc = launch_spmv<double, double, double>(P->Q, P->Q.vd, I.X[0], I.X[0], nullptr, q))) return rc;
+                hipLaunchKernelGGL(k_copy<double>, dim3(P->elem_grid), dim3(kBlock), 0, stream(), I.QX[1], I.QX[0], P->n);
+            }
         }
         rc = launch_status();
     }
@@ -1220,6 +1313,50 @@ int skk_pdhg_init(const skk_pdhg *P, int prec, double eta0, double omega, double
 int skk_pdhg_adaptive_iter(const skk_pdhg *P, int prec)
 {
     return prec == SKK_F32 ? adaptive_iter<float>(*P) : adaptive_iter<double>(*P);
+}
+
+int skk_pdhg_adaptive_part(const skk_pdhg *P, int prec, int part)
+{
+    return prec == SKK_F32 ? adaptive_part<float>(*P, part) : adaptive_part<double>(*P, part);
+}
+
+static hipEvent_t *g_timeline[kMaxDevices] = {};
+static int g_timeline_n[kMaxDevices] = {};
+
+int skk_timeline_reserve(int count)
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev < 0 || dev >= kMaxDevices || count < 0) return -1;
+    if (count <= g_timeline_n[dev]) return 0;
+    hipEvent_t *ev = static_cast<hipEvent_t *>(realloc(g_timeline[dev], sizeof(hipEvent_t) * count));
+    if (!ev) return -1;
+    g_timeline[dev] = ev;
+    for (int k = g_timeline_n[dev]; k < count; ++k) {
+        if (check(hipEventCreate(&ev[k]))) { g_timeline_n[dev] = k; return g_last; }
+    }
+    g_timeline_n[dev] = count;
+    return 0;
+}
+
+int skk_timeline_mark(int k)
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev < 0 || dev >= kMaxDevices || k < 0 || k >= g_timeline_n[dev]) return -1;
+    return check(hipEventRecord(g_timeline[dev][k], stream()));
+}
+
+double skk_timeline_ms(int a, int b)
+{
+    int dev = 0;
+    float ms = 0.0f;
+    (void)hipGetDevice(&dev);
+    if (dev < 0 || dev >= kMaxDevices || a < 0 || b < 0 || a >= g_timeline_n[dev] || b >= g_timeline_n[dev])
+        return -1.0;
+    if (hipEventSynchronize(g_timeline[dev][b]) != hipSuccess) return -1.0;
+    if (hipEventElapsedTime(&ms, g_timeline[dev][a], g_timeline[dev][b]) != hipSuccess) return -1.0;
+    return static_cast<double>(ms);
 }
 
 int skk_pdhg_fixed_iter(const skk_pdhg *P, int prec)

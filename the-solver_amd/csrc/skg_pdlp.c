@@ -59,7 +59,7 @@ static double finite_norm(const double *v, int n)       /* GpuPdlp.cpp:84-90 */
 /* Ruiz equilibration, src/lp/Scaling.cpp:11-92: simultaneous row/column
  * division by sqrt of the current inf-norms until every nonzero norm is
  * within tol of 1; identity if that does not happen within max_iters. */
-static void ruiz(const hcsr *a, const hcsr *at, int max_iters, double tol, double *R, double *C)
+void skg_ruiz(const hcsr *a, const hcsr *at, int max_iters, double tol, double *R, double *C)
 {
     const int m = a->rows, n = a->cols;
     int i, j, k, iter, converged = 0;
@@ -105,7 +105,7 @@ static void ruiz(const hcsr *a, const hcsr *at, int max_iters, double tol, doubl
  * src/lp/Scaling.cpp:111-145: R_i = 1/sqrt(sum_j |a'_ij|),
  * C_j = 1/sqrt(sum_i |a'_ij|); empty rows/columns keep 1.  Multiplies the
  * result into R0, C0 (compose_scaling, Scaling.cpp:147-152). */
-static void pock_chambolle_into(const hcsr *a, double *R, double *C)
+void skg_pock_chambolle_into(const hcsr *a, double *R, double *C)
 {
     const int m = a->rows, n = a->cols;
     int i, j, k;
@@ -128,7 +128,8 @@ static void pock_chambolle_into(const hcsr *a, double *R, double *C)
 
 typedef struct engine {
     int m, n, mixed;
-    dcsr A, AT;
+    dcsr A, AT, Q;
+    int has_q;
     int *perm_r, *perm_c;        /* slot -> original index */
     double *R, *C;               /* scale factors, original indexing */
     double cost_norm, bound_norm;
@@ -156,30 +157,10 @@ static void engine_free(engine *E)
     int k;
     for (k = 0; k < 2; ++k) skk_graph_destroy(E->graph[k]);
     for (k = 0; k < E->ndev; ++k) skk_free(E->dev[k]);
-    dcsr_free(&E->A); dcsr_free(&E->AT);
+    dcsr_free(&E->A); dcsr_free(&E->AT); dcsr_free(&E->Q);
     skk_host_free(E->h_eval);
     free(E->perm_r); free(E->perm_c); free(E->R); free(E->C);
     memset(E, 0, sizeof(*E));
-}
-
-static int upload_vec_d(engine *E, double **dst, const double *src, int n)
-{
-    int rc;
-    if ((rc = dalloc(E, (void **)dst, sizeof(double) * (size_t)(n ? n : 1)))) return rc;
-    if ((rc = skk_h2d(*dst, src, sizeof(double) * (size_t)n))) return skg_hip_fail("upload", rc);
-    return 0;
-}
-
-static int upload_vec_f(engine *E, float **dst, const double *src, int n)
-{
-    int rc, k;
-    float *tmp = (float *)malloc(sizeof(float) * (size_t)(n ? n : 1));
-    if (!tmp) return skg_fail("out of host memory");
-    for (k = 0; k < n; ++k) tmp[k] = (float)src[k];
-    if ((rc = dalloc(E, (void **)dst, sizeof(float) * (size_t)(n ? n : 1)))) { free(tmp); return rc; }
-    rc = skk_h2d(*dst, tmp, sizeof(float) * (size_t)n);
-    free(tmp);
-    return rc ? skg_hip_fail("upload", rc) : 0;
 }
 
 static int alloc_iter(engine *E, skk_iter *it, size_t es)
@@ -196,17 +177,65 @@ static int alloc_iter(engine *E, skk_iter *it, size_t es)
         (rc = dalloc(E, &it->xavg, es * n)) || (rc = dalloc(E, &it->yavg, es * m)) ||
         (rc = dalloc(E, &it->xbar, es * n)))
         return rc;
+    if (E->has_q && ((rc = dalloc(E, &it->QX[0], es * n)) || (rc = dalloc(E, &it->QX[1], es * n))))
+        return rc;
     return 0;
+}
+
+/* Scale (original -> slot order) and upload c, bounds in FP64 (and FP32). */
+static int engine_set_data(engine *E, const double *c, const double *rl, const double *ru,
+                           const double *l, const double *u)
+{
+    const int m = E->m, n = E->n;
+    double *tc = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    double *tl = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    double *tu = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    double *trl = (double *)malloc(sizeof(double) * (size_t)(m + 1));
+    double *tru = (double *)malloc(sizeof(double) * (size_t)(m + 1));
+    int k, rc = 0;
+    skk_pdhg *Q = &E->P;
+    if (!tc || !tl || !tu || !trl || !tru) { rc = skg_fail("out of host memory"); goto done; }
+    for (k = 0; k < n; ++k) {
+        const int col = E->perm_c[k];
+        tc[k] = c[col] * E->C[col];
+        tl[k] = to_ieee(l[col]) / E->C[col];
+        tu[k] = to_ieee(u[col]) / E->C[col];
+    }
+    for (k = 0; k < m; ++k) {
+        const int r = E->perm_r[k];
+        trl[k] = E->R[r] * to_ieee(rl[r]);
+        tru[k] = E->R[r] * to_ieee(ru[r]);
+    }
+    E->cost_norm = finite_norm(tc, n);
+    E->bound_norm = sqrt(finite_norm(trl, m) * finite_norm(trl, m) + finite_norm(tru, m) * finite_norm(tru, m));
+#define PUT_D(dst, src, cnt) if (!rc && (rc = skk_h2d((void *)(dst), (src), sizeof(double) * (size_t)(cnt)))) rc = skg_hip_fail("upload", rc)
+    PUT_D(Q->cd, tc, n); PUT_D(Q->ld, tl, n); PUT_D(Q->ud, tu, n); PUT_D(Q->rld, trl, m); PUT_D(Q->rud, tru, m);
+#undef PUT_D
+    if (!rc && E->mixed) {
+        const double *src[5] = {tc, tl, tu, trl, tru};
+        const float *dst[5] = {Q->cf, Q->lf, Q->uf, Q->rlf, Q->ruf};
+        const int cnt[5] = {n, n, n, m, m};
+        float *tmp = (float *)malloc(sizeof(float) * (size_t)((m > n ? m : n) + 1));
+        int v;
+        if (!tmp) { rc = skg_fail("out of host memory"); goto done; }
+        for (v = 0; v < 5 && !rc; ++v) {
+            for (k = 0; k < cnt[v]; ++k) tmp[k] = (float)src[v][k];
+            if ((rc = skk_h2d((void *)dst[v], tmp, sizeof(float) * (size_t)cnt[v]))) rc = skg_hip_fail("upload", rc);
+        }
+        free(tmp);
+    }
+done:
+    free(tc); free(tl); free(tu); free(trl); free(tru);
+    return rc;
 }
 
 /* Build the scaled, permuted, binned problem on the device. */
 static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, const double *v,
-                        const double *c, const double *rl, const double *ru, const double *l,
-                        const double *u, const sk_gpu_pdlp_params *P)
+                        const int *qrp, const int *qci, const double *qv,
+                        const sk_gpu_pdlp_params *P)
 {
     hcsr A0, AT0, Ap, ATp;
     int *inv_r = NULL, *inv_c = NULL, i, j, k, q, rc = -1;
-    double *tc = NULL, *tl = NULL, *tu = NULL, *trl = NULL, *tru = NULL;
     skk_bins bA, bAT;
     memset(&Ap, 0, sizeof(Ap)); memset(&ATp, 0, sizeof(ATp)); memset(&AT0, 0, sizeof(AT0));
     E->m = m; E->n = n;
@@ -219,17 +248,12 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
     E->perm_c = (int *)malloc(sizeof(int) * (size_t)(n + 1));
     inv_r = (int *)malloc(sizeof(int) * (size_t)(m + 1));
     inv_c = (int *)malloc(sizeof(int) * (size_t)(n + 1));
-    tc = (double *)malloc(sizeof(double) * (size_t)(n + 1));
-    tl = (double *)malloc(sizeof(double) * (size_t)(n + 1));
-    tu = (double *)malloc(sizeof(double) * (size_t)(n + 1));
-    trl = (double *)malloc(sizeof(double) * (size_t)(m + 1));
-    tru = (double *)malloc(sizeof(double) * (size_t)(m + 1));
-    if (!E->R || !E->C || !E->perm_r || !E->perm_c || !inv_r || !inv_c || !tc || !tl || !tu || !trl || !tru) {
+    if (!E->R || !E->C || !E->perm_r || !E->perm_c || !inv_r || !inv_c) {
         skg_fail("out of host memory"); goto done;
     }
     if (P->scaling) {
-        ruiz(&A0, &AT0, P->ruiz_iterations, P->ruiz_tol, E->R, E->C);
-        pock_chambolle_into(&A0, E->R, E->C);
+        skg_ruiz(&A0, &AT0, P->ruiz_iterations, P->ruiz_tol, E->R, E->C);
+        skg_pock_chambolle_into(&A0, E->R, E->C);
     } else {
         for (i = 0; i < m; ++i) E->R[i] = 1.0;
         for (j = 0; j < n; ++j) E->C[j] = 1.0;
@@ -265,34 +289,50 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
         skg_plan_bins(ATp.p, n, &g_skg_tuning, NULL, &bAT)) { skg_fail("out of host memory"); goto done; }
     if ((rc = dcsr_upload(&E->A, &Ap, &bA, NULL, E->mixed)) ||
         (rc = dcsr_upload(&E->AT, &ATp, &bAT, NULL, E->mixed))) goto done;
-    for (k = 0; k < n; ++k) {
-        const int col = E->perm_c[k];
-        tc[k] = c[col] * E->C[col];
-        tl[k] = to_ieee(l[col]) / E->C[col];
-        tu[k] = to_ieee(u[col]) / E->C[col];
+    if (E->has_q) {
+        /* Q' = C Q C in slot order on both sides; its rows are not sorted by
+           their own length, so its bins address rows through a list */
+        hcsr Qp;
+        skk_bins bQ;
+        int *qorder = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+        if (!qorder || hcsr_alloc(&Qp, n, n, qrp[n])) { free(qorder); skg_fail("out of host memory"); goto done; }
+        q = 0;
+        for (k = 0; k < n; ++k) {
+            const int r = E->perm_c[k];
+            Qp.p[k] = q;
+            for (i = qrp[r]; i < qrp[r + 1]; ++i, ++q) {
+                Qp.j[q] = inv_c[qci[i]];
+                Qp.v[q] = qv[i] * E->C[r] * E->C[qci[i]];
+            }
+        }
+        Qp.p[n] = q;
+        rc = skg_plan_bins(Qp.p, n, &g_skg_tuning, qorder, &bQ) ? skg_fail("out of host memory")
+                                                                : dcsr_upload(&E->Q, &Qp, &bQ, qorder, E->mixed);
+        free(qorder);
+        hcsr_free(&Qp);
+        if (rc) goto done;
     }
-    for (k = 0; k < m; ++k) {
-        const int r = E->perm_r[k];
-        trl[k] = E->R[r] * to_ieee(rl[r]);
-        tru[k] = E->R[r] * to_ieee(ru[r]);
-    }
-    E->cost_norm = finite_norm(tc, n);
-    E->bound_norm = sqrt(finite_norm(trl, m) * finite_norm(trl, m) + finite_norm(tru, m) * finite_norm(tru, m));
     {
         skk_pdhg *Q = &E->P;
         double *cd, *ld, *ud, *rld, *rud;
         float *cf = NULL, *lf = NULL, *uf = NULL, *rlf = NULL, *ruf = NULL;
+        const size_t nn = (size_t)(n ? n : 1), mm = (size_t)(m ? m : 1);
         int cap, eg;
         memset(Q, 0, sizeof(*Q));
         Q->m = m; Q->n = n;
         Q->A = E->A.k; Q->AT = E->AT.k;
-        if ((rc = upload_vec_d(E, &cd, tc, n)) || (rc = upload_vec_d(E, &ld, tl, n)) ||
-            (rc = upload_vec_d(E, &ud, tu, n)) || (rc = upload_vec_d(E, &rld, trl, m)) ||
-            (rc = upload_vec_d(E, &rud, tru, m))) goto done;
+        Q->has_q = E->has_q;
+        if (E->has_q) {
+            Q->Q = E->Q.k;
+            if (E->Q.k.bins.total_blocks > 0 && (rc = dalloc(E, (void **)&Q->qx_d, 8 * nn))) goto done;
+        }
+        if ((rc = dalloc(E, (void **)&cd, 8 * nn)) || (rc = dalloc(E, (void **)&ld, 8 * nn)) ||
+            (rc = dalloc(E, (void **)&ud, 8 * nn)) || (rc = dalloc(E, (void **)&rld, 8 * mm)) ||
+            (rc = dalloc(E, (void **)&rud, 8 * mm))) goto done;
         if (E->mixed) {
-            if ((rc = upload_vec_f(E, &cf, tc, n)) || (rc = upload_vec_f(E, &lf, tl, n)) ||
-                (rc = upload_vec_f(E, &uf, tu, n)) || (rc = upload_vec_f(E, &rlf, trl, m)) ||
-                (rc = upload_vec_f(E, &ruf, tru, m))) goto done;
+            if ((rc = dalloc(E, (void **)&cf, 4 * nn)) || (rc = dalloc(E, (void **)&lf, 4 * nn)) ||
+                (rc = dalloc(E, (void **)&uf, 4 * nn)) || (rc = dalloc(E, (void **)&rlf, 4 * mm)) ||
+                (rc = dalloc(E, (void **)&ruf, 4 * mm))) goto done;
         }
         Q->cd = cd; Q->ld = ld; Q->ud = ud; Q->rld = rld; Q->rud = rud;
         Q->cf = cf; Q->lf = lf; Q->uf = uf; Q->rlf = rlf; Q->ruf = ruf;
@@ -305,17 +345,15 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
         cap = eg;
         if (bA.total_blocks > cap) cap = bA.total_blocks;
         if (bAT.total_blocks > cap) cap = bAT.total_blocks;
+        if (E->has_q && E->Q.k.bins.total_blocks > cap) cap = E->Q.k.bins.total_blocks;
         Q->part_capacity = cap;
-        if ((rc = dalloc(E, (void **)&Q->ax_d, sizeof(double) * (size_t)(m ? m : 1))) ||
-            (rc = dalloc(E, (void **)&Q->aty_d, sizeof(double) * (size_t)(n ? n : 1))) ||
+        if ((rc = dalloc(E, (void **)&Q->ax_d, 8 * mm)) || (rc = dalloc(E, (void **)&Q->aty_d, 8 * nn)) ||
             (rc = dalloc(E, (void **)&Q->st, sizeof(skk_state))) ||
             (rc = dalloc(E, (void **)&Q->part, sizeof(double) * 16 * (size_t)cap)) ||
             (rc = dalloc(E, (void **)&Q->counters, sizeof(unsigned) * 16)) ||
             (rc = dalloc(E, (void **)&Q->eval_out, sizeof(double) * 16)) ||
-            (rc = dalloc(E, (void **)&E->pv, sizeof(double) * (size_t)(n ? n : 1))) ||
-            (rc = dalloc(E, (void **)&E->pw, sizeof(double) * (size_t)(m ? m : 1))) ||
-            (rc = dalloc(E, (void **)&E->pv2, sizeof(double) * (size_t)(n ? n : 1))) ||
-            (rc = dalloc(E, (void **)&E->pnorm, sizeof(double))))
+            (rc = dalloc(E, (void **)&E->pv, 8 * nn)) || (rc = dalloc(E, (void **)&E->pw, 8 * mm)) ||
+            (rc = dalloc(E, (void **)&E->pv2, 8 * nn)) || (rc = dalloc(E, (void **)&E->pnorm, 8)))
             goto done;
         {
             unsigned zeros[16] = {0};
@@ -329,7 +367,7 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
     rc = 0;
 done:
     hcsr_free(&AT0); hcsr_free(&Ap); hcsr_free(&ATp);
-    free(inv_r); free(inv_c); free(tc); free(tl); free(tu); free(trl); free(tru);
+    free(inv_r); free(inv_c);
     return rc;
 }
 
@@ -408,91 +446,216 @@ static int run_window(engine *E, int prec, int period, int adaptive, double fixe
     return 0;
 }
 
-int sk_gpu_pdlp_solve(int rows, int cols, int nnz, const int *row_offsets, const int *column_indices,
-                      const double *values, const int *q_row_offsets, const int *q_column_indices,
-                      const double *q_values, int q_nnz, const double *c, const double *row_lower,
-                      const double *row_upper, const double *col_lower, const double *col_upper,
-                      const double *x0, const double *y0, const sk_gpu_pdlp_params *params,
-                      double *x_out, double *y_out, sk_gpu_pdlp_stats *stats)
-{
+struct sk_gpu_pdlp_session {
     engine E;
     sk_gpu_pdlp_params P;
-    const double t0 = skg_now();
-    double norm, eta_global, omega = 1.0, s_norm;
-    double mu_at_restart = INFINITY, mu_previous = INFINITY, best_mu_seen = INFINITY;
-    int iteration = 0, windows_since_restart = 0, its_since_restart = 0, prec, rc = 0, k;
-    int stall = 0, tail = 0, diverged = 0, use_avg_final = 0, final_prec;
-    kkt best;
-    memset(&E, 0, sizeof(E));
-    memset(&best, 0, sizeof(best));
-    best.primal = best.dual = best.gap = INFINITY;
-    skg_clear_error();
-    if (!stats) return skg_fail("null stats");
-    memset(stats, 0, sizeof(*stats));
-    if (params) P = *params; else sk_gpu_pdlp_params_default(&P);
-    if (!c || !row_lower || !row_upper || !col_lower || !col_upper || !x_out || (rows > 0 && !y_out))
-        return skg_fail("null model or output pointer");
-    if (q_row_offsets || q_column_indices || q_values || q_nnz)
-        return skg_fail("PDLP engine: quadratic objective not supported by this entry point");
-    if (P.eps_optimal <= 0.0 || P.max_iterations <= 0 || P.restart_period <= 0 ||
-        P.power_iterations < 0 || P.eta_min_factor <= 0.0 || P.eta_max_factor < P.eta_min_factor ||
-        (P.precision != SK_GPU_PREC_FP64 && P.precision != SK_GPU_PREC_MIXED))
-        return skg_fail("invalid PDLP parameters");
-    if (hcsr_validate(rows, cols, nnz, row_offsets, column_indices, values)) return -1;
+    double norm, eta_global;
+    int ready;
+};
+
+static int check_problem(int rows, int cols, int nnz, const int *rp, const int *ci, const double *v,
+                         const double *c, const double *rl, const double *ru, const double *l,
+                         const double *u)
+{
+    int k;
+    if (!c || !rl || !ru || !l || !u) return skg_fail("null model pointer");
+    if (hcsr_validate(rows, cols, nnz, rp, ci, v)) return -1;
     for (k = 0; k < cols; ++k)
-        if (!isfinite(c[k]) || isnan(col_lower[k]) || isnan(col_upper[k]) || col_lower[k] > col_upper[k] ||
-            to_ieee(col_lower[k]) == INFINITY || to_ieee(col_upper[k]) == -INFINITY)
+        if (!isfinite(c[k]) || isnan(l[k]) || isnan(u[k]) || l[k] > u[k] ||
+            to_ieee(l[k]) == INFINITY || to_ieee(u[k]) == -INFINITY)
             return skg_fail("invalid cost or column bounds at %d", k);
     for (k = 0; k < rows; ++k)
-        if (isnan(row_lower[k]) || isnan(row_upper[k]) || row_lower[k] > row_upper[k] ||
-            to_ieee(row_lower[k]) == INFINITY || to_ieee(row_upper[k]) == -INFINITY)
+        if (isnan(rl[k]) || isnan(ru[k]) || rl[k] > ru[k] || to_ieee(rl[k]) == INFINITY ||
+            to_ieee(ru[k]) == -INFINITY)
             return skg_fail("invalid row bounds at %d", k);
-    if ((rc = skg_have_device())) return -1;
-    E.mixed = P.precision == SK_GPU_PREC_MIXED;
-    if ((rc = engine_build(&E, rows, cols, row_offsets, column_indices, values, c, row_lower, row_upper,
-                           col_lower, col_upper, &P))) { engine_free(&E); return -1; }
-    /* ||A||_2 of the scaled matrix (GpuPdlp.cpp:179-211) */
-    if ((rc = skk_power_norm(&E.P, E.pv, E.pw, E.pv2, P.power_iterations, E.pnorm)) ||
-        (rc = skk_d2h(&s_norm, E.pnorm, sizeof(double)))) {
-        skg_hip_fail("power iteration", rc); engine_free(&E); return -1;
+    return 0;
+}
+
+static int check_params(const sk_gpu_pdlp_params *P)
+{
+    if (P->eps_optimal <= 0.0 || P->max_iterations <= 0 || P->restart_period <= 0 ||
+        P->power_iterations < 0 || P->eta_min_factor <= 0.0 || P->eta_max_factor < P->eta_min_factor ||
+        (P->precision != SK_GPU_PREC_FP64 && P->precision != SK_GPU_PREC_MIXED))
+        return skg_fail("invalid PDLP parameters");
+    return 0;
+}
+
+int sk_gpu_pdlp_session_create(int rows, int cols, int nnz, const int *row_offsets,
+                               const int *column_indices, const double *values, const double *c,
+                               const double *row_lower, const double *row_upper,
+                               const double *col_lower, const double *col_upper,
+                               const sk_gpu_pdlp_params *params, sk_gpu_pdlp_session **out)
+{
+    return sk_gpu_pdlp_session_create_qp(rows, cols, nnz, row_offsets, column_indices, values, NULL,
+                                         NULL, NULL, 0, c, row_lower, row_upper, col_lower, col_upper,
+                                         params, out);
+}
+
+int sk_gpu_pdlp_session_create_qp(int rows, int cols, int nnz, const int *row_offsets,
+                                  const int *column_indices, const double *values,
+                                  const int *q_row_offsets, const int *q_column_indices,
+                                  const double *q_values, int q_nnz, const double *c,
+                                  const double *row_lower, const double *row_upper,
+                                  const double *col_lower, const double *col_upper,
+                                  const sk_gpu_pdlp_params *params, sk_gpu_pdlp_session **out)
+{
+    sk_gpu_pdlp_session *S;
+    double s_norm = 0.0;
+    int rc;
+    const int has_q = q_row_offsets != NULL;
+    skg_clear_error();
+    if (!out) return skg_fail("null session pointer");
+    *out = NULL;
+    if (check_problem(rows, cols, nnz, row_offsets, column_indices, values, c, row_lower, row_upper,
+                      col_lower, col_upper)) return -1;
+    if (has_q && hcsr_validate(cols, cols, q_nnz, q_row_offsets, q_column_indices, q_values)) return -1;
+    if (!has_q && (q_column_indices || q_values || q_nnz)) return skg_fail("Hessian arrays without offsets");
+    S = (sk_gpu_pdlp_session *)calloc(1, sizeof(*S));
+    if (!S) return skg_fail("out of host memory");
+    if (params) S->P = *params; else sk_gpu_pdlp_params_default(&S->P);
+    if (check_params(&S->P) || skg_have_device()) { free(S); return -1; }
+    if (has_q && !S->P.adaptive_step) { free(S); return skg_fail("a Hessian needs the adaptive step"); }
+    S->E.mixed = S->P.precision == SK_GPU_PREC_MIXED;
+    S->E.has_q = has_q;
+    if (engine_build(&S->E, rows, cols, row_offsets, column_indices, values, q_row_offsets,
+                     q_column_indices, q_values, &S->P) ||
+        engine_set_data(&S->E, c, row_lower, row_upper, col_lower, col_upper)) {
+        engine_free(&S->E); free(S); return -1;
     }
-    E.launches += 2 + 3LL * P.power_iterations;
-    stats->host_syncs++;
-    norm = sqrt(sqrt(fmax(0.0, s_norm)));
-    if (!(norm > 0.0) || !isfinite(norm)) norm = 1.0;
-    stats->matrix_norm = norm;
-    eta_global = 0.9 / norm;
-    if (E.cost_norm > 0.0 && E.bound_norm > 0.0)
-        omega = fmin(1e6, fmax(1e-6, E.cost_norm / E.bound_norm));
-    prec = E.mixed ? SKK_F32 : SKK_F64;
-    /* warm start: x' = x0 / C, y' = y0 / R, in slot order */
+    /* ||A||_2 of the scaled matrix (GpuPdlp.cpp:179-211) */
+    if ((rc = skk_power_norm(&S->E.P, S->E.pv, S->E.pw, S->E.pv2, S->P.power_iterations, S->E.pnorm)) ||
+        (rc = skk_d2h(&s_norm, S->E.pnorm, sizeof(double)))) {
+        skg_hip_fail("power iteration", rc); engine_free(&S->E); free(S); return -1;
+    }
+    S->norm = sqrt(sqrt(fmax(0.0, s_norm)));
+    if (!(S->norm > 0.0) || !isfinite(S->norm)) S->norm = 1.0;
+    S->eta_global = 0.9 / S->norm;
+    S->ready = 1;
+    *out = S;
+    return 0;
+}
+
+int sk_gpu_pdlp_session_update(sk_gpu_pdlp_session *S, const double *c, const double *row_lower,
+                               const double *row_upper, const double *col_lower,
+                               const double *col_upper)
+{
+    skg_clear_error();
+    if (!S || !S->ready) return skg_fail("invalid session");
+    if (!c || !row_lower || !row_upper || !col_lower || !col_upper) return skg_fail("null data pointer");
+    return engine_set_data(&S->E, c, row_lower, row_upper, col_lower, col_upper) ? -1 : 0;
+}
+
+void sk_gpu_pdlp_session_destroy(sk_gpu_pdlp_session *S)
+{
+    if (!S) return;
+    engine_free(&S->E);
+    free(S);
+}
+
+/* Cold (x=0, y=0, GpuPdlp.cpp:278-291) or warm start in slot order. */
+static int session_start(sk_gpu_pdlp_session *S, int prec, const double *x0, const double *y0,
+                         double *omega_out)
+{
+    engine *E = &S->E;
+    const int rows = E->m, cols = E->n;
+    double omega = 1.0;
+    int k, rc = 0;
+    if (E->cost_norm > 0.0 && E->bound_norm > 0.0)
+        omega = fmin(1e6, fmax(1e-6, E->cost_norm / E->bound_norm));
     if (x0 || y0) {
         double *tx = (double *)malloc(sizeof(double) * (size_t)(cols + 1));
         double *ty = (double *)malloc(sizeof(double) * (size_t)(rows + 1));
-        if (!tx || !ty) { free(tx); free(ty); engine_free(&E); return skg_fail("out of host memory"); }
-        for (k = 0; k < cols; ++k) tx[k] = x0 ? x0[E.perm_c[k]] / E.C[E.perm_c[k]] : 0.0;
-        for (k = 0; k < rows; ++k) ty[k] = y0 ? y0[E.perm_r[k]] / E.R[E.perm_r[k]] : 0.0;
+        float *fx = (float *)malloc(sizeof(float) * (size_t)(cols + 1));
+        float *fy = (float *)malloc(sizeof(float) * (size_t)(rows + 1));
+        if (!tx || !ty || !fx || !fy) { free(tx); free(ty); free(fx); free(fy); return skg_fail("out of host memory"); }
+        for (k = 0; k < cols; ++k) tx[k] = x0 ? x0[E->perm_c[k]] / E->C[E->perm_c[k]] : 0.0;
+        for (k = 0; k < rows; ++k) ty[k] = y0 ? y0[E->perm_r[k]] / E->R[E->perm_r[k]] : 0.0;
         if (prec == SKK_F32) {
-            float *fx = (float *)malloc(sizeof(float) * (size_t)(cols + 1));
-            float *fy = (float *)malloc(sizeof(float) * (size_t)(rows + 1));
-            if (!fx || !fy) { free(fx); free(fy); free(tx); free(ty); engine_free(&E); return skg_fail("out of host memory"); }
             for (k = 0; k < cols; ++k) fx[k] = (float)tx[k];
             for (k = 0; k < rows; ++k) fy[k] = (float)ty[k];
-            rc = skk_h2d(E.P.it[prec].X[0], fx, sizeof(float) * (size_t)cols);
-            if (!rc) rc = skk_h2d(E.P.it[prec].Y[0], fy, sizeof(float) * (size_t)rows);
-            free(fx); free(fy);
+            rc = skk_h2d(E->P.it[prec].X[0], fx, sizeof(float) * (size_t)cols);
+            if (!rc) rc = skk_h2d(E->P.it[prec].Y[0], fy, sizeof(float) * (size_t)rows);
         } else {
-            rc = skk_h2d(E.P.it[prec].X[0], tx, sizeof(double) * (size_t)cols);
-            if (!rc) rc = skk_h2d(E.P.it[prec].Y[0], ty, sizeof(double) * (size_t)rows);
+            rc = skk_h2d(E->P.it[prec].X[0], tx, sizeof(double) * (size_t)cols);
+            if (!rc) rc = skk_h2d(E->P.it[prec].Y[0], ty, sizeof(double) * (size_t)rows);
         }
-        free(tx); free(ty);
-        if (rc) { skg_hip_fail("warm start upload", rc); engine_free(&E); return -1; }
+        free(tx); free(ty); free(fx); free(fy);
+        if (rc) return skg_hip_fail("warm start upload", rc);
     }
-    if ((rc = skk_pdhg_init(&E.P, prec, eta_global, omega, P.eta_min_factor / norm,
-                            P.eta_max_factor / norm, x0 != NULL, y0 != NULL))) {
-        skg_hip_fail("PDHG init", rc); engine_free(&E); return -1;
+    if ((rc = skk_pdhg_init(&E->P, prec, S->eta_global, omega, S->P.eta_min_factor / S->norm,
+                            S->P.eta_max_factor / S->norm, x0 != NULL, y0 != NULL)))
+        return skg_hip_fail("PDHG init", rc);
+    *omega_out = omega;
+    return 0;
+}
+
+int sk_gpu_pdlp_session_iterate(sk_gpu_pdlp_session *S, int iterations, double *kernel_ms)
+{
+    engine *E;
+    int prec, k, part, rc;
+    double omega;
+    skg_clear_error();
+    if (!S || !S->ready || iterations < 0) return skg_fail("invalid session or count");
+    E = &S->E;
+    prec = E->mixed ? SKK_F32 : SKK_F64;
+    if (session_start(S, prec, NULL, NULL, &omega)) return -1;
+    if (kernel_ms) {
+        const int parts = S->P.adaptive_step ? 3 : 1;
+        if ((rc = skk_timeline_reserve(4 * iterations + 4))) return skg_hip_fail("event timeline", rc);
+        if ((rc = skk_sync())) return skg_hip_fail("sync", rc);
+        for (k = 0; k < iterations; ++k) {
+            if ((rc = skk_timeline_mark(4 * k))) return skg_hip_fail("event record", rc);
+            for (part = 0; part < parts; ++part) {
+                rc = S->P.adaptive_step ? skk_pdhg_adaptive_part(&E->P, prec, part)
+                                        : skk_pdhg_fixed_iter(&E->P, prec);
+                if (rc) return skg_hip_fail("PDHG launch", rc);
+                if ((rc = skk_timeline_mark(4 * k + part + 1))) return skg_hip_fail("event record", rc);
+            }
+        }
+        if ((rc = skk_sync())) return skg_hip_fail("sync", rc);
+        for (part = 0; part < 4; ++part) kernel_ms[part] = 0.0;
+        for (k = 0; k < iterations; ++k) {
+            for (part = 0; part < parts; ++part)
+                kernel_ms[part] += skk_timeline_ms(4 * k + part, 4 * k + part + 1);
+            kernel_ms[3] += skk_timeline_ms(4 * k, 4 * k + parts);
+        }
+        for (part = 0; part < 4; ++part) kernel_ms[part] /= (iterations > 0 ? iterations : 1);
+        return 0;
     }
-    if ((rc = skk_sync())) { skg_hip_fail("setup", rc); engine_free(&E); return -1; }
+    for (k = 0; k < iterations; ++k) {
+        rc = S->P.adaptive_step ? skk_pdhg_adaptive_iter(&E->P, prec) : skk_pdhg_fixed_iter(&E->P, prec);
+        if (rc) return skg_hip_fail("PDHG launch", rc);
+    }
+    if ((rc = skk_sync())) return skg_hip_fail("sync", rc);
+    return 0;
+}
+
+int sk_gpu_pdlp_session_solve(sk_gpu_pdlp_session *S, const double *x0, const double *y0,
+                              double *x_out, double *y_out, sk_gpu_pdlp_stats *stats)
+{
+    engine *E;
+    sk_gpu_pdlp_params P;
+    const double t0 = skg_now();
+    double omega = 1.0;
+    double mu_at_restart = INFINITY, mu_previous = INFINITY, best_mu_seen = INFINITY;
+    int iteration = 0, windows_since_restart = 0, its_since_restart = 0, prec, rc = 0, k, rows, cols;
+    int stall = 0, tail = 0, diverged = 0, use_avg_final = 0;
+    kkt best;
+    skg_clear_error();
+    if (!stats) return skg_fail("null stats");
+    memset(stats, 0, sizeof(*stats));
+    if (!S || !S->ready) return skg_fail("invalid session");
+    E = &S->E;
+    P = S->P;
+    rows = E->m; cols = E->n;
+    if (!x_out || (rows > 0 && !y_out)) return skg_fail("null output pointer");
+    memset(&best, 0, sizeof(best));
+    best.primal = best.dual = best.gap = INFINITY;
+    E->launches = 0;
+    stats->matrix_norm = S->norm;
+    prec = E->mixed ? SKK_F32 : SKK_F64;
+    if (session_start(S, prec, x0, y0, &omega)) return -1;
+    if ((rc = skk_sync())) return skg_hip_fail("setup", rc);
     stats->setup_seconds = skg_now() - t0;
     {
         const double tl0 = skg_now();
@@ -503,52 +666,49 @@ int sk_gpu_pdlp_solve(int rows, int cols, int nnz, const int *row_offsets, const
             int average_better, sufficient, necessary, artificial;
             double mu;
             if (period <= 0) break;
-            rc = run_window(&E, prec, period, P.adaptive_step,
-                            (double)(windows_since_restart + 1) * (double)P.restart_period, P.use_graph,
-                            P.restart_period);
-            if (rc) { engine_free(&E); return -1; }
+            if (run_window(E, prec, period, P.adaptive_step,
+                           (double)(windows_since_restart + 1) * (double)P.restart_period, P.use_graph,
+                           P.restart_period)) return -1;
             stats->host_syncs++;
             iteration += period;
             if (prec == SKK_F32) stats->fp32_iterations += period; else stats->fp64_iterations += period;
             its_since_restart += period;
             ++windows_since_restart;
-            cur = kkt_from(E.h_eval, E.cost_norm, E.bound_norm);
-            avg = kkt_from(E.h_eval + 8, E.cost_norm, E.bound_norm);
+            cur = kkt_from(E->h_eval, E->cost_norm, E->bound_norm);
+            avg = kkt_from(E->h_eval + 8, E->cost_norm, E->bound_norm);
             average_better = kkt_worst(&avg) < kkt_worst(&cur);
             best = average_better ? avg : cur;
             if (P.verbosity > 1)
                 fprintf(stderr, "it %d prec %s cur %.3e avg %.3e (p %.2e d %.2e g %.2e) eta %.3e omega %.3e\n",
                         iteration, prec == SKK_F32 ? "f32" : "f64", kkt_worst(&cur), kkt_worst(&avg),
-                        best.primal, best.dual, best.gap, E.h_state->eta, E.h_state->omega);
+                        best.primal, best.dual, best.gap, E->h_state->eta, E->h_state->omega);
             if (!isfinite(kkt_worst(&cur)) && !isfinite(kkt_worst(&avg))) { diverged = 1; break; }
             if (best.primal <= P.eps_optimal && best.dual <= P.eps_optimal && best.gap <= P.eps_optimal) {
                 use_avg_final = average_better;
                 stats->converged = 1;
                 break;
             }
-            if (P.time_limit_seconds > 0.0 && skg_now() - t0 > P.time_limit_seconds) {
-                use_avg_final = average_better;
-                break;
-            }
+            use_avg_final = average_better;
+            if (P.time_limit_seconds > 0.0 && skg_now() - t0 > P.time_limit_seconds) break;
             mu = kkt_worst(&best);
             /* mixed precision: hand over to FP64 once FP32 is near its floor */
             if (prec == SKK_F32 && !tail) {
                 if (mu < 0.5 * best_mu_seen) { best_mu_seen = mu; stall = 0; }
                 else ++stall;
                 if (mu <= P.tail_switch || stall >= P.tail_stall_windows) {
-                    if ((rc = skk_pdhg_restart(&E.P, SKK_F32, average_better, P.adaptive_step)) ||
-                        (rc = skk_pdhg_promote(&E.P)) ||
-                        (rc = skk_pdhg_eval(&E.P, SKK_F64, 0, P.adaptive_step, 1.0)) ||
-                        (rc = skk_pdhg_restart(&E.P, SKK_F64, 0, P.adaptive_step))) {
-                        skg_hip_fail("precision switch", rc); engine_free(&E); return -1;
-                    }
-                    E.launches += 8;
+                    if ((rc = skk_pdhg_restart(&E->P, SKK_F32, average_better, P.adaptive_step)) ||
+                        (rc = skk_pdhg_promote(&E->P)) ||
+                        (rc = skk_pdhg_eval(&E->P, SKK_F64, 0, P.adaptive_step, 1.0)) ||
+                        (rc = skk_pdhg_restart(&E->P, SKK_F64, 0, P.adaptive_step)))
+                        return skg_hip_fail("precision switch", rc);
+                    E->launches += 8;
                     prec = SKK_F64;
                     tail = 1;
                     windows_since_restart = 0;
                     its_since_restart = 0;
                     mu_at_restart = mu;
                     mu_previous = INFINITY;
+                    use_avg_final = 0;
                     continue;
                 }
             }
@@ -558,10 +718,10 @@ int sk_gpu_pdlp_solve(int rows, int cols, int nnz, const int *row_offsets, const
             artificial = (double)its_since_restart >= 0.36 * (double)iteration;
             mu_previous = mu;
             if (!(sufficient || necessary || artificial)) continue;
-            if ((rc = skk_pdhg_restart(&E.P, prec, average_better, P.adaptive_step))) {
-                skg_hip_fail("restart launch", rc); engine_free(&E); return -1;
-            }
-            E.launches += 1;
+            if ((rc = skk_pdhg_restart(&E->P, prec, average_better, P.adaptive_step)))
+                return skg_hip_fail("restart launch", rc);
+            E->launches += 1;
+            use_avg_final = 0;
             windows_since_restart = 0;
             its_since_restart = 0;
             mu_at_restart = mu;
@@ -573,31 +733,27 @@ int sk_gpu_pdlp_solve(int rows, int cols, int nnz, const int *row_offsets, const
                 const double updated = exp(0.5 * log(target) + 0.5 * log(omega));
                 if (isfinite(updated)) {
                     omega = fmin(1e6, fmax(1e-6, updated));
-                    if ((rc = skk_pdhg_set_omega(&E.P, omega))) {
-                        skg_hip_fail("primal weight update", rc); engine_free(&E); return -1;
-                    }
-                    E.launches += 1;
+                    if ((rc = skk_pdhg_set_omega(&E->P, omega))) return skg_hip_fail("primal weight update", rc);
+                    E->launches += 1;
                 }
             }
         }
-        final_prec = prec;
-        if ((rc = skk_pdhg_export(&E.P, final_prec, use_avg_final, E.pv, E.pw)) ||
-            (rc = skk_d2h_async(E.h_state, E.P.st, sizeof(skk_state))) || (rc = skk_sync())) {
-            skg_hip_fail("export", rc); engine_free(&E); return -1;
-        }
+        if ((rc = skk_pdhg_export(&E->P, prec, use_avg_final, E->pv, E->pw)) ||
+            (rc = skk_d2h_async(E->h_state, E->P.st, sizeof(skk_state))) || (rc = skk_sync()))
+            return skg_hip_fail("export", rc);
         stats->loop_seconds = skg_now() - tl0;
     }
     {
         double *xs = (double *)malloc(sizeof(double) * (size_t)(cols + 1));
         double *ys = (double *)malloc(sizeof(double) * (size_t)(rows + 1));
-        if (!xs || !ys) { free(xs); free(ys); engine_free(&E); return skg_fail("out of host memory"); }
-        if ((rc = skk_d2h(xs, E.pv, sizeof(double) * (size_t)cols)) ||
-            (rc = skk_d2h(ys, E.pw, sizeof(double) * (size_t)rows))) {
-            free(xs); free(ys); skg_hip_fail("result copy", rc); engine_free(&E); return -1;
+        if (!xs || !ys) { free(xs); free(ys); return skg_fail("out of host memory"); }
+        if ((rc = skk_d2h(xs, E->pv, sizeof(double) * (size_t)cols)) ||
+            (rc = skk_d2h(ys, E->pw, sizeof(double) * (size_t)rows))) {
+            free(xs); free(ys); return skg_hip_fail("result copy", rc);
         }
         stats->host_syncs += 2;
-        for (k = 0; k < cols; ++k) x_out[E.perm_c[k]] = xs[k] * E.C[E.perm_c[k]];
-        for (k = 0; k < rows; ++k) y_out[E.perm_r[k]] = ys[k] * E.R[E.perm_r[k]];
+        for (k = 0; k < cols; ++k) x_out[E->perm_c[k]] = xs[k] * E->C[E->perm_c[k]];
+        for (k = 0; k < rows; ++k) y_out[E->perm_r[k]] = ys[k] * E->R[E->perm_r[k]];
         free(xs); free(ys);
     }
     stats->iterations = iteration;
@@ -608,17 +764,44 @@ int sk_gpu_pdlp_solve(int rows, int cols, int nnz, const int *row_offsets, const
     stats->relative_gap = best.gap;
     stats->primal_weight = omega;
     if (P.adaptive_step) {
-        const double rej = E.h_state->updates - E.h_state->accept_count;
-        stats->final_step_size = E.h_state->eta;
-        stats->step_size_ratio = E.h_state->eta * norm;
+        const double rej = E->h_state->updates - E->h_state->accept_count;
+        stats->final_step_size = E->h_state->eta;
+        stats->step_size_ratio = E->h_state->eta * S->norm;
         stats->rejected_steps = (int)(rej > 0.0 ? rej : 0.0);
     } else {
-        stats->final_step_size = eta_global;
+        stats->final_step_size = S->eta_global;
         stats->step_size_ratio = 0.9;
     }
-    stats->kernel_launches = E.launches;
+    stats->kernel_launches = E->launches;
     stats->seconds = skg_now() - t0;
-    engine_free(&E);
     if (diverged) { skg_fail("PDHG iterate became non-finite"); return -2; }
     return 0;
+}
+
+int sk_gpu_pdlp_solve(int rows, int cols, int nnz, const int *row_offsets, const int *column_indices,
+                      const double *values, const int *q_row_offsets, const int *q_column_indices,
+                      const double *q_values, int q_nnz, const double *c, const double *row_lower,
+                      const double *row_upper, const double *col_lower, const double *col_upper,
+                      const double *x0, const double *y0, const sk_gpu_pdlp_params *params,
+                      double *x_out, double *y_out, sk_gpu_pdlp_stats *stats)
+{
+    sk_gpu_pdlp_session *S = NULL;
+    const double t0 = skg_now();
+    double build;
+    int rc;
+    skg_clear_error();
+    if (!stats) return skg_fail("null stats");
+    memset(stats, 0, sizeof(*stats));
+    if (!x_out || (rows > 0 && !y_out)) return skg_fail("null output pointer");
+    if (sk_gpu_pdlp_session_create_qp(rows, cols, nnz, row_offsets, column_indices, values,
+                                      q_row_offsets, q_column_indices, q_values, q_nnz, c, row_lower,
+                                      row_upper, col_lower, col_upper, params, &S)) return -1;
+    build = skg_now() - t0;
+    rc = sk_gpu_pdlp_session_solve(S, x0, y0, x_out, y_out, stats);
+    stats->setup_seconds += build;
+    stats->seconds = skg_now() - t0;
+    stats->host_syncs += 1;
+    stats->kernel_launches += 2 + 3LL * (params ? params->power_iterations : 30);
+    sk_gpu_pdlp_session_destroy(S);
+    return rc;
 }

@@ -121,6 +121,28 @@ class GpuOptions(C.Structure):
                 ("stats", PdlpStats)]
 
 
+class BatchOptions(C.Structure):
+    _fields_ = [("pdlp", PdlpParams), ("block_threads", C.c_int), ("use_graph", C.c_int)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("restarts", C.c_int),
+                ("relative_primal_residual", C.c_double), ("relative_dual_residual", C.c_double),
+                ("relative_gap", C.c_double), ("objective", C.c_double)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("setup_seconds", C.c_double), ("device_seconds", C.c_double),
+                ("seconds", C.c_double), ("block_threads", C.c_int), ("grid", C.c_int),
+                ("lds_bytes", C.c_int), ("converged", C.c_int), ("total_iterations", C.c_longlong),
+                ("tune_ms", C.c_double * 3)]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["tune_ms"] = list(self.tune_ms)
+        return d
+
+
 def default_pdlp_params(**overrides) -> PdlpParams:
     """Defaults of PdlpParams (src/cuda/GpuPdlp.hpp:19-67) except eps 1e-6;
     the same values the library's sk_gpu_pdlp_params_default sets."""
@@ -233,6 +255,20 @@ def load_gpu():
                  [C.c_int, C.c_int, C.c_int, c_int_p, c_int_p, c_double_p, c_double_p,
                   c_double_p, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
                   c_double_p])
+        vp = C.c_void_p
+        _declare(lib, "sk_gpu_pdlp_session_create", C.c_int,
+                 [C.c_int, C.c_int, C.c_int, c_int_p, c_int_p, c_double_p, c_double_p, c_double_p,
+                  c_double_p, c_double_p, c_double_p, C.POINTER(PdlpParams), C.POINTER(vp)])
+        _declare(lib, "sk_gpu_pdlp_session_update", C.c_int,
+                 [vp, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p])
+        _declare(lib, "sk_gpu_pdlp_session_iterate", C.c_int, [vp, C.c_int, c_double_p])
+        _declare(lib, "sk_gpu_pdlp_session_solve", C.c_int,
+                 [vp, c_double_p, c_double_p, c_double_p, c_double_p, C.POINTER(PdlpStats)])
+        _declare(lib, "sk_gpu_pdlp_session_destroy", None, [vp])
+        _declare(lib, "sk_gpu_batch_options_default", None, [C.POINTER(BatchOptions)])
+        _declare(lib, "sk_gpu_batch_solve", C.c_int,
+                 [C.c_int, M, C.POINTER(BatchOptions), c_double_p, c_double_p,
+                  C.POINTER(BatchResult), C.POINTER(BatchStats)])
         _declare(lib, "sk_gpu_model_validate", C.c_int, [M])
         _declare(lib, "sk_gpu_solution_init", None, [S])
         _declare(lib, "sk_gpu_solution_free", None, [S])
@@ -352,3 +388,86 @@ def csc_to_csr(nrow, ncol, p, i, x):
     cols = np.repeat(np.arange(ncol, dtype=np.int32), np.diff(p))
     order = np.argsort(i[:nnz], kind="stable")
     return rp, cols[order].astype(np.int32), x[:nnz][order].astype(np.float64)
+
+
+# ------------------------------------------------- device memory (tests)
+
+_hip = None
+
+
+def _hip_runtime():
+    """The HIP runtime libsankhya_gpu.so itself links (libamdhip64.so.7), so
+    buffers made here live in the same runtime as the library's work."""
+    global _hip
+    if _hip is None:
+        load_gpu()
+        _hip = C.CDLL("libamdhip64.so.7")
+        _hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        _hip.hipFree.argtypes = [C.c_void_p]
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipDeviceSynchronize.argtypes = []
+    return _hip
+
+
+class DeviceArray:
+    """A float64 device buffer (hipMalloc) with host round trips."""
+
+    H2D, D2H, D2D = 1, 2, 3
+
+    def __init__(self, host):
+        host = np.ascontiguousarray(host, np.float64)
+        self.n = host.size
+        self.ptr = C.c_void_p()
+        hip = _hip_runtime()
+        if hip.hipMalloc(C.byref(self.ptr), max(8, 8 * self.n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+        if self.n and hip.hipMemcpy(self.ptr, host.ctypes.data, 8 * self.n, self.H2D) != 0:
+            raise RuntimeError("hipMemcpy H2D failed")
+
+    @property
+    def value(self):
+        return self.ptr.value
+
+    def numpy(self):
+        out = np.empty(self.n)
+        hip = _hip_runtime()
+        hip.hipDeviceSynchronize()
+        if self.n and hip.hipMemcpy(out.ctypes.data, self.ptr, 8 * self.n, self.D2H) != 0:
+            raise RuntimeError("hipMemcpy D2H failed")
+        return out
+
+    def __del__(self):
+        if self.ptr and _hip is not None:
+            _hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+def device_synchronize():
+    _hip_runtime().hipDeviceSynchronize()
+
+
+def batch_solve(models, block_threads=0, use_graph=1, **pdlp):
+    """sk_gpu_batch_solve over a list of Model objects; returns
+    (rc, [x_k], [y_k], [BatchResult], BatchStats)."""
+    lib = load_gpu()
+    opt = BatchOptions()
+    lib.sk_gpu_batch_options_default(C.byref(opt))
+    opt.block_threads = block_threads
+    opt.use_graph = use_graph
+    for k, v in pdlp.items():
+        setattr(opt.pdlp, k, v)
+    arr = (SkModel * len(models))()
+    for k, md in enumerate(models):
+        arr[k] = md.m
+    cols = sum(md.ncol for md in models)
+    rows = sum(md.nrow for md in models)
+    x = np.zeros(max(cols, 1))
+    y = np.zeros(max(rows, 1))
+    res = (BatchResult * max(len(models), 1))()
+    st = BatchStats()
+    rc = lib.sk_gpu_batch_solve(len(models), arr, C.byref(opt), dptr(x), dptr(y), res, C.byref(st))
+    xs, ys, c0, r0 = [], [], 0, 0
+    for md in models:
+        xs.append(x[c0:c0 + md.ncol].copy()); c0 += md.ncol
+        ys.append(y[r0:r0 + md.nrow].copy()); r0 += md.nrow
+    return rc, xs, ys, [res[k] for k in range(len(models))], st
