@@ -77,12 +77,21 @@ class Session:
             raise RuntimeError(f"iterate failed: {sg.last_error()}")
         return ms
 
-    def solve(self):
+    def solve(self, x0=None, y0=None):
         x = np.zeros(self.model.ncol)
         y = np.zeros(self.model.nrow)
         st = sg.PdlpStats()
-        rc = self.lib.sk_gpu_pdlp_session_solve(self.h, None, None, sg.dptr(x), sg.dptr(y), C.byref(st))
+        x0 = np.ascontiguousarray(x0, np.float64) if x0 is not None else None
+        y0 = np.ascontiguousarray(y0, np.float64) if y0 is not None else None
+        rc = self.lib.sk_gpu_pdlp_session_solve(self.h, sg.dptr(x0) if x0 is not None else None,
+                                                sg.dptr(y0) if y0 is not None else None,
+                                                sg.dptr(x), sg.dptr(y), C.byref(st))
         return rc, x, y, st
+
+    def update(self, c, rl, ru, l, u):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (c, rl, ru, l, u)]
+        if self.lib.sk_gpu_pdlp_session_update(self.h, *[sg.dptr(a) for a in arrs]) != 0:
+            raise RuntimeError(sg.last_error())
 
     def close(self):
         if self.h:
@@ -118,22 +127,8 @@ def cpu_baseline(model, iters):
                       f"threads=1) on the same LP, {dt:.1f} s incl. its setup"}
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=30)
-    ap.add_argument("--skip-solve", action="store_true")
-    ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=100)
-    ap.add_argument("--quick", action="store_true", help="small instance (smoke)")
-    ap.add_argument("--solve-time-limit", type=float, default=150.0)
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
-
+def workload_lp1(args):
+    """Headline: the 1M x 2M power-law LP (iterations/s, time to 1e-6 KKT)."""
     lib = sg.load_gpu()
     t0 = time.perf_counter()
     model, q = make_workload(args.quick)
@@ -215,8 +210,289 @@ def main():
     if not args.skip_cpu:
         log("CPU baseline ...")
         line["cpu_baseline"] = cpu_baseline(model, args.cpu_iters)
+    return line
+
+
+# ------------------------------------------------------------ other configs
+
+def _arrays(model):
+    p, i, x = model.csc()
+    rp, cj, v = sg.csc_to_csr(model.nrow, model.ncol, p, i, x)
+    vec = {k: model.vec(k) for k in ("c", "rlow", "rupp", "clow", "cupp")}
+    return rp, cj, v, vec
+
+
+def _session_qp(lib, model, arrays, q, **params):
+    """A PDLP session with an optional full symmetric Hessian q = (p, i, x)."""
+    rp, cj, v, vec = arrays
+    S = Session.__new__(Session)
+    S.lib, S.model = lib, model
+    S.P = sg.default_pdlp_params(**params)
+    S.h = C.c_void_p()
+    qa = [None, None, None, 0] if q is None else [sg.iptr(q[0]), sg.iptr(q[1]), sg.dptr(q[2]), int(q[0][-1])]
+    rc = lib.sk_gpu_pdlp_session_create_qp(model.nrow, model.ncol, int(rp[-1]), sg.iptr(rp), sg.iptr(cj),
+                                           sg.dptr(v), *qa, sg.dptr(vec["c"]), sg.dptr(vec["rlow"]),
+                                           sg.dptr(vec["rupp"]), sg.dptr(vec["clow"]), sg.dptr(vec["cupp"]),
+                                           C.byref(S.P), C.byref(S.h))
+    if rc != 0:
+        raise RuntimeError(f"session create failed: {sg.last_error()}")
+    return S
+
+
+def _kkt(lib, model, arrays, q, x, y):
+    rp, cj, v, vec = arrays
+    out = np.zeros(5)
+    qa = [None, None, None, 0] if q is None else [sg.iptr(q[0]), sg.iptr(q[1]), sg.dptr(q[2]), int(q[0][-1])]
+    rc = lib.sk_gpu_relative_kkt_qp(model.nrow, model.ncol, int(rp[-1]), sg.iptr(rp), sg.iptr(cj), sg.dptr(v),
+                                    *qa, sg.dptr(vec["c"]), sg.dptr(vec["rlow"]), sg.dptr(vec["rupp"]),
+                                    sg.dptr(vec["clow"]), sg.dptr(vec["cupp"]), sg.dptr(x), sg.dptr(y),
+                                    sg.dptr(out))
+    if rc != 0:
+        raise RuntimeError(sg.last_error())
+    return out
+
+
+def _engines(lib, model, arrays, q, args, use_graph=0):
+    """Iterations/s and cold time to 1e-6 for the mixed and the FP64 engine."""
+    res = {}
+    for name, prec in (("mixed", sg.PREC_MIXED), ("fp64", sg.PREC_FP64)):
+        S = _session_qp(lib, model, arrays, q, precision=prec, use_graph=use_graph,
+                        time_limit_seconds=args.solve_time_limit)
+        S.iterate(args.warmup)
+        t = time.perf_counter()
+        S.iterate(args.steps)
+        wall = time.perf_counter() - t
+        ms = S.iterate(min(args.steps, 200), timed=True)
+        r = dict(iters_per_s=args.steps / wall, kernel_ms=ms.tolist())
+        if not args.skip_solve:
+            rc, x, y, st = S.solve()
+            kkt = _kkt(lib, model, arrays, q, x, y)
+            r.update(converged=bool(st.converged), iterations=st.iterations, fp32_iterations=st.fp32_iterations,
+                     fp64_iterations=st.fp64_iterations, loop_s=st.loop_seconds, restarts=st.restarts,
+                     original_kkt=kkt[:3].tolist(), objective=float(kkt[3]))
+            log(f"{name}: {r['iters_per_s']:.0f} it/s, solve conv={st.converged} it={st.iterations} "
+                f"loop={st.loop_seconds:.3f}s kkt={kkt[:3]}")
+        S.close()
+        res[name] = r
+    return res
+
+
+def _line(metric_value, unit, args, workload, extra):
+    line = {"metric": METRIC, "value": metric_value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp32 iterates + fp64 checks/tail", "data": "synthetic (seeded generator, in-process)",
+            "config": {"workload": workload}}
+    line.update(extra)
+    return line
+
+
+def workload_lp0(args):
+    """The reference's CPU-runnable LP: m=2000, n=5000, 5 nnz per column."""
+    lib = sg.load_gpu()
+    model = sg.Model.generated("skw_bounded_lp", 2000, 5000, 5, 20260825)
+    arrays = _arrays(model)
+    res = _engines(lib, model, arrays, None, args, use_graph=1)
+    extra = {"engines": res}
+    if not args.skip_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import sk_oracle as so
+        kind = "reference" if so.load_ref() is not None else "port"
+        fn = so.first_order_ref if kind == "reference" else so.first_order_orc
+        t = time.perf_counter()
+        rc, s = fn(model, 0)
+        dt = time.perf_counter() - t
+        extra["cpu_baseline"] = {"value": s.iterations / dt, "unit": "PDHG iterations/s", "cores": 1, "kind": kind,
+                                 "sample": f"full reference CPU PDHG run: {s.iterations} iterations, {dt:.2f} s, "
+                                           f"result {sg.RESULT_NAMES[s.result]}, objective {s.objective:.10g}"}
+    mix = res["mixed"]
+    if not args.skip_solve and mix.get("converged") and res["fp64"].get("converged"):
+        extra["speedup_time_to_kkt_vs_fp64"] = round(res["fp64"]["loop_s"] / mix["loop_s"], 3)
+    return _line(round(mix["iters_per_s"], 1), "PDHG iterations/s", args, "LP m=2000 n=5000 (config 0)", extra)
+
+
+def workload_qp2(args):
+    """Convex QP: n=500,000, m=250,000, Q = D + B'B."""
+    lib = sg.load_gpu()
+    t = time.perf_counter()
+    model = sg.Model.generated("skw_sparse_qp", 250000, 500000, 6, 50000, 3, 20260827)
+    arrays = _arrays(model)
+    q = model.q_csc()
+    log(f"QP built in {time.perf_counter() - t:.1f}s, nnz(A)={int(arrays[0][-1])} nnz(Q)={int(q[0][-1])}")
+    res = _engines(lib, model, arrays, q, args)
+    extra = {"engines": res, "nnz_A": int(arrays[0][-1]), "nnz_Q": int(q[0][-1])}
+    mix = res["mixed"]
+    if not args.skip_solve and mix.get("converged") and res["fp64"].get("converged"):
+        extra["speedup_time_to_kkt_vs_fp64"] = round(res["fp64"]["loop_s"] / mix["loop_s"], 3)
+    return _line(round(mix["iters_per_s"], 1), "PDHG iterations/s", args,
+                 "convex QP n=500000 m=250000, A 6/col, Q=D+B'B (config 2)", extra)
+
+
+def workload_batch3(args):
+    """4096 small LPs in one persistent launch."""
+    count = 4096 if not args.quick else 256
+    w = sg.load_workloads()
+    sizes = (C.c_int * count)()
+    seed = 20260828
+    w.skw_batch_sizes(count, 20, 400, seed, sizes)
+    t = time.perf_counter()
+    models = [sg.Model.generated("skw_bounded_lp", sizes[i], 2 * sizes[i], 5, seed * count + i) for i in range(count)]
+    log(f"{count} instances built in {time.perf_counter() - t:.1f}s")
+    rc, xs, ys, res, st = sg.batch_solve(models, eps_optimal=1e-6, max_iterations=200000)
+    if rc != 0:
+        raise RuntimeError(sg.last_error())
+    its = np.array([r.iterations for r in res])
+    conv = np.array([r.status == 0 for r in res])
+    log(f"batch: device {st.device_seconds:.3f}s setup {st.setup_seconds:.2f}s converged {conv.sum()}/{count} "
+        f"threads {st.block_threads} tune {list(st.tune_ms)}")
+    extra = {"batch": {"instances": count, "converged": int(conv.sum()), "device_s": st.device_seconds,
+                       "setup_s": st.setup_seconds, "block_threads": st.block_threads, "grid": st.grid,
+                       "tune_ms": list(st.tune_ms),
+                       "iterations": {"min": int(its.min()), "median": float(np.median(its)),
+                                      "p90": float(np.percentile(its, 90)), "max": int(its.max()),
+                                      "total": int(its.sum())}}}
+    # plain FP64 GPU PDHG, one instance at a time, on a sample
+    lib = sg.load_gpu()
+    sample = list(range(0, count, max(1, count // 32)))
+    t = time.perf_counter()
+    for i in sample:
+        S = _session_qp(lib, models[i], _arrays(models[i]), None, precision=sg.PREC_FP64, use_graph=1)
+        S.solve()
+        S.close()
+    seq = (time.perf_counter() - t) / len(sample)
+    extra["sequential_fp64_s_per_instance"] = seq
+    extra["batch_speedup_vs_sequential_fp64"] = round(seq * count / st.device_seconds, 1)
+    if not args.skip_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import sk_oracle as so
+        kind = "reference" if so.load_ref() is not None else "port"
+        fn = so.first_order_ref if kind == "reference" else so.first_order_orc
+        t = time.perf_counter()
+        done = 0
+        while done < count and time.perf_counter() - t < 20.0:
+            fn(models[done], 0)
+            done += 1
+        dt = time.perf_counter() - t
+        extra["cpu_baseline"] = {"value": done / dt, "unit": "instances/s", "cores": 1, "kind": kind,
+                                 "sample": f"first {done} instances through the reference CPU PDHG to its own stop"}
+    return _line(round(count / st.device_seconds, 1), "LP instances/s (to 1e-6 KKT)", args,
+                 "4096 bounded LPs m~U{20..400}, n=2m, 5 nnz/col (config 3)", extra)
+
+
+def workload_warm4(args):
+    """Learned warm start on the parametric LP family (m=20,000, n=50,000)."""
+    lib = sg.load_gpu()
+    w = sg.load_workloads()
+    m, n, dim = (20000, 50000, 16) if not args.quick else (2000, 5000, 16)
+    seed = 20260829
+    model = sg.Model()
+    P = np.zeros(n * dim); R = np.zeros(m * dim); c0 = np.zeros(n); b0 = np.zeros(m)
+    if w.skw_parametric_family(C.byref(model.m), m, n, 6, dim, seed, sg.dptr(P), sg.dptr(R), sg.dptr(c0),
+                               sg.dptr(b0)) != 0:
+        raise RuntimeError("family generation failed")
+    model._generated = True
+    P = P.reshape(n, dim); R = R.reshape(m, dim)
+    th_train = np.zeros(args.train * dim); th_test = np.zeros(args.test * dim)
+    w.skw_normal_matrix(args.train, dim, seed + 1, sg.dptr(th_train))
+    w.skw_normal_matrix(args.test, dim, seed + 2, sg.dptr(th_test))
+    th_train = th_train.reshape(args.train, dim); th_test = th_test.reshape(args.test, dim)
+    l, u = model.vec("clow"), model.vec("cupp")
+
+    def instance(theta):
+        half = 0.5 + 0.1 * np.abs(R @ theta)
+        return c0 + P @ theta, b0 - half, b0 + half
+
+    arrays = _arrays(model)
+    eps_train = 1e-4
+    S = _session_qp(lib, model, arrays, None, precision=sg.PREC_MIXED, use_graph=1, eps_optimal=eps_train)
+    Y = np.zeros((args.train, n + m), np.float32)
+    t0 = time.perf_counter()
+    gen_iters = []
+    xp = yp = None
+    for k in range(args.train):
+        c, rl, ru = instance(th_train[k])
+        S.update(c, rl, ru, l, u)
+        rc, x, y, st = S.solve(xp, yp)   # plain PDHG, warm-started from the previous training solution
+        Y[k, :n] = x; Y[k, n:] = y
+        xp, yp = x, y
+        gen_iters.append(st.iterations)
+        if (k + 1) % 500 == 0:
+            log(f"training data {k + 1}/{args.train}: {time.perf_counter() - t0:.1f}s, "
+                f"median iterations {np.median(gen_iters):.0f}")
+    gen_s = time.perf_counter() - t0
+    S.close()
+    X = th_train.astype(np.float32)
+    net = sg.Mlp(dim, 512, n + m, seed=seed)
+    ysd = Y.std(0)
+    net.set_scaling(X.mean(0), X.std(0), Y.mean(0), np.where(ysd > 1e-6, ysd, 1.0))
+    t = time.perf_counter()
+    hist = net.train(X, Y, epochs=args.epochs, batch=256, lr=1e-3)
+    train_s = time.perf_counter() - t
+    log(f"MLP trained in {train_s:.1f}s, loss {hist[0]:.4f} -> {hist[-1]:.4f}")
+    t = time.perf_counter()
+    Yhat = net.predict(th_test.astype(np.float32))
+    infer_s = time.perf_counter() - t
+    # cold versus warm on held-out instances, both to 1e-6 relative KKT
+    S = _session_qp(lib, model, arrays, None, precision=sg.PREC_MIXED, use_graph=1, eps_optimal=1e-6,
+                    time_limit_seconds=60.0)
+    cold_it, warm_it, cold_s, warm_s, errs, fallbacks, conv_c, conv_w = [], [], [], [], [], 0, 0, 0
+    for k in range(min(args.compare, args.test)):
+        c, rl, ru = instance(th_test[k])
+        S.update(c, rl, ru, l, u)
+        rc, xc, yc, stc = S.solve()
+        x0 = np.clip(Yhat[k, :n].astype(np.float64), l, u)
+        y0 = Yhat[k, n:].astype(np.float64)
+        rc, xw, yw, stw = S.solve(x0, y0)
+        if not stw.converged and stc.converged:
+            fallbacks += 1               # the cold start is the fallback
+        cold_it.append(stc.iterations); warm_it.append(stw.iterations)
+        cold_s.append(stc.loop_seconds); warm_s.append(stw.loop_seconds)
+        conv_c += stc.converged; conv_w += stw.converged
+        sol = np.concatenate([xc, yc])
+        errs.append(float(np.linalg.norm(Yhat[k] - sol) / max(np.linalg.norm(sol), 1e-12)))
+    S.close()
+    extra = {"family": {"m": m, "n": n, "theta_dim": dim, "train": args.train, "test": args.test,
+                        "compared": len(cold_it)},
+             "training_data": {"seconds": gen_s, "eps": eps_train, "median_iterations": float(np.median(gen_iters))},
+             "mlp": {"params": net.n_params, "epochs": args.epochs, "train_s": train_s,
+                     "loss_first": float(hist[0]), "loss_last": float(hist[-1]), "inference_s_all_test": infer_s,
+                     "median_rel_error_vs_cold_solution": float(np.median(errs))},
+             "cold": {"converged": conv_c, "median_iterations": float(np.median(cold_it)),
+                      "mean_seconds": float(np.mean(cold_s))},
+             "warm": {"converged": conv_w, "median_iterations": float(np.median(warm_it)),
+                      "mean_seconds": float(np.mean(warm_s)), "fallbacks": fallbacks},
+             "speedup_time_warm_vs_cold": float(np.mean(cold_s) / max(np.mean(warm_s), 1e-12)),
+             "speedup_iterations_warm_vs_cold": float(np.median(cold_it) / max(np.median(warm_it), 1))}
+    log(json.dumps(extra))
+    return _line(round(extra["speedup_time_warm_vs_cold"], 3), "x wall-time speedup (warm vs cold, to 1e-6)", args,
+                 "parametric LP m=20000 n=50000, theta in R^16, MLP 16-512-512-70000 (config 4)", extra)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--workload", default="lp1", choices=sorted(WORKLOADS),
+                    help="lp1 (default, headline) | lp0 | qp2 | batch3 | warm4")
+    ap.add_argument("--skip-solve", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=100)
+    ap.add_argument("--quick", action="store_true", help="small instance (smoke)")
+    ap.add_argument("--solve-time-limit", type=float, default=150.0)
+    ap.add_argument("--train", type=int, default=16000, help="warm4: training instances")
+    ap.add_argument("--test", type=int, default=4000, help="warm4: held-out instances")
+    ap.add_argument("--compare", type=int, default=100, help="warm4: held-out instances solved cold and warm")
+    ap.add_argument("--epochs", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    line = WORKLOADS[args.workload](args)
     print(json.dumps(line), flush=True)
 
+
+WORKLOADS = {"lp1": workload_lp1, "lp0": workload_lp0, "qp2": workload_qp2, "batch3": workload_batch3, "warm4": workload_warm4}
 
 if __name__ == "__main__":
     main()

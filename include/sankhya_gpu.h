@@ -402,6 +402,35 @@ int  sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_op
                         sk_gpu_batch_stats *stats);
 
 /* ------------------------------------------------------------------------
+ * Learned warm start.  No reference counterpart (the reference cold-starts
+ * every PDHG solve, c/src/sk_solve.c:871).  A 3-layer MLP
+ * inputs -> hidden -> hidden -> outputs with ReLU, trained and evaluated on
+ * the GPU by hand-written BF16 MFMA kernels with FP32 accumulation and FP32
+ * master weights (Adam).  Inputs and targets are standardised with the
+ * means / standard deviations given to sk_gpu_mlp_set_scaling; the loss is
+ * the mean squared error of the standardised targets.  Parameter order in
+ * get/set/gradients: W1 [in x hid], b1, W2 [hid x hid], b2, W3 [hid x out],
+ * b3 (row-major, x @ W).
+ * ---------------------------------------------------------------------- */
+typedef struct sk_gpu_mlp sk_gpu_mlp;
+int       sk_gpu_mlp_create(int inputs, int hidden, int outputs, unsigned long long seed,
+                            sk_gpu_mlp **mlp);
+void      sk_gpu_mlp_destroy(sk_gpu_mlp *mlp);
+long long sk_gpu_mlp_param_count(const sk_gpu_mlp *mlp);
+int       sk_gpu_mlp_set_scaling(sk_gpu_mlp *mlp, const float *in_mean, const float *in_std,
+                                 const float *out_mean, const float *out_std);
+int       sk_gpu_mlp_get_params(sk_gpu_mlp *mlp, float *params);
+int       sk_gpu_mlp_set_params(sk_gpu_mlp *mlp, const float *params);
+/* X [count x inputs], Y [count x outputs] host FP32; one Adam step per
+ * batch of `batch` rows, epoch_loss[epochs] receives the mean batch loss. */
+int       sk_gpu_mlp_train(sk_gpu_mlp *mlp, const float *X, const float *Y, int count, int epochs,
+                           int batch, float lr, unsigned long long seed, float *epoch_loss);
+int       sk_gpu_mlp_predict(sk_gpu_mlp *mlp, const float *X, int count, float *Y);
+/* Loss and the flat gradient of one batch (for checking against an oracle). */
+int       sk_gpu_mlp_gradients(sk_gpu_mlp *mlp, const float *X, const float *Y, int count,
+                               float *loss, float *grad);
+
+/* ------------------------------------------------------------------------
  * sk_solve-style entry point.  Mirrors sk_solve (c/include/sankhya.h:173,
  * c/src/sk_solve.c:1009-1046) for continuous LP / convex QP, running the
  * GPU engine; OPTIMAL is reported only after the FP64 certificate computed
@@ -429,6 +458,16 @@ int sk_gpu_relative_kkt(int rows, int cols, int nnz, const int *row_offsets,
                         const double *row_upper, const double *col_lower,
                         const double *col_upper, const double *x,
                         const double *y, double out[5]);
+
+/* The same triple with an optional full symmetric Hessian (CSR, n x n):
+ * primal objective c'x + x'Qx/2, reduced cost c + Qx + A'y. */
+int sk_gpu_relative_kkt_qp(int rows, int cols, int nnz, const int *row_offsets,
+                           const int *column_indices, const double *values,
+                           const int *q_row_offsets, const int *q_column_indices,
+                           const double *q_values, int q_nnz, const double *c,
+                           const double *row_lower, const double *row_upper,
+                           const double *col_lower, const double *col_upper, const double *x,
+                           const double *y, double out[5]);
 
 #ifdef __cplusplus
 }

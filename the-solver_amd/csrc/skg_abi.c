@@ -453,13 +453,14 @@ sk_status sk_gpu_verify(const sk_model *m, sk_solution *s)
     if (s->rc && s->y && m->ncol) skk_d2h(s->rc, w.rc, sizeof(double) * (size_t)m->ncol);
     vws_free(&w);
     {
-        const double pobj = m->objshift + out[0] + (m->Q ? 0.5 * out[5] : 0.0);
-        const double dobj = m->objshift + out[4];
+        /* out: c'x, dual-objective terms, x'Qx, pinf, dinf, max complementarity product */
+        const double pobj = m->objshift + out[0] + (m->Q ? 0.5 * out[2] : 0.0);
+        const double dobj = m->objshift + out[1];
         s->objective = pobj;
-        s->primal_infeasibility = out[1] > 0.0 ? out[1] : 0.0;
+        s->primal_infeasibility = out[3] > 0.0 ? out[3] : 0.0;
         if (s->y) {
-            s->dual_infeasibility = out[2] > 0.0 ? out[2] : 0.0;
-            s->complementarity = m->Q ? out[3] / (1.0 + fabs(pobj)) : fabs(pobj - dobj) / (1.0 + fabs(pobj));
+            s->dual_infeasibility = out[4] > 0.0 ? out[4] : 0.0;
+            s->complementarity = m->Q ? out[5] / (1.0 + fabs(pobj)) : fabs(pobj - dobj) / (1.0 + fabs(pobj));
         } else {
             s->dual_infeasibility = INFINITY;
             s->complementarity = INFINITY;
@@ -519,6 +520,26 @@ int sk_gpu_relative_kkt(int rows, int cols, int nnz, const int *row_offsets, con
     A.rows = rows; A.cols = cols; A.nnz = nnz;
     A.p = (int *)row_offsets; A.j = (int *)column_indices; A.v = (double *)values;
     return relkkt_hcsr(&A, NULL, c, row_lower, row_upper, col_lower, col_upper, x, y, out);
+}
+
+int sk_gpu_relative_kkt_qp(int rows, int cols, int nnz, const int *row_offsets, const int *column_indices,
+                           const double *values, const int *q_row_offsets, const int *q_column_indices,
+                           const double *q_values, int q_nnz, const double *c, const double *row_lower,
+                           const double *row_upper, const double *col_lower, const double *col_upper,
+                           const double *x, const double *y, double out[5])
+{
+    hcsr A, Q;
+    skg_clear_error();
+    if (!c || !row_lower || !row_upper || !col_lower || !col_upper || !x || (rows && !y) || !out)
+        return skg_fail("null input");
+    if (hcsr_validate(rows, cols, nnz, row_offsets, column_indices, values)) return -1;
+    if (q_row_offsets && hcsr_validate(cols, cols, q_nnz, q_row_offsets, q_column_indices, q_values)) return -1;
+    if (skg_have_device()) return -1;
+    A.rows = rows; A.cols = cols; A.nnz = nnz;
+    A.p = (int *)row_offsets; A.j = (int *)column_indices; A.v = (double *)values;
+    Q.rows = Q.cols = cols; Q.nnz = q_nnz;
+    Q.p = (int *)q_row_offsets; Q.j = (int *)q_column_indices; Q.v = (double *)q_values;
+    return relkkt_hcsr(&A, q_row_offsets ? &Q : NULL, c, row_lower, row_upper, col_lower, col_upper, x, y, out);
 }
 
 /* -------------------------------------------------- sk_solve mirror */

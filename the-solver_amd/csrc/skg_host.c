@@ -276,12 +276,16 @@ void skg_tuning_default(skg_tuning *t)
     t->long_row = 512;
 }
 
-/* Stable counting sort of rows by lane class: order[k] = row in slot k.
- * Fills bins (contiguous slot ranges per class, at least one block). */
-int skg_plan_bins(const int *rp, int rows, const skg_tuning *t, int *order, skk_bins *bins)
+/* Counting sort of rows by lane class: order[k] = row in slot k.  Inside a
+ * class rows keep ascending `key` order when a key is given (a counting sort
+ * by key first, key values in [0, key_range]), else their own order.  Fills
+ * bins (contiguous slot ranges per class, at least one block). */
+int skg_plan_bins_keyed(const int *rp, int rows, const skg_tuning *t, const int *key, int key_range,
+                        int *order, skk_bins *bins)
 {
     int cls_count[9] = {0}, cls_start[9], k, c, nb = 0, blocks = 0;
     unsigned char *cls = (unsigned char *)malloc((size_t)(rows > 0 ? rows : 1));
+    int *pre = NULL;
     if (!cls) return -1;
     for (k = 0; k < rows; ++k) {
         cls[k] = (unsigned char)skg_lanes_log2(rp[k + 1] - rp[k], t);
@@ -312,11 +316,30 @@ int skg_plan_bins(const int *rp, int rows, const skg_tuning *t, int *order, skk_
     bins->total_blocks = blocks;
     if (order) {
         int next[9];
+        if (key) {
+            /* rows in ascending key order (stable), then a stable pass by class */
+            int *cnt = (int *)calloc((size_t)key_range + 2, sizeof(int));
+            pre = (int *)malloc(sizeof(int) * (size_t)(rows > 0 ? rows : 1));
+            if (!cnt || !pre) { free(cnt); free(pre); free(cls); return -1; }
+            for (k = 0; k < rows; ++k) cnt[key[k] + 1]++;
+            for (k = 0; k <= key_range; ++k) cnt[k + 1] += cnt[k];
+            for (k = 0; k < rows; ++k) pre[cnt[key[k]]++] = k;
+            free(cnt);
+        }
         memcpy(next, cls_start, sizeof(next));
-        for (k = 0; k < rows; ++k) order[next[cls[k]]++] = k;
+        for (k = 0; k < rows; ++k) {
+            const int r = pre ? pre[k] : k;
+            order[next[cls[r]]++] = r;
+        }
     }
+    free(pre);
     free(cls);
     return 0;
+}
+
+int skg_plan_bins(const int *rp, int rows, const skg_tuning *t, int *order, skk_bins *bins)
+{
+    return skg_plan_bins_keyed(rp, rows, t, NULL, 0, order, bins);
 }
 
 /* ----------------------------------------------------------- device CSR */

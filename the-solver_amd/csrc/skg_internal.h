@@ -39,6 +39,8 @@ typedef struct skg_tuning {
 void skg_tuning_default(skg_tuning *t);
 int  skg_lanes_log2(int len, const skg_tuning *t);
 int  skg_plan_bins(const int *rp, int rows, const skg_tuning *t, int *order, skk_bins *bins);
+int  skg_plan_bins_keyed(const int *rp, int rows, const skg_tuning *t, const int *key, int key_range,
+                         int *order, skk_bins *bins);
 
 /* Device CSR with its bin plan. */
 typedef struct dcsr {

@@ -139,9 +139,10 @@ int skk_pdhg_set_eta(const skk_pdhg *P, double eta);
 /* ------------------------------------------------------ certificates */
 
 /* sk_verify (c/src/sk_core.c:394-510) on the device in FP64 over the
- * original model: out[0] objective (without objshift), out[1] primal
- * infeasibility, out[2] dual infeasibility, out[3] complementarity max
- * product, out[4] dual objective contribution.  act[m], rc[n] are written. */
+ * original model: out[0] c'x, out[1] dual-objective terms, out[2] x'Qx,
+ * out[3] primal infeasibility, out[4] dual infeasibility, out[5] largest
+ * complementarity product.  act[m], rc[n] are written.  part holds
+ * 8 x grid doubles. */
 typedef struct skk_verify_in {
     int m, n;
     skk_csr A, AT;

@@ -124,60 +124,45 @@ __device__ __forceinline__ double load_coherent(const double *p)
     return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// True in exactly one workgroup: the last to retire.  The ticket counter
-// wraps back to zero, so it needs no reset launch (graph-safe).
-__device__ __forceinline__ bool last_block(unsigned *counter)
-{
-    __shared__ int is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned prev =
-            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (prev == gridDim.x - 1);
-        if (is_last) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    if (is_last) __threadfence();
-    return is_last != 0;
-}
-
-// Block partials of S slots -> part[s*stride + blockIdx.x]; if this is the
-// last block, totals[s] (ascending block order) are returned in every thread
-// of that block.
+// Block partials of S slots -> part[s * stride + blockIdx.x].  A later
+// single-workgroup launch combines them: kernel boundaries order the writes,
+// so no workgroup ever fences (an agent-scope fence on a multi-XCD part
+// writes back and invalidates L2, which would evict the gathered vectors).
 template <int S, bool kMax = false>
-__device__ bool partials_and_total(double (&v)[S], double *part, int stride, unsigned *counter,
-                                   double (&totals)[S])
+__device__ __forceinline__ void store_partials(double (&v)[S], double *part, int stride)
 {
     double blk[S];
     block_reduce<S, kMax>(v, blk);
-    if (threadIdx.x < S) {
-#pragma unroll
-        for (int s = 0; s < S; ++s)
-            if (s == (int)threadIdx.x) store_coherent(part + (size_t)s * stride + blockIdx.x, blk[s]);
-    }
-    if (!last_block(counter)) return false;
-    double acc[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        acc[s] = kMax ? -INFINITY : 0.0;
-        for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock) {
-            const double t = load_coherent(part + (size_t)s * stride + b);
-            acc[s] = kMax ? fmax(acc[s], t) : acc[s] + t;
-        }
-    }
-    block_reduce<S, kMax>(acc, totals);
-    return true;
+    if (threadIdx.x == 0)
+        for (int s = 0; s < S; ++s) part[(size_t)s * stride + blockIdx.x] = blk[s];
 }
 
-// Reduce `count` partials of one slot written by an earlier launch.
-__device__ double total_of(const double *part, int count)
+// Combine `count` partials of one slot: strided per-thread sums in a fixed
+// order, then the workgroup tree, so the total is bit-reproducible.
+__device__ double combine(const double *part, int count, bool is_max)
 {
-    double acc[1] = {0.0};
-    for (int b = threadIdx.x; b < count; b += kBlock) acc[0] += load_coherent(part + b);
+    double acc[1] = {is_max ? -INFINITY : 0.0};
+    for (int b = threadIdx.x; b < count; b += kBlock) acc[0] = is_max ? fmax(acc[0], part[b]) : acc[0] + part[b];
     double out[1];
-    block_reduce<1>(acc, out);
+    if (is_max) block_reduce<1, true>(acc, out);
+    else block_reduce<1>(acc, out);
     return out[0];
+}
+
+// out[s] for slots [0, nsum) (sums) then [nsum, nsum + nmax) (maxima).
+__global__ __launch_bounds__(kBlock) void k_finalize(const double *part, int stride, int count,
+                                                     int nsum, int nmax, double *out)
+{
+    for (int s = 0; s < nsum + nmax; ++s) {
+        const double t = combine(part + (size_t)s * stride, count, s >= nsum);
+        if (threadIdx.x == 0) out[s] = t;
+    }
+}
+
+int finalize(const double *part, int stride, int count, int nsum, int nmax, double *out)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, stream(), part, stride, count, nsum, nmax, out);
+    return launch_status();
 }
 
 // ----------------------------------------------------------- binned SpMV
@@ -408,24 +393,18 @@ template <typename T, typename AT> struct EpiPrimalAdaptive {
     }
     __device__ void end(double *p) const
     {
-        double v[1] = {p[0]}, out[1];
-        block_reduce<1>(v, out);
-        if (threadIdx.x == 0) store_coherent(part + blockIdx.x, out[0]);
+        double v[1] = {p[0]};
+        store_partials<1>(v, part, 0);
     }
 };
 
-// A x_cand epilogue: AX[a2] = A x_cand; partial ||dy||^2 and dy'(A dx).  The
-// last workgroup takes the accept/reject decision and the next step size
-// (PdlpKernels.cu:454-493) and commits by rotating buffer indices.
+// A x_cand epilogue: AX[a2] = A x_cand; partials of ||dy||^2 and dy'(A dx).
 template <typename T, typename AT> struct EpiProbe {
     static constexpr int kSlots = 2;
     Iter<T> I;
-    skk_state *st;
+    const skk_state *st;
     double *part;       // region base
     int stride;         // part_capacity
-    unsigned *counter;
-    int dx_blocks;      // workgroups of the A^T launch
-    int dq_blocks;      // workgroups of the Q launch (0: no Hessian)
     const T *y, *yc, *ax;
     T *axc;
     __device__ void begin()
@@ -443,38 +422,47 @@ template <typename T, typename AT> struct EpiProbe {
     }
     __device__ void end(double *p) const
     {
-        double v[2] = {p[0], p[1]}, tot[2];
-        if (!partials_and_total<2>(v, part + (size_t)R_DY * stride, stride, counter, tot)) return;
-        const double dx2 = total_of(part + (size_t)R_DX * stride, dx_blocks);
-        const double dq = dq_blocks > 0 ? total_of(part + (size_t)R_DQ * stride, dq_blocks) : 0.0;
-        if (threadIdx.x != 0) return;
-        const double dy2 = tot[0];
-        // with a Hessian the local bound also covers dx'Q dx (>= 0 for convex Q)
-        const double inter = fabs(tot[1]) + 0.5 * fmax(dq, 0.0);
-        const double omega = st->omega, eta = st->eta;
-        const double movement = 0.5 * (omega * dx2 + dy2 / omega);
-        const double eta_bar = (inter > 0.0 && movement > 0.0) ? movement / inter : st->eta_max;
-        const bool accept = eta <= eta_bar;
-        const double k = st->updates + 1.0;
-        const double shrink = 1.0 - pow(k + 1.0, -0.3);
-        const double grow = 1.0 + pow(k + 1.0, -0.6);
-        double next = fmin(shrink * eta_bar, grow * eta);
-        if (!(next > 0.0) || !isfinite(next)) next = eta;
-        next = fmin(fmax(next, st->eta_min), st->eta_max);
-        st->eta = next;
-        st->eta_bar = eta_bar;
-        st->dx2 = dx2; st->dy2 = dy2; st->inter = inter;
-        st->updates = k;
-        st->accepted = accept ? 1 : 0;
-        st->pending = accept ? 1 : 0;
-        if (accept) {
-            st->accept_count += 1.0;
-            st->cur ^= 1;
-            const int a0 = st->a0, a1 = st->a1, a2 = st->a2;
-            st->a0 = a2; st->a1 = a0; st->a2 = a1;   // x_prev <- x, x <- cand
-        }
+        double v[2] = {p[0], p[1]};
+        store_partials<2>(v, part + (size_t)R_DY * stride, stride);
     }
 };
+
+// One workgroup after the A x_cand launch: combine the partials, take the
+// accept/reject decision and the next step size (PdlpKernels.cu:454-493)
+// and commit by rotating buffer indices.
+__global__ __launch_bounds__(kBlock) void k_probe_finalize(skk_state *st, const double *part, int stride,
+                                                           int dx_blocks, int dq_blocks, int dy_blocks)
+{
+    const double dx2 = combine(part + (size_t)R_DX * stride, dx_blocks, false);
+    const double dq = dq_blocks > 0 ? combine(part + (size_t)R_DQ * stride, dq_blocks, false) : 0.0;
+    const double dy2 = combine(part + (size_t)R_DY * stride, dy_blocks, false);
+    const double dya = combine(part + (size_t)R_INTER * stride, dy_blocks, false);
+    if (threadIdx.x != 0) return;
+    // with a Hessian the local bound also covers dx'Q dx (>= 0 for convex Q)
+    const double inter = fabs(dya) + 0.5 * fmax(dq, 0.0);
+    const double omega = st->omega, eta = st->eta;
+    const double movement = 0.5 * (omega * dx2 + dy2 / omega);
+    const double eta_bar = (inter > 0.0 && movement > 0.0) ? movement / inter : st->eta_max;
+    const bool accept = eta <= eta_bar;
+    const double k = st->updates + 1.0;
+    const double shrink = 1.0 - pow(k + 1.0, -0.3);
+    const double grow = 1.0 + pow(k + 1.0, -0.6);
+    double next = fmin(shrink * eta_bar, grow * eta);
+    if (!(next > 0.0) || !isfinite(next)) next = eta;
+    next = fmin(fmax(next, st->eta_min), st->eta_max);
+    st->eta = next;
+    st->eta_bar = eta_bar;
+    st->dx2 = dx2; st->dy2 = dy2; st->inter = inter;
+    st->updates = k;
+    st->accepted = accept ? 1 : 0;
+    st->pending = accept ? 1 : 0;
+    if (accept) {
+        st->accept_count += 1.0;
+        st->cur ^= 1;
+        const int a0 = st->a0, a1 = st->a1, a2 = st->a2;
+        st->a0 = a2; st->a1 = a0; st->a2 = a1;   // x_prev <- x, x <- cand
+    }
+}
 
 // Q x_cand epilogue: QX[cur^1] = Q x_cand; partial dx'(Q x_cand - Q x).
 template <typename T, typename AT> struct EpiQProbe {
@@ -498,9 +486,8 @@ template <typename T, typename AT> struct EpiQProbe {
     }
     __device__ void end(double *p) const
     {
-        double v[1] = {p[0]}, out[1];
-        block_reduce<1>(v, out);
-        if (threadIdx.x == 0) store_coherent(part + blockIdx.x, out[0]);
+        double v[1] = {p[0]};
+        store_partials<1>(v, part, 0);
     }
 };
 
@@ -591,7 +578,6 @@ __global__ __launch_bounds__(kBlock) void k_residuals_sel(Iter<T> I, int which,
                                                           const double *__restrict__ aty,
                                                           Data<double> D, int n, int m,
                                                           double *part, int stride,
-                                                          unsigned *counter, double *out,
                                                           const skk_state *__restrict__ st,
                                                           const double *__restrict__ qx)
 {
@@ -632,10 +618,7 @@ __global__ __launch_bounds__(kBlock) void k_residuals_sel(Iter<T> I, int which,
         else if (yi < 0.0) support = isfinite(rl) ? yi * rl : 0.0;
         acc[3] -= support;
     }
-    double tot[8];
-    if (!partials_and_total<8>(acc, part, stride, counter, tot)) return;
-    if (threadIdx.x == 0)
-        for (int s = 0; s < 8; ++s) out[s] = tot[s];
+    store_partials<8>(acc, part, stride);
 }
 
 // AX[a0] <- A x from the FP64 product of the evaluation.
@@ -713,16 +696,13 @@ __global__ void k_export(Iter<T> I, const skk_state *__restrict__ st, int n, int
 
 struct EpiNormSq {
     static constexpr int kSlots = 1;
-    double *y, *part, *out;
-    int stride;
-    unsigned *counter;
+    double *y, *part;
     __device__ void begin() {}
     __device__ void row(int r, double s, double *p) const { y[r] = s; p[0] += s * s; }
     __device__ void end(double *p) const
     {
-        double v[1] = {p[0]}, tot[1];
-        if (!partials_and_total<1>(v, part, stride, counter, tot)) return;
-        if (threadIdx.x == 0) *out = tot[0];
+        double v[1] = {p[0]};
+        store_partials<1>(v, part, 0);
     }
 };
 
@@ -818,14 +798,8 @@ __global__ __launch_bounds__(kBlock) void k_verify(skk_verify_in V, int have_y)
         }
     }
     if (bad_x) mx[0] = INFINITY;
-    double ts[3], tm[3];
-    const int stride = V.grid;
-    const bool last_s = partials_and_total<3>(s, V.part, stride, V.counters + 0, ts);
-    // second reduction over the same grid with its own ticket
-    const bool last_m = partials_and_total<3, true>(mx, V.part + 3 * (size_t)stride, stride,
-                                                    V.counters + 1, tm);
-    if (last_s && threadIdx.x == 0) { V.out[0] = ts[0]; V.out[4] = ts[1]; V.out[5] = ts[2]; }
-    if (last_m && threadIdx.x == 0) { V.out[1] = tm[0]; V.out[2] = tm[1]; V.out[3] = tm[2]; }
+    store_partials<3>(s, V.part, V.grid);
+    store_partials<3, true>(mx, V.part + 3 * (size_t)V.grid, V.grid);
 }
 
 // Relative-KKT sums in the original space (|b| >= 1e30 absent), with
@@ -862,10 +836,7 @@ __global__ __launch_bounds__(kBlock) void k_relkkt(skk_verify_in V)
         else if (yi < 0.0) support = isfinite(rl) ? yi * rl : 0.0;
         acc[3] -= support;
     }
-    double tot[5];
-    if (!partials_and_total<5>(acc, V.part, V.grid, V.counters + 2, tot)) return;
-    if (threadIdx.x == 0)
-        for (int s = 0; s < 5; ++s) V.out[s] = tot[s];
+    store_partials<5>(acc, V.part, V.grid);
 }
 
 // ------------------------------------------------ plain PDHG (legacy ABI)
@@ -974,13 +945,8 @@ __global__ __launch_bounds__(kBlock) void k_legacy_check(LegacyCheck K)
         if (mu < 0.0 && has_lo(rl)) mx[5] = fmax(mx[5], -mu * fabs(a - rl));
         else if (mu > 0.0 && has_hi(ru)) mx[5] = fmax(mx[5], mu * fabs(ru - a));
     }
-    double ts[2], tm[6];
-    const bool ls = partials_and_total<2>(s, K.part, K.stride, K.counters + 0, ts);
-    const bool lm = partials_and_total<6, true>(mx, K.part + 2 * (size_t)K.stride, K.stride,
-                                                K.counters + 1, tm);
-    if (ls && threadIdx.x == 0) { K.out[0] = ts[0]; K.out[1] = ts[1]; }
-    if (lm && threadIdx.x == 0)
-        for (int k = 0; k < 6; ++k) K.out[2 + k] = tm[k];
+    store_partials<2>(s, K.part, K.stride);
+    store_partials<6, true>(mx, K.part + 2 * (size_t)K.stride, K.stride);
 }
 
 __global__ void k_legacy_init(double *x, double *xbar, const double *l, const double *u, int n,
@@ -1011,6 +977,19 @@ template <typename T> int q_probe(const skk_pdhg &P, const Iter<T> &I)
     return launch_spmv<T, T, T>(P.Q, vals_of<T>(P.Q), I.X[0], I.X[1], P.st, eq);
 }
 
+// A x_cand with the probe epilogue (x_cand is X[cur ^ 1]), then the
+// single-workgroup decision.
+template <typename T> int probe(const skk_pdhg &P, const Iter<T> &I)
+{
+    EpiProbe<T, T> pr{};
+    pr.I = I; pr.st = P.st; pr.part = P.part; pr.stride = P.part_capacity;
+    int rc = launch_spmv<T, T, T>(P.A, vals_of<T>(P.A), I.X[0], I.X[1], P.st, pr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_probe_finalize, dim3(1), dim3(kBlock), 0, stream(), P.st, P.part, P.part_capacity,
+                       P.AT.bins.total_blocks, P.has_q ? P.Q.bins.total_blocks : 0, P.A.bins.total_blocks);
+    return launch_status();
+}
+
 template <typename T> int adaptive_iter(const skk_pdhg &P)
 {
     const Iter<T> I(P.it[prec_index<T>()]);
@@ -1026,12 +1005,7 @@ template <typename T> int adaptive_iter(const skk_pdhg &P)
     // A^T y_cand: y_cand is Y[cur ^ 1], picked on the device
     if ((rc = launch_spmv<T, T, T>(P.AT, vals_of<T>(P.AT), I.Y[0], I.Y[1], P.st, ep))) return rc;
     if (P.has_q && (rc = q_probe<T>(P, I))) return rc;
-    EpiProbe<T, T> pr{};
-    pr.I = I; pr.st = P.st; pr.part = P.part; pr.stride = P.part_capacity;
-    pr.counter = P.counters + C_PROBE; pr.dx_blocks = P.AT.bins.total_blocks;
-    pr.dq_blocks = P.has_q ? P.Q.bins.total_blocks : 0;
-    // A x_cand: x_cand is X[cur ^ 1]
-    return launch_spmv<T, T, T>(P.A, vals_of<T>(P.A), I.X[0], I.X[1], P.st, pr);
+    return probe<T>(P, I);
 }
 
 template <typename T> int adaptive_part(const skk_pdhg &P, int part)
@@ -1052,11 +1026,7 @@ template <typename T> int adaptive_part(const skk_pdhg &P, int part)
     }
     int rc;
     if (P.has_q && (rc = q_probe<T>(P, I))) return rc;
-    EpiProbe<T, T> pr{};
-    pr.I = I; pr.st = P.st; pr.part = P.part; pr.stride = P.part_capacity;
-    pr.counter = P.counters + C_PROBE; pr.dx_blocks = P.AT.bins.total_blocks;
-    pr.dq_blocks = P.has_q ? P.Q.bins.total_blocks : 0;
-    return launch_spmv<T, T, T>(P.A, vals_of<T>(P.A), I.X[0], I.X[1], P.st, pr);
+    return probe<T>(P, I);
 }
 
 template <typename T> int fixed_iter(const skk_pdhg &P)
@@ -1112,11 +1082,12 @@ template <typename T> int eval(const skk_pdhg &P, int which, int adaptive, doubl
         if ((rc = launch_spmv<double, T, double>(P.Q, P.Q.vd, x, x1, sel, sq))) return rc;
     }
     const Data<double> D = data_of<double>(P);
+    double *rp = P.part + (size_t)R_EVAL * P.part_capacity;
     hipLaunchKernelGGL(k_residuals_sel<T>, dim3(P.elem_grid), dim3(kBlock), 0, stream(), I, which,
-                       P.ax_d, P.aty_d, D, P.n, P.m, P.part + (size_t)R_EVAL * P.part_capacity,
-                       P.part_capacity, P.counters + C_EVAL, P.eval_out + 8 * which, P.st,
+                       P.ax_d, P.aty_d, D, P.n, P.m, rp, P.part_capacity, P.st,
                        P.has_q ? P.qx_d : nullptr);
     if ((rc = launch_status())) return rc;
+    if ((rc = finalize(rp, P.part_capacity, P.elem_grid, 8, 0, P.eval_out + 8 * which))) return rc;
     if (!which && adaptive) {
         // the live A x (and Q x) are refreshed from the FP64 products (GpuPdlp.cpp:367)
         hipLaunchKernelGGL(k_to_t_sel<T>, dim3(P.elem_grid), dim3(kBlock), 0, stream(), I, P.ax_d,
@@ -1428,9 +1399,9 @@ int skk_power_norm(const skk_pdhg *P, double *v, double *w, double *v2, int iter
         EpiStore<double, double> e{w};
         if ((rc = launch_spmv<double, double, double>(P->A, P->A.vd, v, v, nullptr, e))) return rc;
         EpiNormSq ns{};
-        ns.y = v2; ns.part = P->part + (size_t)R_POWER * P->part_capacity; ns.out = out;
-        ns.stride = P->part_capacity; ns.counter = P->counters + C_POWER;
+        ns.y = v2; ns.part = P->part + (size_t)R_POWER * P->part_capacity;
         if ((rc = launch_spmv<double, double, double>(P->AT, P->AT.vd, w, w, nullptr, ns))) return rc;
+        if ((rc = finalize(ns.part, P->part_capacity, P->AT.bins.total_blocks, 1, 0, out))) return rc;
         hipLaunchKernelGGL(k_normalize_from, dim3(grid_for(n, P->elem_grid)), dim3(kBlock), 0,
                            stream(), v, v2, out, n);
         if ((rc = launch_status())) return rc;
@@ -1456,7 +1427,8 @@ int skk_verify(const skk_verify_in *V)
     skk_verify_in v = *V;
     if (!V->Q) v.qx = nullptr;
     hipLaunchKernelGGL(k_verify, dim3(V->grid), dim3(kBlock), 0, stream(), v, V->y ? 1 : 0);
-    return launch_status();
+    if ((rc = launch_status())) return rc;
+    return finalize(V->part, V->grid, V->grid, 3, 3, V->out);
 }
 
 int skk_relative_kkt(const skk_verify_in *V)
@@ -1464,7 +1436,9 @@ int skk_relative_kkt(const skk_verify_in *V)
     skk_verify_in v = *V;
     if (!V->Q) v.qx = nullptr;
     hipLaunchKernelGGL(k_relkkt, dim3(V->grid), dim3(kBlock), 0, stream(), v);
-    return launch_status();
+    int rc = launch_status();
+    if (rc) return rc;
+    return finalize(V->part, V->grid, V->grid, 5, 0, V->out);
 }
 
 
@@ -1503,7 +1477,8 @@ int skk_legacy_check(const skk_legacy *L, int have_prev)
     LegacyCheck K{L->m, L->n, have_prev, L->x, L->y, L->act, L->grad, L->Q ? L->qg : nullptr, L->qd,
                   L->c, L->l, L->u, L->rl, L->ru, L->prev, L->part, L->grid, L->counters, L->out};
     hipLaunchKernelGGL(k_legacy_check, dim3(L->grid), dim3(kBlock), 0, stream(), K);
-    return launch_status();
+    if ((rc = launch_status())) return rc;
+    return finalize(L->part, L->grid, L->grid, 2, 6, L->out);
 }
 
 }  // extern "C"

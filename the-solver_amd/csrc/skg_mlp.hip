@@ -1,0 +1,320 @@
+// Learned warm start: a 3-layer MLP trained and evaluated on the GPU with
+// hand-written BF16 MFMA kernels (FP32 accumulation, FP32 master weights).
+//
+// No reference counterpart: the reference always cold-starts PDHG
+// (c/src/sk_solve.c:871, cuda/src/sankhya_cuda_lp.cu:205-213).
+//
+// GEMM: C[M x N] = op(A)[M x K] * op(B)[K x N] with op = identity or
+// transpose, 64 x 64 x 32 tiles, 256 threads = 4 wavefronts each owning a
+// 32 x 32 quadrant computed with v_mfma_f32_32x32x8bf16_1k.  Both LDS tiles
+// are stored K-contiguous ([row][k] for A, [col][k] for B) so every MFMA
+// operand is one 8-byte LDS read.  Long-K products (the gradient through
+// the 70,000-wide output layer) split K over blockIdx.z and finish in a
+// reduction kernel that applies the same epilogue.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "skg_mlp.h"
+
+namespace {
+
+hipStream_t mstream() { return static_cast<hipStream_t>(skk_stream()); }
+
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float16_t __attribute__((ext_vector_type(16)));
+
+constexpr int BM = 64, BN = 64, BK = 32, PAD = 8, THREADS = 256;
+
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    // round to nearest even (finite inputs)
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+struct Epi {
+    const float *bias;      // per column, or null
+    const float *mask;      // keep where mask[r*ldm + c] > 0, or null
+    int ldm;
+    int relu;
+    float *c32;             // fp32 output (ldc) or null
+    uint16_t *c16;          // bf16 output (ldc) or null
+    int ldc;
+};
+
+__device__ __forceinline__ void epilogue(const Epi &e, int r, int c, float v)
+{
+    if (e.bias) v += e.bias[c];
+    if (e.relu) v = fmaxf(v, 0.0f);
+    if (e.mask && !(e.mask[(size_t)r * e.ldm + c] > 0.0f)) v = 0.0f;
+    if (e.c32) e.c32[(size_t)r * e.ldc + c] = v;
+    if (e.c16) e.c16[(size_t)r * e.ldc + c] = f2bf(v);
+}
+
+// op(A)[i][k]: TA ? A[k*lda + i] : A[i*lda + k]; op(B)[k][j]: TB ? B[j*ldb + k] : B[k*ldb + j].
+// With kchunk > 0 the z-slice z covers k in [z*kchunk, (z+1)*kchunk) and the
+// raw sums go to part[z][M][N] instead of the epilogue.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(THREADS) void k_gemm(int M, int N, int K, const uint16_t *__restrict__ A,
+                                                  int lda, const uint16_t *__restrict__ B, int ldb,
+                                                  Epi e, float *part, int kchunk)
+{
+    __shared__ uint16_t As[BM][BK + PAD];
+    __shared__ uint16_t Bs[BN][BK + PAD];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int row0 = blockIdx.y * BM, col0 = blockIdx.x * BN;
+    int kbeg = 0, kend = K;
+    if (kchunk > 0) {
+        kbeg = blockIdx.z * kchunk;
+        kend = min(K, kbeg + kchunk);
+    }
+    float16_t acc;
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        // ---- A tile: BM x BK, 8 elements per thread
+        if (!TA) {
+            const int r = t >> 2, kk = (t & 3) * 8;
+            const int gi = row0 + r;
+            for (int q = 0; q < 8; ++q) {
+                const int gk = k0 + kk + q;
+                As[r][kk + q] = (gi < M && gk < kend) ? A[(size_t)gi * lda + gk] : (uint16_t)0;
+            }
+        } else {
+            const int kk = t >> 3, i0 = (t & 7) * 8;
+            const int gk = k0 + kk;
+            for (int q = 0; q < 8; ++q) {
+                const int gi = row0 + i0 + q;
+                As[i0 + q][kk] = (gi < M && gk < kend) ? A[(size_t)gk * lda + gi] : (uint16_t)0;
+            }
+        }
+        // ---- B tile: BK x BN stored [n][k]
+        if (!TB) {
+            const int kk = t >> 3, j0 = (t & 7) * 8;
+            const int gk = k0 + kk;
+            for (int q = 0; q < 8; ++q) {
+                const int gj = col0 + j0 + q;
+                Bs[j0 + q][kk] = (gj < N && gk < kend) ? B[(size_t)gk * ldb + gj] : (uint16_t)0;
+            }
+        } else {
+            const int j = t >> 2, kk = (t & 3) * 8;
+            const int gj = col0 + j;
+            for (int q = 0; q < 8; ++q) {
+                const int gk = k0 + kk + q;
+                Bs[j][kk + q] = (gj < N && gk < kend) ? B[(size_t)gj * ldb + gk] : (uint16_t)0;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 8) {
+            const int kq = ks + (lane >> 5) * 4;
+            const short4_t a = *reinterpret_cast<const short4_t *>(&As[wm * 32 + (lane & 31)][kq]);
+            const short4_t b = *reinterpret_cast<const short4_t *>(&Bs[wn * 32 + (lane & 31)][kq]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // D[i][j]: j = lane % 32, i = 8 * (r / 4) + 4 * (lane / 32) + r % 4
+    const int j = col0 + wn * 32 + (lane & 31);
+    for (int r = 0; r < 16; ++r) {
+        const int i = row0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (i >= M || j >= N) continue;
+        if (kchunk > 0) part[((size_t)blockIdx.z * M + i) * N + j] = acc[r];
+        else epilogue(e, i, j, acc[r]);
+    }
+}
+
+__global__ void k_splitk_reduce(int M, int N, int splits, const float *part, Epi e)
+{
+    const size_t total = (size_t)M * N;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        for (int z = 0; z < splits; ++z) s += part[(size_t)z * total + idx];
+        epilogue(e, (int)(idx / N), (int)(idx % N), s);
+    }
+}
+
+// dP = 2 (P - T) / (B * out); block partials of sum (P - T)^2.
+__global__ void k_mse_grad(int count, const float *P, const float *T, float scale, float *dP,
+                           uint16_t *dP16, double *part)
+{
+    double acc = 0.0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
+        const float d = P[k] - T[k];
+        acc += (double)d * d;
+        const float g = 2.0f * d * scale;
+        dP[k] = g;
+        dP16[k] = f2bf(g);
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    __shared__ double sh[THREADS / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / 64); ++w) s += sh[w];
+        part[blockIdx.x] = s;
+    }
+}
+
+__global__ void k_sum_partials(const double *part, int count, double *out)
+{
+    __shared__ double sh[THREADS];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < count; b += blockDim.x) acc += part[b];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+// db[c] = sum_r dZ[r][c]
+__global__ void k_colsum(int rows, int cols, const float *dZ, float *db)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < cols; c += gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        for (int r = 0; r < rows; ++r) s += dZ[(size_t)r * cols + c];
+        db[c] = s;
+    }
+}
+
+// Adam on the flat FP32 parameter vector; refresh the BF16 weight copies.
+__global__ void k_adam(long long n, float *p, const float *g, float *m, float *v, float lr, float b1,
+                       float b2, float eps, float c1, float c2)
+{
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x) {
+        const float gk = g[k];
+        const float mk = b1 * m[k] + (1.0f - b1) * gk;
+        const float vk = b2 * v[k] + (1.0f - b2) * gk * gk;
+        m[k] = mk;
+        v[k] = vk;
+        p[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+    }
+}
+
+__global__ void k_to_bf16(long long n, const float *src, uint16_t *dst)
+{
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x)
+        dst[k] = f2bf(src[k]);
+}
+
+// rows of src gathered by idx, standardised: dst[r][c] = (src[idx[r]][c] - mean[c]) / std[c]
+__global__ void k_gather_std(int rows, int cols, const int *idx, const float *src, const float *mean,
+                             const float *stdv, float *dst32, uint16_t *dst16)
+{
+    const size_t total = (size_t)rows * cols;
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < total;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(k / cols), c = (int)(k % cols);
+        const float v = (src[(size_t)idx[r] * cols + c] - mean[c]) / stdv[c];
+        if (dst32) dst32[k] = v;
+        if (dst16) dst16[k] = f2bf(v);
+    }
+}
+
+// y = mean + std * p
+__global__ void k_destandardize(int rows, int cols, const float *p, const float *mean, const float *stdv,
+                                float *y)
+{
+    const size_t total = (size_t)rows * cols;
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < total;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(k % cols);
+        y[k] = mean[c] + stdv[c] * p[k];
+    }
+}
+
+int grid1d(size_t n)
+{
+    size_t g = (n + THREADS - 1) / THREADS;
+    if (g < 1) g = 1;
+    if (g > 8192) g = 8192;
+    return (int)g;
+}
+
+}  // namespace
+
+extern "C" {
+
+int skm_gemm(int M, int N, int K, const uint16_t *A, int lda, int trans_a, const uint16_t *B, int ldb,
+             int trans_b, const skm_epilogue *ep, float *splitk_buf, int splits)
+{
+    Epi e{ep->bias, ep->mask, ep->ldm, ep->relu, ep->c32, ep->c16, ep->ldc};
+    const int kchunk = splits > 1 ? ((K + splits - 1) / splits + BK - 1) / BK * BK : 0;
+    const int zs = splits > 1 ? (K + kchunk - 1) / kchunk : 1;
+    const dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, zs);
+#define GO(ta, tb) hipLaunchKernelGGL((k_gemm<ta, tb>), grid, dim3(THREADS), 0, mstream(), M, N, K, A, lda, B, ldb, e, splitk_buf, kchunk)
+    if (!trans_a && !trans_b) GO(false, false);
+    else if (!trans_a && trans_b) GO(false, true);
+    else if (trans_a && !trans_b) GO(true, false);
+    else GO(true, true);
+#undef GO
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return (int)err;
+    if (splits > 1) {
+        hipLaunchKernelGGL(k_splitk_reduce, dim3(grid1d((size_t)M * N)), dim3(THREADS), 0, mstream(), M, N, zs,
+                           splitk_buf, e);
+        err = hipGetLastError();
+    }
+    return (int)err;
+}
+
+int skm_mse_grad(int count, const float *P, const float *T, float scale, float *dP, uint16_t *dP16,
+                 double *part, double *loss_sum)
+{
+    const int g = grid1d((size_t)count) > 2048 ? 2048 : grid1d((size_t)count);
+    hipLaunchKernelGGL(k_mse_grad, dim3(g), dim3(THREADS), 0, mstream(), count, P, T, scale, dP, dP16, part);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return (int)err;
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(THREADS), 0, mstream(), part, g, loss_sum);
+    return (int)hipGetLastError();
+}
+
+int skm_colsum(int rows, int cols, const float *dZ, float *db)
+{
+    hipLaunchKernelGGL(k_colsum, dim3(grid1d((size_t)cols)), dim3(THREADS), 0, mstream(), rows, cols, dZ, db);
+    return (int)hipGetLastError();
+}
+
+int skm_adam(long long n, float *p, const float *g, float *m, float *v, float lr, float b1, float b2, float eps,
+             int step)
+{
+    const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3(grid1d((size_t)n)), dim3(THREADS), 0, mstream(), n, p, g, m, v, lr, b1, b2,
+                       eps, c1, c2);
+    return (int)hipGetLastError();
+}
+
+int skm_to_bf16(long long n, const float *src, uint16_t *dst)
+{
+    hipLaunchKernelGGL(k_to_bf16, dim3(grid1d((size_t)n)), dim3(THREADS), 0, mstream(), n, src, dst);
+    return (int)hipGetLastError();
+}
+
+int skm_gather_std(int rows, int cols, const int *idx, const float *src, const float *mean, const float *stdv,
+                   float *dst32, uint16_t *dst16)
+{
+    hipLaunchKernelGGL(k_gather_std, dim3(grid1d((size_t)rows * cols)), dim3(THREADS), 0, mstream(), rows, cols,
+                       idx, src, mean, stdv, dst32, dst16);
+    return (int)hipGetLastError();
+}
+
+int skm_destandardize(int rows, int cols, const float *p, const float *mean, const float *stdv, float *y)
+{
+    hipLaunchKernelGGL(k_destandardize, dim3(grid1d((size_t)rows * cols)), dim3(THREADS), 0, mstream(), rows,
+                       cols, p, mean, stdv, y);
+    return (int)hipGetLastError();
+}
+
+}  // extern "C"

@@ -258,9 +258,19 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
         for (i = 0; i < m; ++i) E->R[i] = 1.0;
         for (j = 0; j < n; ++j) E->C[j] = 1.0;
     }
-    /* bin classes decide the permutations */
-    if (skg_plan_bins(A0.p, m, &g_skg_tuning, E->perm_r, &bA) ||
-        skg_plan_bins(AT0.p, n, &g_skg_tuning, E->perm_c, &bAT)) { skg_fail("out of host memory"); goto done; }
+    /* Bin classes decide the permutations.  Rows of A keep their order
+       inside a class; columns are ordered inside a class by their first
+       row, so the many short columns of a power-law matrix are gathered
+       almost sequentially by both A x and A^T y. */
+    {
+        int *first = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+        if (!first) { skg_fail("out of host memory"); goto done; }
+        for (j = 0; j < n; ++j) first[j] = AT0.p[j] < AT0.p[j + 1] ? AT0.j[AT0.p[j]] : m;
+        rc = skg_plan_bins(A0.p, m, &g_skg_tuning, E->perm_r, &bA) ||
+             skg_plan_bins_keyed(AT0.p, n, &g_skg_tuning, first, m, E->perm_c, &bAT);
+        free(first);
+        if (rc) { rc = skg_fail("out of host memory"); goto done; }
+    }
     for (k = 0; k < m; ++k) inv_r[E->perm_r[k]] = k;
     for (k = 0; k < n; ++k) inv_c[E->perm_c[k]] = k;
     if (hcsr_alloc(&Ap, m, n, A0.nnz) || hcsr_alloc(&ATp, n, m, A0.nnz)) { skg_fail("out of host memory"); goto done; }
@@ -691,11 +701,13 @@ int sk_gpu_pdlp_session_solve(sk_gpu_pdlp_session *S, const double *x0, const do
             use_avg_final = average_better;
             if (P.time_limit_seconds > 0.0 && skg_now() - t0 > P.time_limit_seconds) break;
             mu = kkt_worst(&best);
-            /* mixed precision: hand over to FP64 once FP32 is near its floor */
+            /* mixed precision: hand over to FP64 once FP32 is near its floor:
+               at tail_switch, or when within 100x of it no window has improved
+               the best merit by 10% for tail_stall_windows windows */
             if (prec == SKK_F32 && !tail) {
-                if (mu < 0.5 * best_mu_seen) { best_mu_seen = mu; stall = 0; }
+                if (mu < 0.9 * best_mu_seen) { best_mu_seen = mu; stall = 0; }
                 else ++stall;
-                if (mu <= P.tail_switch || stall >= P.tail_stall_windows) {
+                if (mu <= P.tail_switch || (mu <= 100.0 * P.tail_switch && stall >= P.tail_stall_windows)) {
                     if ((rc = skk_pdhg_restart(&E->P, SKK_F32, average_better, P.adaptive_step)) ||
                         (rc = skk_pdhg_promote(&E->P)) ||
                         (rc = skk_pdhg_eval(&E->P, SKK_F64, 0, P.adaptive_step, 1.0)) ||

@@ -269,6 +269,25 @@ def load_gpu():
         _declare(lib, "sk_gpu_batch_solve", C.c_int,
                  [C.c_int, M, C.POINTER(BatchOptions), c_double_p, c_double_p,
                   C.POINTER(BatchResult), C.POINTER(BatchStats)])
+        F = C.POINTER(C.c_float)
+        _declare(lib, "sk_gpu_mlp_create", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(vp)])
+        _declare(lib, "sk_gpu_mlp_destroy", None, [vp])
+        _declare(lib, "sk_gpu_mlp_param_count", C.c_longlong, [vp])
+        _declare(lib, "sk_gpu_mlp_set_scaling", C.c_int, [vp, F, F, F, F])
+        _declare(lib, "sk_gpu_mlp_get_params", C.c_int, [vp, F])
+        _declare(lib, "sk_gpu_mlp_set_params", C.c_int, [vp, F])
+        _declare(lib, "sk_gpu_mlp_train", C.c_int, [vp, F, F, C.c_int, C.c_int, C.c_int, C.c_float,
+                                                    C.c_ulonglong, F])
+        _declare(lib, "sk_gpu_mlp_predict", C.c_int, [vp, F, C.c_int, F])
+        _declare(lib, "sk_gpu_mlp_gradients", C.c_int, [vp, F, F, C.c_int, F, F])
+        _declare(lib, "sk_gpu_relative_kkt_qp", C.c_int,
+                 [C.c_int, C.c_int, C.c_int, c_int_p, c_int_p, c_double_p, c_int_p, c_int_p, c_double_p,
+                  C.c_int, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                  c_double_p, c_double_p])
+        _declare(lib, "sk_gpu_pdlp_session_create_qp", C.c_int,
+                 [C.c_int, C.c_int, C.c_int, c_int_p, c_int_p, c_double_p, c_int_p, c_int_p, c_double_p,
+                  C.c_int, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                  C.POINTER(PdlpParams), C.POINTER(vp)])
         _declare(lib, "sk_gpu_model_validate", C.c_int, [M])
         _declare(lib, "sk_gpu_solution_init", None, [S])
         _declare(lib, "sk_gpu_solution_free", None, [S])
@@ -471,3 +490,65 @@ def batch_solve(models, block_threads=0, use_graph=1, **pdlp):
         xs.append(x[c0:c0 + md.ncol].copy()); c0 += md.ncol
         ys.append(y[r0:r0 + md.nrow].copy()); r0 += md.nrow
     return rc, xs, ys, [res[k] for k in range(len(models))], st
+
+
+def fptr(a):
+    return _ptr(a, C.c_float) if a is not None else None
+
+
+class Mlp:
+    """The warm-start MLP on the GPU (sk_gpu_mlp_*)."""
+
+    def __init__(self, n_in, hidden, n_out, seed=1):
+        self.lib = load_gpu()
+        self.h = C.c_void_p()
+        self.shape = (n_in, hidden, n_out)
+        if self.lib.sk_gpu_mlp_create(n_in, hidden, n_out, seed, C.byref(self.h)) != 0:
+            raise RuntimeError(last_error())
+        self.n_params = int(self.lib.sk_gpu_mlp_param_count(self.h))
+
+    def _f32(self, a):
+        return np.ascontiguousarray(a, np.float32)
+
+    def set_scaling(self, in_mean, in_std, out_mean, out_std):
+        arrs = [self._f32(a) for a in (in_mean, in_std, out_mean, out_std)]
+        if self.lib.sk_gpu_mlp_set_scaling(self.h, *[fptr(a) for a in arrs]) != 0:
+            raise RuntimeError(last_error())
+
+    def params(self):
+        out = np.zeros(self.n_params, np.float32)
+        assert self.lib.sk_gpu_mlp_get_params(self.h, fptr(out)) == 0
+        return out
+
+    def set_params(self, p):
+        p = self._f32(p)
+        assert self.lib.sk_gpu_mlp_set_params(self.h, fptr(p)) == 0
+
+    def gradients(self, X, Y):
+        X, Y = self._f32(X), self._f32(Y)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_float()
+        rc = self.lib.sk_gpu_mlp_gradients(self.h, fptr(X), fptr(Y), X.shape[0], C.byref(loss), fptr(g))
+        if rc != 0:
+            raise RuntimeError(last_error())
+        return float(loss.value), g
+
+    def train(self, X, Y, epochs, batch, lr, seed=7):
+        X, Y = self._f32(X), self._f32(Y)
+        hist = np.zeros(max(epochs, 1), np.float32)
+        rc = self.lib.sk_gpu_mlp_train(self.h, fptr(X), fptr(Y), X.shape[0], epochs, batch, lr, seed, fptr(hist))
+        if rc != 0:
+            raise RuntimeError(last_error())
+        return hist[:epochs]
+
+    def predict(self, X):
+        X = self._f32(X)
+        Y = np.zeros((X.shape[0], self.shape[2]), np.float32)
+        if self.lib.sk_gpu_mlp_predict(self.h, fptr(X), X.shape[0], fptr(Y)) != 0:
+            raise RuntimeError(last_error())
+        return Y
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.sk_gpu_mlp_destroy(self.h)
+            self.h = C.c_void_p()
