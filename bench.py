@@ -483,11 +483,25 @@ def main():
     ap.add_argument("--test", type=int, default=4000, help="warm4: held-out instances")
     ap.add_argument("--compare", type=int, default=100, help="warm4: held-out instances solved cold and warm")
     ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="copy this many MiB device-to-device 3 times first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        nbytes = args.calibrate_copy << 20
+        hip = sg._hip_runtime()
+        a, b = C.c_void_p(), C.c_void_p()
+        hip.hipMalloc(C.byref(a), nbytes)
+        hip.hipMalloc(C.byref(b), nbytes)
+        for _ in range(3):
+            hip.hipMemcpy(b, a, nbytes, 3)
+        hip.hipDeviceSynchronize()
+        hip.hipFree(a)
+        hip.hipFree(b)
+        log(f"copy calibration: 3 x {nbytes} bytes device-to-device")
     line = WORKLOADS[args.workload](args)
     print(json.dumps(line), flush=True)
 

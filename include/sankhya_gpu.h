@@ -367,7 +367,11 @@ void sk_gpu_pdlp_session_destroy(sk_gpu_pdlp_session *session);
  * Batched PDLP: many small LPs in one persistent launch.  No reference
  * counterpart (the reference solves one model per call, sk_solve,
  * c/src/sk_solve.c:1009); each instance follows the same algorithm as
- * sk_gpu_pdlp_solve with FP32 iterates and FP64 KKT evaluation.
+ * sk_gpu_pdlp_solve: with pdlp.precision SK_GPU_PREC_MIXED, FP32 iterates
+ * until the relative KKT merit reaches pdlp.tail_switch (or stalls near
+ * it), then a second persistent launch continues the handed-off instances
+ * with FP64 iterates; with SK_GPU_PREC_FP64, FP64 iterates throughout.
+ * KKT evaluation is FP64 in both.
  * ---------------------------------------------------------------------- */
 typedef struct sk_gpu_batch_options {
     sk_gpu_pdlp_params pdlp;  /* eps, iteration limit, restart period, ... */
@@ -377,8 +381,9 @@ typedef struct sk_gpu_batch_options {
 
 typedef struct sk_gpu_batch_result {
     int    status;            /* 0 relative KKT <= eps, 1 iteration limit  */
-    int    iterations;
+    int    iterations;        /* total, FP32 phase and FP64 tail           */
     int    restarts;
+    int    fp32_iterations;   /* iterations run with FP32 iterates         */
     double relative_primal_residual;
     double relative_dual_residual;
     double relative_gap;
@@ -387,11 +392,15 @@ typedef struct sk_gpu_batch_result {
 
 typedef struct sk_gpu_batch_stats {
     double setup_seconds;     /* scaling, packing, upload                  */
-    double device_seconds;    /* the persistent launch (event-timed)       */
+    double device_seconds;    /* the persistent launches (event-timed)     */
     double seconds;
     int    block_threads, grid, lds_bytes, converged;
     long long total_iterations;
     double tune_ms[3];        /* autotuner timings for 64/128/256 threads  */
+    long long fp32_iterations;
+    int    handoffs;          /* instances finished by the FP64 tail        */
+    int    grid_fp64, lds_bytes_fp64;   /* launch shape of the FP64 phase   */
+    int    pad;
 } sk_gpu_batch_stats;
 
 void sk_gpu_batch_options_default(sk_gpu_batch_options *o);

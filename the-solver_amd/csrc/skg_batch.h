@@ -13,7 +13,10 @@ extern "C" {
 
 typedef struct skb_params {
     double eps, eta_min_factor, eta_max_factor;
-    int max_iterations, restart_period, power_iterations, pad;
+    double tail_switch;    /* FP32 -> FP64 handoff merit (mixed precision)  */
+    int max_iterations, restart_period, power_iterations;
+    int mixed;             /* 1: FP32 phase then FP64 tail; 0: FP64 only    */
+    int tail_stall_windows, pad;
 } skb_params;
 
 /* One instance: offsets into the packed arrays. */
@@ -26,9 +29,13 @@ typedef struct skb_inst {
     double cost_norm, bound_norm;
 } skb_inst;
 
+/* Per-instance result, and the state the FP64 tail resumes from. */
 typedef struct skb_out {
-    int iterations, restarts, converged, pad;
+    int iterations, restarts, converged;
+    int handoff;           /* 1: the FP32 phase stopped for the FP64 tail  */
+    int fp32_iterations, pad;
     double rel_primal, rel_dual, rel_gap, matrix_norm, final_step;
+    double omega, updates;
 } skb_out;
 
 typedef struct skb_batch {
@@ -46,8 +53,13 @@ typedef struct skb_batch {
     skb_params params;
 } skb_batch;
 
-size_t skb_lds_bytes(int max_m, int max_n);
-int    skb_launch(const skb_batch *B, int block_threads, int grid);
+/* LDS of one workgroup: 7 column and 9 row vectors of 4- or 8-byte words. */
+size_t skb_lds_bytes(int max_m, int max_n, int word_bytes);
+/* One phase over the whole batch: fp64 = 0 runs the FP32 phase from a cold
+ * start; fp64 = 1 runs FP64 iterates -- from a cold start when
+ * params.mixed is 0, else resuming every instance the FP32 phase handed
+ * off (the others are skipped). */
+int    skb_launch(const skb_batch *B, int block_threads, int grid, int fp64);
 int    skb_max_lds(void);
 int    skb_cu_count(void);
 

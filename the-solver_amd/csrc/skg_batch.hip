@@ -54,7 +54,7 @@ __device__ __forceinline__ void wg_sum(double (&v)[S], double *sh /* S * BT/64 *
     __syncthreads();
 }
 
-__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+template <typename T> __device__ __forceinline__ T clampv(T v, T lo, T hi) { return fmin(fmax(v, lo), hi); }
 
 struct Inst {
     int m, n;
@@ -64,7 +64,29 @@ struct Inst {
     const float *c, *l, *u, *rl, *ru;
     const double *cd, *ld, *ud, *rld, *rud;
     double cost_norm, bound_norm;
+    // operator values and data in the iterate precision
+    template <typename T> __device__ const T *a() const;
+    template <typename T> __device__ const T *t() const;
+    template <typename T> __device__ const T *cv() const;
+    template <typename T> __device__ const T *lv() const;
+    template <typename T> __device__ const T *uv() const;
+    template <typename T> __device__ const T *rlv() const;
+    template <typename T> __device__ const T *ruv() const;
 };
+template <> __device__ const float *Inst::a<float>() const { return av; }
+template <> __device__ const double *Inst::a<double>() const { return avd; }
+template <> __device__ const float *Inst::t<float>() const { return tv; }
+template <> __device__ const double *Inst::t<double>() const { return tvd; }
+template <> __device__ const float *Inst::cv<float>() const { return c; }
+template <> __device__ const double *Inst::cv<double>() const { return cd; }
+template <> __device__ const float *Inst::lv<float>() const { return l; }
+template <> __device__ const double *Inst::lv<double>() const { return ld; }
+template <> __device__ const float *Inst::uv<float>() const { return u; }
+template <> __device__ const double *Inst::uv<double>() const { return ud; }
+template <> __device__ const float *Inst::rlv<float>() const { return rl; }
+template <> __device__ const double *Inst::rlv<double>() const { return rld; }
+template <> __device__ const float *Inst::ruv<float>() const { return ru; }
+template <> __device__ const double *Inst::ruv<double>() const { return rud; }
 
 __device__ Inst load_inst(const skb_batch &B, int k)
 {
@@ -97,9 +119,9 @@ __device__ Kkt make_kkt(const double *s, double cost_norm, double bound_norm)
 
 // Eight residual sums of PdlpKernels.cu:101-209 in FP64 for the point
 // (x * xs, y * ys) -- xs = ys = 1 for the iterate, 1/terms for the average.
-template <int BT>
-__device__ void residual_sums(const Inst &I, const float *x, const float *y, const float *xr,
-                              const float *yr, double xs, double ys, double *sh, double (&out)[8])
+template <int BT, typename T>
+__device__ void residual_sums(const Inst &I, const T *x, const T *y, const T *xr, const T *yr,
+                              double xs, double ys, double *sh, double (&out)[8])
 {
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int j = threadIdx.x; j < I.n; j += BT) {
@@ -137,94 +159,126 @@ __device__ void residual_sums(const Inst &I, const float *x, const float *y, con
     for (int s = 0; s < 8; ++s) out[s] = acc[s];
 }
 
-// y = A x (FP32, LDS vectors), one thread per row.
-template <int BT>
-__device__ __forceinline__ void spmv_rows(int rows, const int *rp, const int *ci, const float *v,
-                                          const float *x, float *y)
+// y = A x (LDS vectors), one thread per row.
+template <int BT, typename T>
+__device__ __forceinline__ void spmv_rows(int rows, const int *rp, const int *ci, const T *v, const T *x,
+                                          T *y)
 {
     for (int i = threadIdx.x; i < rows; i += BT) {
-        float s = 0.0f;
+        T s = 0;
         for (int p = rp[i]; p < rp[i + 1]; ++p) s += v[p] * x[ci[p]];
         y[i] = s;
     }
 }
 
-template <int BT>
-__device__ void solve_instance(const skb_batch &B, int k, float *lds, double *sh)
+// One instance, start to finish (T = float: the FP32 phase, which may hand
+// off to the FP64 tail; T = double: FP64 from a cold start, or the tail
+// resuming from the FP32 phase's point and state).
+template <int BT, typename T>
+__device__ void solve_instance(const skb_batch &B, int k, T *lds, double *sh)
 {
+    const skb_params &P = B.params;
+    constexpr bool kFp32 = sizeof(T) == 4;
+    const bool resume = !kFp32 && P.mixed;
+    skb_out &o = B.out[k];
+    if (resume && !o.handoff) return;    // finished in the FP32 phase (uniform)
     const Inst I = load_inst(B, k);
     const int m = I.m, n = I.n;
-    const skb_params &P = B.params;
+    const T *av = I.a<T>(), *tv = I.t<T>();
+    const T *cg = I.cv<T>(), *lg = I.lv<T>(), *ug = I.uv<T>(), *rlg = I.rlv<T>(), *rug = I.ruv<T>();
     // LDS layout: 7 column vectors, 9 row vectors
-    float *X0 = lds, *X1 = X0 + n, *xsum = X1 + n, *xr = xsum + n, *cl = xr + n, *ll = cl + n, *ul = ll + n;
-    float *Y0 = ul + n, *Y1 = Y0 + m, *AX0 = Y1 + m, *AX1 = AX0 + m, *AX2 = AX1 + m, *ysum = AX2 + m,
-          *yr = ysum + m, *rll = yr + m, *rul = rll + m;
-    for (int j = threadIdx.x; j < n; j += BT) { cl[j] = I.c[j]; ll[j] = I.l[j]; ul[j] = I.u[j]; }
-    for (int i = threadIdx.x; i < m; i += BT) { rll[i] = I.rl[i]; rul[i] = I.ru[i]; }
+    T *X0 = lds, *X1 = X0 + n, *xsum = X1 + n, *xr = xsum + n, *cl = xr + n, *ll = cl + n, *ul = ll + n;
+    T *Y0 = ul + n, *Y1 = Y0 + m, *AX0 = Y1 + m, *AX1 = AX0 + m, *AX2 = AX1 + m, *ysum = AX2 + m,
+      *yr = ysum + m, *rll = yr + m, *rul = rll + m;
+    for (int j = threadIdx.x; j < n; j += BT) { cl[j] = cg[j]; ll[j] = lg[j]; ul[j] = ug[j]; }
+    for (int i = threadIdx.x; i < m; i += BT) { rll[i] = rlg[i]; rul[i] = rug[i]; }
 
-    // ||A||_2: power iteration on A'A from v = 1/sqrt(n) (GpuPdlp.cpp:179-211)
-    double norm_sq = 0.0;
-    {
-        const float v0 = (float)(1.0 / sqrt((double)(n > 0 ? n : 1)));
+    double anorm, eta, omega = 1.0, updates = 0.0;
+    int iteration = 0, restarts = 0, fp32_iterations = 0;
+    const skb_inst &d = B.inst[k];
+    if (resume) {
+        anorm = o.matrix_norm; eta = o.final_step; omega = o.omega; updates = o.updates;
+        iteration = o.iterations; restarts = o.restarts; fp32_iterations = o.fp32_iterations;
+    } else {
+        // ||A||_2: power iteration on A'A from v = 1/sqrt(n) (GpuPdlp.cpp:179-211)
+        double norm_sq = 0.0;
+        const T v0 = (T)(1.0 / sqrt((double)(n > 0 ? n : 1)));
         for (int j = threadIdx.x; j < n; j += BT) X0[j] = v0;
         __syncthreads();
         for (int it = 0; it < P.power_iterations; ++it) {
-            spmv_rows<BT>(m, I.arp, I.aci, I.av, X0, Y0);
+            spmv_rows<BT, T>(m, I.arp, I.aci, av, X0, Y0);
             __syncthreads();
             double acc[1] = {0.0};
             for (int j = threadIdx.x; j < n; j += BT) {
-                float s = 0.0f;
-                for (int p = I.trp[j]; p < I.trp[j + 1]; ++p) s += I.tv[p] * Y0[I.tci[p]];
+                T s = 0;
+                for (int p = I.trp[j]; p < I.trp[j + 1]; ++p) s += tv[p] * Y0[I.tci[p]];
                 X1[j] = s;
                 acc[0] += (double)s * (double)s;
             }
             wg_sum<BT, 1>(acc, sh);
             norm_sq = acc[0];
             const double nrm = sqrt(norm_sq);
-            const float inv = nrm > 0.0 ? (float)(1.0 / nrm) : 1.0f;
+            const T inv = nrm > 0.0 ? (T)(1.0 / nrm) : (T)1;
             for (int j = threadIdx.x; j < n; j += BT) X0[j] = X1[j] * inv;
             __syncthreads();
         }
+        anorm = sqrt(sqrt(fmax(0.0, norm_sq)));
+        if (!(anorm > 0.0) || !isfinite(anorm)) anorm = 1.0;
+        eta = 0.9 / anorm;
+        if (I.cost_norm > 0.0 && I.bound_norm > 0.0) omega = fmin(1e6, fmax(1e-6, I.cost_norm / I.bound_norm));
     }
-    double anorm = sqrt(sqrt(fmax(0.0, norm_sq)));
-    if (!(anorm > 0.0) || !isfinite(anorm)) anorm = 1.0;
     const double eta_min = P.eta_min_factor / anorm, eta_max = P.eta_max_factor / anorm;
-    double eta = 0.9 / anorm, omega = 1.0, updates = 0.0, accepted = 0.0, accepted_at_restart = 0.0;
-    if (I.cost_norm > 0.0 && I.bound_norm > 0.0) omega = fmin(1e6, fmax(1e-6, I.cost_norm / I.bound_norm));
+    double accepted = 0.0, accepted_at_restart = 0.0;
 
-    // cold start (GpuPdlp.cpp:278-291)
-    for (int j = threadIdx.x; j < n; j += BT) { X0[j] = 0.0f; X1[j] = 0.0f; xsum[j] = 0.0f; xr[j] = 0.0f; }
-    for (int i = threadIdx.x; i < m; i += BT) {
-        Y0[i] = 0.0f; Y1[i] = 0.0f; AX0[i] = 0.0f; AX1[i] = 0.0f; AX2[i] = 0.0f; ysum[i] = 0.0f; yr[i] = 0.0f;
+    if (resume) {
+        // the FP64 tail starts a fresh restart window at the handed-off point
+        for (int j = threadIdx.x; j < n; j += BT) {
+            const T v = (T)B.x_out[d.col + j];
+            X0[j] = v; X1[j] = v; xsum[j] = 0; xr[j] = v;
+        }
+        for (int i = threadIdx.x; i < m; i += BT) {
+            const T v = (T)B.y_out[d.row + i];
+            Y0[i] = v; Y1[i] = v; AX2[i] = 0; ysum[i] = 0; yr[i] = v;
+        }
+        __syncthreads();
+        spmv_rows<BT, T>(m, I.arp, I.aci, av, X0, AX0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < m; i += BT) AX1[i] = AX0[i];
+    } else {
+        // cold start (GpuPdlp.cpp:278-291)
+        for (int j = threadIdx.x; j < n; j += BT) { X0[j] = 0; X1[j] = 0; xsum[j] = 0; xr[j] = 0; }
+        for (int i = threadIdx.x; i < m; i += BT) {
+            Y0[i] = 0; Y1[i] = 0; AX0[i] = 0; AX1[i] = 0; AX2[i] = 0; ysum[i] = 0; yr[i] = 0;
+        }
     }
     __syncthreads();
-    float *x = X0, *xc = X1, *y = Y0, *yc = Y1, *ax = AX0, *axp = AX1, *axc = AX2;
+    T *x = X0, *xc = X1, *y = Y0, *yc = Y1, *ax = AX0, *axp = AX1, *axc = AX2;
 
-    double mu_at_restart = INFINITY, mu_prev = INFINITY;
-    int iteration = 0, since_restart = 0, restarts = 0, converged = 0, final_avg = 0;
+    double mu_at_restart = INFINITY, mu_prev = INFINITY, best_mu_seen = INFINITY;
+    int since_restart = 0, converged = 0, final_avg = 0, handoff = 0, stall = 0;
     Kkt best{INFINITY, INFINITY, INFINITY, 0, 0, INFINITY};
     while (iteration < P.max_iterations) {
         const int period = min(P.restart_period, P.max_iterations - iteration);
         for (int t = 0; t < period; ++t) {
-            const float sigma = (float)(eta * omega), tau = (float)(eta / omega);
+            const T sigma = (T)(eta * omega), tau = (T)(eta / omega);
             for (int i = threadIdx.x; i < m; i += BT) {
-                const float v = y[i] + sigma * (2.0f * ax[i] - axp[i]);
-                yc[i] = v - sigma * clampf(v / sigma, rll[i], rul[i]);
+                const T v = y[i] + sigma * ((T)2 * ax[i] - axp[i]);
+                yc[i] = v - sigma * clampv<T>(v / sigma, rll[i], rul[i]);
             }
             __syncthreads();
             double acc[3] = {0.0, 0.0, 0.0};
             for (int j = threadIdx.x; j < n; j += BT) {
-                float s = 0.0f;
-                for (int p = I.trp[j]; p < I.trp[j + 1]; ++p) s += I.tv[p] * yc[I.tci[p]];
-                const float xn = clampf(x[j] - tau * (cl[j] + s), ll[j], ul[j]);
+                T s = 0;
+                for (int p = I.trp[j]; p < I.trp[j + 1]; ++p) s += tv[p] * yc[I.tci[p]];
+                const T xn = clampv<T>(x[j] - tau * (cl[j] + s), ll[j], ul[j]);
                 xc[j] = xn;
-                const double d = (double)xn - (double)x[j];
-                acc[0] += d * d;
+                const double dd = (double)xn - (double)x[j];
+                acc[0] += dd * dd;
             }
             __syncthreads();
             for (int i = threadIdx.x; i < m; i += BT) {
-                float s = 0.0f;
-                for (int p = I.arp[i]; p < I.arp[i + 1]; ++p) s += I.av[p] * xc[I.aci[p]];
+                T s = 0;
+                for (int p = I.arp[i]; p < I.arp[i + 1]; ++p) s += av[p] * xc[I.aci[p]];
                 axc[i] = s;
                 const double dy = (double)yc[i] - (double)y[i];
                 acc[1] += dy * dy;
@@ -242,7 +296,7 @@ __device__ void solve_instance(const skb_batch &B, int k, float *lds, double *sh
             updates = kk;
             if (accept) {
                 accepted += 1.0;
-                float *t0 = x; x = xc; xc = t0;
+                T *t0 = x; x = xc; xc = t0;
                 t0 = y; y = yc; yc = t0;
                 t0 = axp; axp = ax; ax = axc; axc = t0;
                 for (int j = threadIdx.x; j < n; j += BT) xsum[j] += x[j];
@@ -252,12 +306,13 @@ __device__ void solve_instance(const skb_batch &B, int k, float *lds, double *sh
         }
         iteration += period;
         since_restart += period;
+        if (kFp32) fp32_iterations += period;
         // FP64 evaluation of the iterate and of the running average
         double sc[8], sa[8];
-        residual_sums<BT>(I, x, y, xr, yr, 1.0, 1.0, sh, sc);
+        residual_sums<BT, T>(I, x, y, xr, yr, 1.0, 1.0, sh, sc);
         double terms = accepted - accepted_at_restart;
         if (terms < 1.0) terms = 1.0;
-        residual_sums<BT>(I, xsum, ysum, xr, yr, 1.0 / terms, 1.0 / terms, sh, sa);
+        residual_sums<BT, T>(I, xsum, ysum, xr, yr, 1.0 / terms, 1.0 / terms, sh, sa);
         const Kkt cur = make_kkt(sc, I.cost_norm, I.bound_norm);
         const Kkt avg = make_kkt(sa, I.cost_norm, I.bound_norm);
         const bool avg_better = avg.worst < cur.worst;
@@ -265,22 +320,34 @@ __device__ void solve_instance(const skb_batch &B, int k, float *lds, double *sh
         final_avg = avg_better;
         if (best.primal <= P.eps && best.dual <= P.eps && best.gap <= P.eps) { converged = 1; break; }
         const double mu = best.worst;
+        if (kFp32 && P.mixed) {
+            // hand off to FP64 iterates once the merit reaches tail_switch,
+            // or when within 100x of it no window has improved the best
+            // merit by 10% for tail_stall_windows windows
+            if (mu < 0.9 * best_mu_seen) { best_mu_seen = mu; stall = 0; }
+            else ++stall;
+            if (iteration < P.max_iterations &&
+                (mu <= P.tail_switch || (mu <= 100.0 * P.tail_switch && stall >= P.tail_stall_windows))) {
+                handoff = 1;
+                break;
+            }
+        }
         const bool sufficient = mu <= 0.2 * mu_at_restart;
         const bool necessary = mu <= 0.8 * mu_at_restart && mu > mu_prev;
         const bool artificial = (double)since_restart >= 0.36 * (double)iteration;
         mu_prev = mu;
         if (!(sufficient || necessary || artificial)) continue;
         // restart (GpuPdlp.cpp:453-498)
-        const float inv = (float)(1.0 / terms);
+        const T inv = (T)(1.0 / terms);
         if (avg_better) {
             for (int j = threadIdx.x; j < n; j += BT) x[j] = xsum[j] * inv;
             for (int i = threadIdx.x; i < m; i += BT) y[i] = ysum[i] * inv;
             __syncthreads();
         }
-        spmv_rows<BT>(m, I.arp, I.aci, I.av, x, ax);
+        spmv_rows<BT, T>(m, I.arp, I.aci, av, x, ax);
         __syncthreads();
-        for (int i = threadIdx.x; i < m; i += BT) { axp[i] = ax[i]; ysum[i] = 0.0f; yr[i] = y[i]; }
-        for (int j = threadIdx.x; j < n; j += BT) { xsum[j] = 0.0f; xr[j] = x[j]; }
+        for (int i = threadIdx.x; i < m; i += BT) { axp[i] = ax[i]; ysum[i] = 0; yr[i] = y[i]; }
+        for (int j = threadIdx.x; j < n; j += BT) { xsum[j] = 0; xr[j] = x[j]; }
         __syncthreads();
         accepted_at_restart = accepted;
         since_restart = 0;
@@ -293,28 +360,32 @@ __device__ void solve_instance(const skb_batch &B, int k, float *lds, double *sh
             if (isfinite(upd)) omega = fmin(1e6, fmax(1e-6, upd));
         }
     }
-    // scaled-space result in FP64 (the host unscales)
-    const skb_inst &d = B.inst[k];
+    // scaled-space point in FP64 (the host unscales; the tail resumes here)
     const double inv = 1.0 / fmax(1.0, accepted - accepted_at_restart);
     for (int j = threadIdx.x; j < n; j += BT) B.x_out[d.col + j] = final_avg ? (double)xsum[j] * inv : (double)x[j];
     for (int i = threadIdx.x; i < m; i += BT) B.y_out[d.row + i] = final_avg ? (double)ysum[i] * inv : (double)y[i];
+    __syncthreads();
     if (threadIdx.x == 0) {
-        skb_out &o = B.out[k];
         o.iterations = iteration;
         o.restarts = restarts;
         o.converged = converged;
+        o.handoff = handoff;
+        o.fp32_iterations = fp32_iterations;
         o.rel_primal = best.primal;
         o.rel_dual = best.dual;
         o.rel_gap = best.gap;
         o.matrix_norm = anorm;
         o.final_step = eta;
+        o.omega = omega;
+        o.updates = updates;
     }
     __syncthreads();
 }
 
-template <int BT> __global__ __launch_bounds__(BT) void k_batch_pdlp(skb_batch B)
+template <int BT, typename T> __global__ __launch_bounds__(BT) void k_batch_pdlp(skb_batch B)
 {
-    extern __shared__ float lds[];
+    extern __shared__ double lds_raw[];
+    T *lds = reinterpret_cast<T *>(lds_raw);
     __shared__ double sh[8 * (BT / 64)];
     __shared__ int next;
     for (;;) {
@@ -324,7 +395,7 @@ template <int BT> __global__ __launch_bounds__(BT) void k_batch_pdlp(skb_batch B
         const int q = next;
         __syncthreads();
         if (q >= B.count) break;       // every wave of the workgroup leaves together
-        solve_instance<BT>(B, B.order[q], lds, sh);
+        solve_instance<BT, T>(B, B.order[q], lds, sh);
     }
 }
 
@@ -333,28 +404,35 @@ __global__ void k_queue_reset(unsigned *queue)
     if (threadIdx.x == 0 && blockIdx.x == 0) *queue = 0u;
 }
 
+template <typename T> void launch_phase(const skb_batch *B, int block_threads, int grid, size_t lds)
+{
+    switch (block_threads) {
+    case 64:  hipLaunchKernelGGL((k_batch_pdlp<64, T>), dim3(grid), dim3(64), lds, bstream(), *B); break;
+    case 128: hipLaunchKernelGGL((k_batch_pdlp<128, T>), dim3(grid), dim3(128), lds, bstream(), *B); break;
+    case 256: hipLaunchKernelGGL((k_batch_pdlp<256, T>), dim3(grid), dim3(256), lds, bstream(), *B); break;
+    case 512: hipLaunchKernelGGL((k_batch_pdlp<512, T>), dim3(grid), dim3(512), lds, bstream(), *B); break;
+    default: break;
+    }
+}
+
 }  // namespace
 
 extern "C" {
 
-size_t skb_lds_bytes(int max_m, int max_n)
+size_t skb_lds_bytes(int max_m, int max_n, int word_bytes)
 {
-    return sizeof(float) * (7 * (size_t)max_n + 9 * (size_t)max_m);
+    return (size_t)word_bytes * (7 * (size_t)max_n + 9 * (size_t)max_m);
 }
 
-int skb_launch(const skb_batch *B, int block_threads, int grid)
+int skb_launch(const skb_batch *B, int block_threads, int grid, int fp64)
 {
-    const size_t lds = skb_lds_bytes(B->max_m, B->max_n);
+    if (block_threads != 64 && block_threads != 128 && block_threads != 256 && block_threads != 512) return -1;
+    const size_t lds = skb_lds_bytes(B->max_m, B->max_n, fp64 ? 8 : 4);
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, bstream(), B->queue);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return (int)e;
-    switch (block_threads) {
-    case 64:  hipLaunchKernelGGL(k_batch_pdlp<64>, dim3(grid), dim3(64), lds, bstream(), *B); break;
-    case 128: hipLaunchKernelGGL(k_batch_pdlp<128>, dim3(grid), dim3(128), lds, bstream(), *B); break;
-    case 256: hipLaunchKernelGGL(k_batch_pdlp<256>, dim3(grid), dim3(256), lds, bstream(), *B); break;
-    case 512: hipLaunchKernelGGL(k_batch_pdlp<512>, dim3(grid), dim3(512), lds, bstream(), *B); break;
-    default: return -1;
-    }
+    if (fp64) launch_phase<double>(B, block_threads, grid, lds);
+    else launch_phase<float>(B, block_threads, grid, lds);
     return (int)hipGetLastError();
 }
 

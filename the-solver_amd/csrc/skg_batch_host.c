@@ -14,6 +14,7 @@ void sk_gpu_batch_options_default(sk_gpu_batch_options *o)
     memset(o, 0, sizeof(*o));
     sk_gpu_pdlp_params_default(&o->pdlp);
     o->pdlp.max_iterations = 100000;
+    o->pdlp.precision = SK_GPU_PREC_MIXED;
     o->block_threads = 0;       /* autotune */
     o->use_graph = 1;
 }
@@ -52,7 +53,6 @@ static void hbatch_free(hbatch *h)
     memset(h, 0, sizeof(*h));
 }
 
-static int cmp_desc_nnz;   /* unused placeholder for portability of qsort_r */
 static const skb_inst *g_sort_inst;
 static int by_size_desc(const void *a, const void *b)
 {
@@ -133,7 +133,6 @@ static int pack(hbatch *h, int count, const sk_model *models, const sk_gpu_pdlp_
     }
     g_sort_inst = h->inst;
     qsort(h->order, (size_t)count, sizeof(int), by_size_desc);
-    (void)cmp_desc_nnz;
     return 0;
 }
 
@@ -148,7 +147,7 @@ static int dput(dbatch *d, const void **dst, const void *src, size_t bytes)
     int rc = skk_malloc(&q, bytes ? bytes : 8);
     if (rc) return rc;
     d->p[d->np++] = q;
-    if (bytes && (rc = skk_h2d(q, src, bytes))) return rc;
+    if (src && bytes && (rc = skk_h2d(q, src, bytes))) return rc;
     *dst = q;
     return 0;
 }
@@ -171,28 +170,41 @@ static void dbatch_free(dbatch *d)
     d->np = 0;
 }
 
-/* Time one launch (events on the library stream); milliseconds or < 0. */
-static double timed_launch(const skb_batch *B, int bt, int grid, void **graph, int use_graph)
+/* Launch shape of both phases. */
+typedef struct phases {
+    int bt, grid32, grid64;
+    int run32, run64;          /* which phases run */
+} phases;
+
+static int launch_phases(const skb_batch *B, const phases *ph)
+{
+    int rc = 0;
+    if (ph->run32 && (rc = skb_launch(B, ph->bt, ph->grid32, 0))) return rc;
+    if (ph->run64 && (rc = skb_launch(B, ph->bt, ph->grid64, 1))) return rc;
+    return 0;
+}
+
+/* Time the phases (events on the library stream); milliseconds or < 0. */
+static double timed_launch(const skb_batch *B, const phases *ph, void **graph, int use_graph)
 {
     int rc;
     if (skk_timeline_reserve(2)) return -1.0;
-    if (skk_timeline_mark(0)) return -1.0;
     if (use_graph) {
         if (!*graph) {
             if (skk_graph_begin()) return -1.0;
-            rc = skb_launch(B, bt, grid);
+            rc = launch_phases(B, ph);
             {
                 void *exec = NULL;
                 const int rc2 = skk_graph_end(&exec);
                 if (rc || rc2) { skk_graph_destroy(exec); return -1.0; }
                 *graph = exec;
             }
-            /* the begin mark was recorded before the capture: re-mark */
-            if (skk_timeline_mark(0)) return -1.0;
         }
+        if (skk_timeline_mark(0)) return -1.0;
         if (skk_graph_launch(*graph)) return -1.0;
-    } else if (skb_launch(B, bt, grid)) {
-        return -1.0;
+    } else {
+        if (skk_timeline_mark(0)) return -1.0;
+        if (launch_phases(B, ph)) return -1.0;
     }
     if (skk_timeline_mark(1)) return -1.0;
     return skk_timeline_ms(0, 1);
@@ -220,8 +232,9 @@ int sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_opt
     skb_out *hout = NULL;
     double *xs = NULL, *ys = NULL;
     const double t0 = skg_now();
-    int k, j, rc = -1, bt, grid, max_m = 0, max_n = 0;
-    size_t lds;
+    int k, j, rc = -1, bt, max_m = 0, max_n = 0, mixed;
+    size_t lds32, lds64;
+    phases ph;
     void *graph = NULL;
     skg_clear_error();
     if (!stats) return skg_fail("null stats");
@@ -241,9 +254,13 @@ int sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_opt
     }
     if (count == 0) return 0;
     if (skg_have_device()) return -1;
-    lds = skb_lds_bytes(max_m, max_n);
-    if ((long long)lds > (long long)skb_max_lds())
-        return skg_fail("largest instance needs %zu bytes of LDS (limit %d)", lds, skb_max_lds());
+    mixed = o.pdlp.precision == SK_GPU_PREC_MIXED;
+    if (!mixed && o.pdlp.precision != SK_GPU_PREC_FP64) return skg_fail("unknown precision %d", o.pdlp.precision);
+    lds32 = skb_lds_bytes(max_m, max_n, 4);
+    lds64 = skb_lds_bytes(max_m, max_n, 8);
+    if ((long long)lds32 > (long long)skb_max_lds() || (!mixed && (long long)lds64 > (long long)skb_max_lds()))
+        return skg_fail("largest instance needs %zu bytes of LDS (limit %d)", mixed ? lds32 : lds64,
+                        skb_max_lds());
     memset(&d, 0, sizeof(d));
     memset(&B, 0, sizeof(B));
     if (pack(&h, count, models, &o.pdlp)) { hbatch_free(&h); return -1; }
@@ -273,6 +290,18 @@ int sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_opt
     B.params.max_iterations = o.pdlp.max_iterations;
     B.params.restart_period = o.pdlp.restart_period;
     B.params.power_iterations = o.pdlp.power_iterations;
+    B.params.mixed = mixed;
+    B.params.tail_switch = o.pdlp.tail_switch;
+    B.params.tail_stall_windows = o.pdlp.tail_stall_windows;
+    memset(&ph, 0, sizeof(ph));
+    ph.run32 = mixed;
+    ph.run64 = 1;
+    if (mixed && (long long)lds64 > (long long)skb_max_lds()) {
+        /* FP64 iterates of the largest instance do not fit in LDS: FP32 only */
+        ph.run64 = 0;
+        B.params.tail_switch = -1.0;
+        B.params.tail_stall_windows = 0x7fffffff;
+    }
     stats->setup_seconds = skg_now() - t0;
     /* autotune the workgroup size on a two-window run of the whole batch */
     bt = o.block_threads;
@@ -283,16 +312,23 @@ int sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_opt
         T.params.max_iterations = 2 * o.pdlp.restart_period;
         for (k = 0; k < 3; ++k) {
             void *none = NULL;
-            const double ms = timed_launch(&T, cand[k], grid_for(count, cand[k], lds), &none, 0);
+            phases tp = ph;
+            double ms;
+            tp.bt = cand[k];
+            tp.grid32 = grid_for(count, cand[k], lds32);
+            tp.grid64 = grid_for(count, cand[k], lds64);
+            ms = timed_launch(&T, &tp, &none, 0);
             if (ms < 0.0) { rc = skk_last_launch_error(); goto fail; }
             stats->tune_ms[k] = ms;
             if (ms < best) { best = ms; bt = cand[k]; }
         }
     }
     if (bt != 64 && bt != 128 && bt != 256 && bt != 512) { skg_fail("block_threads must be 64..512"); rc = -1; goto fail; }
-    grid = grid_for(count, bt, lds);
+    ph.bt = bt;
+    ph.grid32 = grid_for(count, bt, lds32);
+    ph.grid64 = grid_for(count, bt, lds64);
     {
-        const double ms = timed_launch(&B, bt, grid, &graph, o.use_graph);
+        const double ms = timed_launch(&B, &ph, &graph, o.use_graph);
         if (ms < 0.0) { rc = skk_last_launch_error(); skg_hip_fail("batch launch", rc ? rc : -1); goto fail; }
         stats->device_seconds = ms * 1e-3;
     }
@@ -316,16 +352,21 @@ int sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_opt
         results[k].status = hout[k].converged ? 0 : 1;
         results[k].iterations = hout[k].iterations;
         results[k].restarts = hout[k].restarts;
+        results[k].fp32_iterations = hout[k].fp32_iterations;
         results[k].relative_primal_residual = hout[k].rel_primal;
         results[k].relative_dual_residual = hout[k].rel_dual;
         results[k].relative_gap = hout[k].rel_gap;
         results[k].objective = obj + md->objshift;
         stats->converged += hout[k].converged;
         stats->total_iterations += hout[k].iterations;
+        stats->fp32_iterations += hout[k].fp32_iterations;
+        stats->handoffs += hout[k].fp32_iterations > 0 && hout[k].fp32_iterations < hout[k].iterations;
     }
     stats->block_threads = bt;
-    stats->grid = grid;
-    stats->lds_bytes = (int)lds;
+    stats->grid = mixed ? ph.grid32 : ph.grid64;
+    stats->lds_bytes = (int)(mixed ? lds32 : lds64);
+    stats->grid_fp64 = ph.grid64;
+    stats->lds_bytes_fp64 = (int)lds64;
     stats->seconds = skg_now() - t0;
     rc = 0;
 fail:

@@ -34,7 +34,11 @@ int skg_hip_fail(const char *what, int code)
     return code > 0 ? code : -1;
 }
 
-void skg_clear_error(void) { g_err[0] = '\0'; }
+void skg_clear_error(void)
+{
+    g_err[0] = '\0';
+    skk_reset_error();
+}
 
 double skg_now(void)
 {

@@ -27,6 +27,7 @@ int  skk_host_alloc(void **ptr, size_t bytes);                 /* pinned host me
 void skk_host_free(void *ptr);
 int  skk_sync(void);
 int  skk_last_launch_error(void);
+void skk_reset_error(void);
 const char *skk_error_text(int code);
 /* Capture everything queued between begin and end into a graph. */
 int  skk_graph_begin(void);

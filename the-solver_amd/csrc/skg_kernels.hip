@@ -133,8 +133,10 @@ __device__ __forceinline__ void store_partials(double (&v)[S], double *part, int
 {
     double blk[S];
     block_reduce<S, kMax>(v, blk);
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
+#pragma unroll
         for (int s = 0; s < S; ++s) part[(size_t)s * stride + blockIdx.x] = blk[s];
+    }
 }
 
 // Combine `count` partials of one slot: strided per-thread sums in a fixed
@@ -194,12 +196,15 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
                                                  skk_bins bins, Epi epi)
 {
     Epi e = epi;
-    e.begin();
+    e.begin(epi);
     const XT *__restrict__ x = (sel && (sel->cur ^ 1)) ? x1 : x0;
     const int blk = blockIdx.x;
-    int b = 0;
-    while (b + 1 < bins.nbins && blk >= bins.b[b + 1].block_begin) ++b;
-    const skk_bin d = bins.b[b];
+    // bin of this workgroup, chosen with constant indices only (a dynamic
+    // index into the kernel-argument table would copy it to scratch)
+    skk_bin d = bins.b[0];
+#pragma unroll
+    for (int q = 1; q < SKK_MAX_BINS; ++q)
+        if (q < bins.nbins && blk >= bins.b[q].block_begin) d = bins.b[q];
     double part[Epi::kSlots > 0 ? Epi::kSlots : 1] = {};
     if (d.log2L <= 6) {
         const int log2L = d.log2L, L = 1 << log2L;
@@ -234,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
 template <typename YT, typename AT> struct EpiStore {
     static constexpr int kSlots = 0;
     YT *y;
-    __device__ void begin() {}
+    template <class K> __device__ void begin(const K &) {}
     __device__ void row(int r, AT s, double *) const { y[r] = static_cast<YT>(s); }
     __device__ void end(double *) const {}
 };
@@ -264,6 +269,12 @@ template <typename T> struct Iter {
         xbar = static_cast<T *>(it.xbar);
         QX[0] = static_cast<T *>(it.QX[0]); QX[1] = static_cast<T *>(it.QX[1]);
     }
+    // Buffer selection by a runtime index through selects, never by
+    // indexing the pointer arrays (that would put them in scratch memory).
+    __device__ __forceinline__ T *x(int k) const { return k ? X[1] : X[0]; }
+    __device__ __forceinline__ T *y(int k) const { return k ? Y[1] : Y[0]; }
+    __device__ __forceinline__ T *qx(int k) const { return k ? QX[1] : QX[0]; }
+    __device__ __forceinline__ T *ax(int k) const { return k == 0 ? AX[0] : (k == 1 ? AX[1] : AX[2]); }
 };
 
 template <typename T> struct Data {
@@ -346,10 +357,10 @@ __global__ __launch_bounds__(kBlock) void k_dual_adaptive(Iter<T> I, Data<T> D,
 {
     const int cur = st->cur, pending = st->pending;
     const T sigma = static_cast<T>(st->eta * st->omega);
-    const T *__restrict__ y = I.Y[cur];
-    const T *__restrict__ ax = I.AX[st->a0];
-    const T *__restrict__ axp = I.AX[st->a1];
-    T *__restrict__ yc = I.Y[cur ^ 1];
+    const T *__restrict__ y = I.y(cur);
+    const T *__restrict__ ax = I.ax(st->a0);
+    const T *__restrict__ axp = I.ax(st->a1);
+    T *__restrict__ yc = I.y(cur ^ 1);
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
         const T yi = y[i];
@@ -373,11 +384,11 @@ template <typename T, typename AT> struct EpiPrimalAdaptive {
     T *xc;
     T tau;
     int pending;
-    __device__ void begin()
+    template <class K> __device__ void begin(const K &)
     {
         const int cur = st->cur;
-        x = I.X[cur]; xc = I.X[cur ^ 1];
-        qx = I.QX[0] ? I.QX[cur] : nullptr;
+        x = I.x(cur); xc = I.x(cur ^ 1);
+        qx = I.QX[0] ? I.qx(cur) : nullptr;
         tau = static_cast<T>(st->eta / st->omega);
         pending = st->pending;
     }
@@ -407,10 +418,13 @@ template <typename T, typename AT> struct EpiProbe {
     int stride;         // part_capacity
     const T *y, *yc, *ax;
     T *axc;
-    __device__ void begin()
+    // Reads the buffer table from the kernel-argument copy `k`: selecting
+    // among this object's own copy by a runtime index would place it in
+    // scratch memory.
+    template <class K> __device__ void begin(const K &k)
     {
         const int cur = st->cur;
-        y = I.Y[cur]; yc = I.Y[cur ^ 1]; ax = I.AX[st->a0]; axc = I.AX[st->a2];
+        y = k.I.y(cur); yc = k.I.y(cur ^ 1); ax = k.I.ax(st->a0); axc = k.I.ax(st->a2);
     }
     __device__ void row(int i, AT s, double *p) const
     {
@@ -472,10 +486,10 @@ template <typename T, typename AT> struct EpiQProbe {
     double *part;
     const T *x, *xc, *qx;
     T *qxc;
-    __device__ void begin()
+    template <class K> __device__ void begin(const K &)
     {
         const int cur = st->cur;
-        x = I.X[cur]; xc = I.X[cur ^ 1]; qx = I.QX[cur]; qxc = I.QX[cur ^ 1];
+        x = I.x(cur); xc = I.x(cur ^ 1); qx = I.qx(cur); qxc = I.qx(cur ^ 1);
     }
     __device__ void row(int j, AT s, double *p) const
     {
@@ -501,9 +515,9 @@ template <typename T, typename AT> struct EpiDualFixed {
     const skk_state *st;
     T *y;
     T sigma;
-    __device__ void begin()
+    template <class K> __device__ void begin(const K &)
     {
-        y = I.Y[st->cur];
+        y = I.y(st->cur);
         sigma = static_cast<T>(st->eta_fixed * st->omega);
     }
     __device__ void row(int i, AT s, double *) const
@@ -525,9 +539,9 @@ template <typename T, typename AT> struct EpiPrimalFixed {
     const skk_state *st;
     T *x;
     T tau;
-    __device__ void begin()
+    template <class K> __device__ void begin(const K &)
     {
-        x = I.X[st->cur];
+        x = I.x(st->cur);
         tau = static_cast<T>(st->eta_fixed / st->omega);
     }
     __device__ void row(int j, AT s, double *) const
@@ -549,8 +563,8 @@ __global__ void k_flush(Iter<T> I, const skk_state *__restrict__ st, int n, int 
     if (!st->pending) return;
     const int cur = st->cur;
     const int stride = gridDim.x * blockDim.x;
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) I.xsum[j] += I.X[cur][j];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) I.ysum[i] += I.Y[cur][i];
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) I.xsum[j] += I.x(cur)[j];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) I.ysum[i] += I.y(cur)[i];
 }
 
 template <typename T>
@@ -582,8 +596,8 @@ __global__ __launch_bounds__(kBlock) void k_residuals_sel(Iter<T> I, int which,
                                                           const double *__restrict__ qx)
 {
     const int cur = st->cur;
-    const T *__restrict__ x = which ? I.xavg : I.X[cur];
-    const T *__restrict__ y = which ? I.yavg : I.Y[cur];
+    const T *__restrict__ x = which ? I.xavg : I.x(cur);
+    const T *__restrict__ y = which ? I.yavg : I.y(cur);
     const T *__restrict__ xr = I.xr;
     const T *__restrict__ yr = I.yr;
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -626,11 +640,11 @@ template <typename T>
 __global__ void k_to_t_sel(Iter<T> I, const double *__restrict__ src, int m,
                            const skk_state *__restrict__ st, const double *__restrict__ qsrc, int n)
 {
-    T *dst = I.AX[st->a0];
+    T *dst = I.ax(st->a0);
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) dst[i] = static_cast<T>(src[i]);
     if (qsrc) {
-        T *q = I.QX[st->cur];
+        T *q = I.qx(st->cur);
         for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) q[j] = static_cast<T>(qsrc[j]);
     }
 }
@@ -644,21 +658,21 @@ __global__ void k_restart(Iter<T> I, skk_state *st, const double *__restrict__ a
     const int cur = st->cur, a0 = st->a0, a1 = st->a1;
     const int stride = gridDim.x * blockDim.x;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
-        T x = I.X[cur][j];
-        if (use_avg) { x = I.xavg[j]; I.X[cur][j] = x; if (qx_avg) I.QX[cur][j] = static_cast<T>(qx_avg[j]); }
+        T x = I.x(cur)[j];
+        if (use_avg) { x = I.xavg[j]; I.x(cur)[j] = x; if (qx_avg) I.qx(cur)[j] = static_cast<T>(qx_avg[j]); }
         I.xbar[j] = x;
         I.xsum[j] = T(0);
         I.xr[j] = x;
     }
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
-        T y = I.Y[cur][i];
-        if (use_avg) { y = I.yavg[i]; I.Y[cur][i] = y; }
+        T y = I.y(cur)[i];
+        if (use_avg) { y = I.yavg[i]; I.y(cur)[i] = y; }
         I.ysum[i] = T(0);
         I.yr[i] = y;
         if (adaptive) {
-            const T a = use_avg ? static_cast<T>(ax_avg[i]) : I.AX[a0][i];
-            I.AX[a0][i] = a;
-            I.AX[a1][i] = a;
+            const T a = use_avg ? static_cast<T>(ax_avg[i]) : I.ax(a0)[i];
+            I.ax(a0)[i] = a;
+            I.ax(a1)[i] = a;
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) st->accepted_at_restart = st->accept_count;
@@ -687,9 +701,9 @@ __global__ void k_export(Iter<T> I, const skk_state *__restrict__ st, int n, int
     const int cur = st->cur;
     const int stride = gridDim.x * blockDim.x;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride)
-        xo[j] = static_cast<double>(use_avg ? I.xavg[j] : I.X[cur][j]);
+        xo[j] = static_cast<double>(use_avg ? I.xavg[j] : I.x(cur)[j]);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride)
-        yo[i] = static_cast<double>(use_avg ? I.yavg[i] : I.Y[cur][i]);
+        yo[i] = static_cast<double>(use_avg ? I.yavg[i] : I.y(cur)[i]);
 }
 
 // ------------------------------------------------------- power iteration
@@ -697,7 +711,7 @@ __global__ void k_export(Iter<T> I, const skk_state *__restrict__ st, int n, int
 struct EpiNormSq {
     static constexpr int kSlots = 1;
     double *y, *part;
-    __device__ void begin() {}
+    template <class K> __device__ void begin(const K &) {}
     __device__ void row(int r, double s, double *p) const { y[r] = s; p[0] += s * s; }
     __device__ void end(double *p) const
     {
@@ -740,7 +754,7 @@ struct EpiRc {
     static constexpr int kSlots = 0;
     double *rc;
     const double *c, *qx;
-    __device__ void begin() {}
+    template <class K> __device__ void begin(const K &) {}
     __device__ void row(int j, double s, double *) const { rc[j] = c[j] + (qx ? qx[j] : 0.0) + s; }
     __device__ void end(double *) const {}
 };
@@ -858,7 +872,7 @@ struct EpiLegacyDual {
     double *y;
     const double *rl, *ru, *sigma_vec;
     double sigma;
-    __device__ void begin() {}
+    template <class K> __device__ void begin(const K &) {}
     __device__ void row(int i, double s, double *) const
     {
         const double sg = sigma_vec ? sigma_vec[i] : sigma;
@@ -875,7 +889,7 @@ struct EpiLegacyPrimal {
     double *x, *xbar;
     const double *c, *l, *u, *tau_vec, *qg, *qd;
     double tau, theta;
-    __device__ void begin() {}
+    template <class K> __device__ void begin(const K &) {}
     __device__ void row(int j, double s, double *) const
     {
         const double t = tau_vec ? tau_vec[j] : tau;
@@ -1169,6 +1183,12 @@ void skk_host_free(void *ptr)
 int skk_sync(void) { return check(hipStreamSynchronize(stream())); }
 
 int skk_last_launch_error(void) { return g_last; }
+
+void skk_reset_error(void)
+{
+    (void)hipGetLastError();   // drop a runtime error left by an earlier, already reported call
+    g_last = 0;
+}
 
 const char *skk_error_text(int code) { return hipGetErrorString(static_cast<hipError_t>(code)); }
 

@@ -127,7 +127,7 @@ class BatchOptions(C.Structure):
 
 class BatchResult(C.Structure):
     _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("restarts", C.c_int),
-                ("relative_primal_residual", C.c_double), ("relative_dual_residual", C.c_double),
+                ("fp32_iterations", C.c_int), ("relative_primal_residual", C.c_double), ("relative_dual_residual", C.c_double),
                 ("relative_gap", C.c_double), ("objective", C.c_double)]
 
 
@@ -135,7 +135,8 @@ class BatchStats(C.Structure):
     _fields_ = [("setup_seconds", C.c_double), ("device_seconds", C.c_double),
                 ("seconds", C.c_double), ("block_threads", C.c_int), ("grid", C.c_int),
                 ("lds_bytes", C.c_int), ("converged", C.c_int), ("total_iterations", C.c_longlong),
-                ("tune_ms", C.c_double * 3)]
+                ("tune_ms", C.c_double * 3), ("fp32_iterations", C.c_longlong), ("handoffs", C.c_int),
+                ("grid_fp64", C.c_int), ("lds_bytes_fp64", C.c_int), ("pad", C.c_int)]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
