@@ -219,8 +219,8 @@ static int solve_qp(const sk_gpu_csr *matrix, const sk_gpu_csr *hessian, const d
     L.m = rows; L.n = cols;
     L.A = &pa->A.k; L.AT = &pa->AT.k; L.Q = hessian ? &ph->A.k : NULL;
     L.tau = st.tau; L.sigma = st.sigma; L.theta = st.theta;
-    cap = pa->A.k.bins.total_blocks;
-    if (pa->AT.k.bins.total_blocks > cap) cap = pa->AT.k.bins.total_blocks;
+    cap = pa->A.k.bins.grid;
+    if (pa->AT.k.bins.grid > cap) cap = pa->AT.k.bins.grid;
     k = ((rows > cols ? rows : cols) + SKK_BLOCK - 1) / SKK_BLOCK;
     if (k < 1) k = 1;
     if (k > 1024) k = 1024;

@@ -318,6 +318,7 @@ int skg_plan_bins_keyed(const int *rp, int rows, const skg_tuning *t, const int 
     }
     bins->nbins = nb;
     bins->total_blocks = blocks;
+    bins->grid = blocks < SKK_SPMV_GRID ? blocks : SKK_SPMV_GRID;
     if (order) {
         int next[9];
         if (key) {

@@ -48,8 +48,12 @@ void  *skk_stream(void);
  * an LDS combine across its four wavefronts).  Bins own consecutive
  * workgroups starting at block_begin. */
 typedef struct skk_bin { int row_begin, row_end, log2L, block_begin; } skk_bin;
+/* Workgroups launched for one product: the first SKK_SPMV_GRID of the
+ * plan's total_blocks row tiles; workgroup g takes tiles g, g + grid, ...
+ * so block partials never exceed SKK_SPMV_GRID per slot. */
+#define SKK_SPMV_GRID 4096
 typedef struct skk_bins {
-    int nbins, total_blocks;
+    int nbins, total_blocks, grid;
     skk_bin b[SKK_MAX_BINS];
     const int *row_list;     /* NULL: bin slot k is row k; else row_list[k] */
 } skk_bins;

@@ -143,8 +143,17 @@ __device__ __forceinline__ void store_partials(double (&v)[S], double *part, int
 // order, then the workgroup tree, so the total is bit-reproducible.
 __device__ double combine(const double *part, int count, bool is_max)
 {
-    double acc[1] = {is_max ? -INFINITY : 0.0};
-    for (int b = threadIdx.x; b < count; b += kBlock) acc[0] = is_max ? fmax(acc[0], part[b]) : acc[0] + part[b];
+    // four independent chains keep four loads in flight per thread
+    const double z = is_max ? -INFINITY : 0.0;
+    double a0 = z, a1 = z, a2 = z, a3 = z;
+    int b = threadIdx.x;
+    for (; b + 3 * kBlock < count; b += 4 * kBlock) {
+        const double v0 = part[b], v1 = part[b + kBlock], v2 = part[b + 2 * kBlock], v3 = part[b + 3 * kBlock];
+        if (is_max) { a0 = fmax(a0, v0); a1 = fmax(a1, v1); a2 = fmax(a2, v2); a3 = fmax(a3, v3); }
+        else { a0 += v0; a1 += v1; a2 += v2; a3 += v3; }
+    }
+    for (; b < count; b += kBlock) a0 = is_max ? fmax(a0, part[b]) : a0 + part[b];
+    double acc[1] = {is_max ? fmax(fmax(a0, a1), fmax(a2, a3)) : (a0 + a1) + (a2 + a3)};
     double out[1];
     if (is_max) block_reduce<1, true>(acc, out);
     else block_reduce<1>(acc, out);
@@ -198,39 +207,41 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
     Epi e = epi;
     e.begin(epi);
     const XT *__restrict__ x = (sel && (sel->cur ^ 1)) ? x1 : x0;
-    const int blk = blockIdx.x;
-    // bin of this workgroup, chosen with constant indices only (a dynamic
-    // index into the kernel-argument table would copy it to scratch)
-    skk_bin d = bins.b[0];
-#pragma unroll
-    for (int q = 1; q < SKK_MAX_BINS; ++q)
-        if (q < bins.nbins && blk >= bins.b[q].block_begin) d = bins.b[q];
     double part[Epi::kSlots > 0 ? Epi::kSlots : 1] = {};
-    if (d.log2L <= 6) {
-        const int log2L = d.log2L, L = 1 << log2L;
-        const int lane = threadIdx.x & (L - 1);
-        const int k = d.row_begin + ((blk - d.block_begin) << (8 - log2L)) + (threadIdx.x >> log2L);
-        const bool valid = k < d.row_end;
-        int row = 0;
-        AT sum = 0;
-        if (valid) {
-            row = bins.row_list ? bins.row_list[k] : k;
-            sum = row_part<VT, XT, AT>(ci, val, x, rp[row] + lane, rp[row + 1], L);
-        }
-        for (int off = L >> 1; off > 0; off >>= 1) sum += shfl_xor_d(sum, off);
-        if (valid && lane == 0) e.row(row, sum, part);
-    } else {
-        __shared__ AT sh[kWaves];
-        const int k = d.row_begin + (blk - d.block_begin);
-        const int row = bins.row_list ? bins.row_list[k] : k;
-        AT sum = row_part<VT, XT, AT>(ci, val, x, rp[row] + threadIdx.x, rp[row + 1], kBlock);
-        sum = wave_sum(sum);
-        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = sum;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            AT t = sh[0];
-            for (int w = 1; w < kWaves; ++w) t += sh[w];
-            e.row(row, t, part);
+    for (int blk = blockIdx.x; blk < bins.total_blocks; blk += gridDim.x) {
+        // bin of this row tile, chosen with constant indices only (a dynamic
+        // index into the kernel-argument table would copy it to scratch)
+        skk_bin d = bins.b[0];
+#pragma unroll
+        for (int q = 1; q < SKK_MAX_BINS; ++q)
+            if (q < bins.nbins && blk >= bins.b[q].block_begin) d = bins.b[q];
+        if (d.log2L <= 6) {
+            const int log2L = d.log2L, L = 1 << log2L;
+            const int lane = threadIdx.x & (L - 1);
+            const int k = d.row_begin + ((blk - d.block_begin) << (8 - log2L)) + (threadIdx.x >> log2L);
+            const bool valid = k < d.row_end;
+            int row = 0;
+            AT sum = 0;
+            if (valid) {
+                row = bins.row_list ? bins.row_list[k] : k;
+                sum = row_part<VT, XT, AT>(ci, val, x, rp[row] + lane, rp[row + 1], L);
+            }
+            for (int off = L >> 1; off > 0; off >>= 1) sum += shfl_xor_d(sum, off);
+            if (valid && lane == 0) e.row(row, sum, part);
+        } else {
+            __shared__ AT sh[kWaves];
+            const int k = d.row_begin + (blk - d.block_begin);
+            const int row = bins.row_list ? bins.row_list[k] : k;
+            AT sum = row_part<VT, XT, AT>(ci, val, x, rp[row] + threadIdx.x, rp[row + 1], kBlock);
+            sum = wave_sum(sum);
+            if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = sum;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                AT t = sh[0];
+                for (int w = 1; w < kWaves; ++w) t += sh[w];
+                e.row(row, t, part);
+            }
+            __syncthreads();   // sh is reused by the next tile
         }
     }
     e.end(part);
@@ -249,7 +260,7 @@ int launch_spmv(const skk_csr &A, const VT *vals, const XT *x0, const XT *x1,
                 const skk_state *sel, const Epi &epi)
 {
     if (A.rows == 0 || A.bins.total_blocks == 0) return 0;
-    hipLaunchKernelGGL((k_spmv<VT, XT, AT, Epi>), dim3(A.bins.total_blocks), dim3(kBlock), 0,
+    hipLaunchKernelGGL((k_spmv<VT, XT, AT, Epi>), dim3(A.bins.grid), dim3(kBlock), 0,
                        stream(), A.rp, A.ci, vals, x0, x1, sel, A.bins, epi);
     return launch_status();
 }
@@ -1000,7 +1011,7 @@ template <typename T> int probe(const skk_pdhg &P, const Iter<T> &I)
     int rc = launch_spmv<T, T, T>(P.A, vals_of<T>(P.A), I.X[0], I.X[1], P.st, pr);
     if (rc) return rc;
     hipLaunchKernelGGL(k_probe_finalize, dim3(1), dim3(kBlock), 0, stream(), P.st, P.part, P.part_capacity,
-                       P.AT.bins.total_blocks, P.has_q ? P.Q.bins.total_blocks : 0, P.A.bins.total_blocks);
+                       P.AT.bins.grid, P.has_q ? P.Q.bins.grid : 0, P.A.bins.grid);
     return launch_status();
 }
 
@@ -1421,7 +1432,7 @@ int skk_power_norm(const skk_pdhg *P, double *v, double *w, double *v2, int iter
         EpiNormSq ns{};
         ns.y = v2; ns.part = P->part + (size_t)R_POWER * P->part_capacity;
         if ((rc = launch_spmv<double, double, double>(P->AT, P->AT.vd, w, w, nullptr, ns))) return rc;
-        if ((rc = finalize(ns.part, P->part_capacity, P->AT.bins.total_blocks, 1, 0, out))) return rc;
+        if ((rc = finalize(ns.part, P->part_capacity, P->AT.bins.grid, 1, 0, out))) return rc;
         hipLaunchKernelGGL(k_normalize_from, dim3(grid_for(n, P->elem_grid)), dim3(kBlock), 0,
                            stream(), v, v2, out, n);
         if ((rc = launch_status())) return rc;

@@ -353,9 +353,9 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
         if (eg > 2048) eg = 2048;
         Q->elem_grid = eg;
         cap = eg;
-        if (bA.total_blocks > cap) cap = bA.total_blocks;
-        if (bAT.total_blocks > cap) cap = bAT.total_blocks;
-        if (E->has_q && E->Q.k.bins.total_blocks > cap) cap = E->Q.k.bins.total_blocks;
+        if (bA.grid > cap) cap = bA.grid;
+        if (bAT.grid > cap) cap = bAT.grid;
+        if (E->has_q && E->Q.k.bins.grid > cap) cap = E->Q.k.bins.grid;
         Q->part_capacity = cap;
         if ((rc = dalloc(E, (void **)&Q->ax_d, 8 * mm)) || (rc = dalloc(E, (void **)&Q->aty_d, 8 * nn)) ||
             (rc = dalloc(E, (void **)&Q->st, sizeof(skk_state))) ||
