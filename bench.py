@@ -71,7 +71,7 @@ class Session:
             raise RuntimeError(f"session create failed: {sg.last_error()}")
 
     def iterate(self, k, timed=False):
-        ms = np.zeros(4)
+        ms = np.zeros(5)
         rc = self.lib.sk_gpu_pdlp_session_iterate(self.h, k, sg.dptr(ms) if timed else None)
         if rc != 0:
             raise RuntimeError(f"iterate failed: {sg.last_error()}")
@@ -174,6 +174,7 @@ def workload_lp1(args):
     k1, k2, k3 = kernel_bytes(m, n, nnz, 4, 4, 1.0)
     kernels = {"dual_prox": (k1, mix["kernel_ms"][0]), "AT_spmv_primal": (k2, mix["kernel_ms"][1]),
                "A_spmv_probe": (k3, mix["kernel_ms"][2])}
+    # [3] is the one-workgroup accept/reject decision (no HBM stream), [4] the iteration
     dom = max(kernels, key=lambda k: kernels[k][1])
     bytes_dom, ms_dom = kernels[dom]
     achieved = bytes_dom / (ms_dom * 1e-3) / 1e9

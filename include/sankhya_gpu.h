@@ -334,9 +334,11 @@ int sk_gpu_pdlp_solve(int rows, int cols, int nnz, const int *row_offsets,
  * stay on the device across solves (the role of a GpuPdlp object,
  * src/cuda/GpuPdlp.hpp:128-196).  `update` replaces c and the bounds (same
  * A) -- the parametric-family case; `iterate` runs raw iterations from a
- * cold start with no checks (benchmarking; with kernel_ms it also returns
- * the mean event-timed milliseconds of the three launches of an iteration
- * and of the whole iteration, kernel_ms[0..3]). */
+ * cold start with no checks (benchmarking; with kernel_ms[5] it also
+ * returns the mean event-timed milliseconds of the parts of an adaptive
+ * iteration -- [0] dual prox, [1] A^T y SpMV + primal prox, [2] (Q x and)
+ * A x SpMV + step probe, [3] accept/reject decision -- and [4] of the whole
+ * iteration). */
 typedef struct sk_gpu_pdlp_session sk_gpu_pdlp_session;
 int  sk_gpu_pdlp_session_create(int rows, int cols, int nnz, const int *row_offsets,
                                 const int *column_indices, const double *values,

@@ -29,14 +29,7 @@ void sk_gpu_csr_destroy(sk_gpu_csr *matrix)     /* cuda/src/sankhya_cuda.cu:154-
 
 static int plan_upload(dcsr *d, const hcsr *a)
 {
-    skk_bins bins;
-    int *order = (int *)malloc(sizeof(int) * (size_t)(a->rows + 1));
-    int rc;
-    if (!order) return skg_fail("out of host memory");
-    if (skg_plan_bins(a->p, a->rows, &g_skg_tuning, order, &bins)) { free(order); return skg_fail("out of host memory"); }
-    rc = dcsr_upload(d, a, &bins, order, 0);
-    free(order);
-    return rc;
+    return dcsr_upload(d, a, 0);
 }
 
 /* cuda/src/sankhya_cuda.cu:116-152 sankhya_cuda_csr_create */
@@ -219,8 +212,8 @@ static int solve_qp(const sk_gpu_csr *matrix, const sk_gpu_csr *hessian, const d
     L.m = rows; L.n = cols;
     L.A = &pa->A.k; L.AT = &pa->AT.k; L.Q = hessian ? &ph->A.k : NULL;
     L.tau = st.tau; L.sigma = st.sigma; L.theta = st.theta;
-    cap = pa->A.k.bins.grid;
-    if (pa->AT.k.bins.grid > cap) cap = pa->AT.k.bins.grid;
+    cap = pa->A.k.grid;
+    if (pa->AT.k.grid > cap) cap = pa->AT.k.grid;
     k = ((rows > cols ? rows : cols) + SKK_BLOCK - 1) / SKK_BLOCK;
     if (k < 1) k = 1;
     if (k > 1024) k = 1024;

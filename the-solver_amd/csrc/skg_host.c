@@ -318,7 +318,6 @@ int skg_plan_bins_keyed(const int *rp, int rows, const skg_tuning *t, const int 
     }
     bins->nbins = nb;
     bins->total_blocks = blocks;
-    bins->grid = blocks < SKK_SPMV_GRID ? blocks : SKK_SPMV_GRID;
     if (order) {
         int next[9];
         if (key) {
@@ -349,29 +348,66 @@ int skg_plan_bins(const int *rp, int rows, const skg_tuning *t, int *order, skk_
 
 /* ----------------------------------------------------------- device CSR */
 
+/* CSR-stream tiles of consecutive rows (greedy, in row order): a tile
+ * closes before it would exceed SKK_TILE_NNZ entries or SKK_TILE_ROWS rows;
+ * a row longer than SKK_TILE_NNZ is a tile of its own.  At least one
+ * (possibly empty) tile, so fused epilogues always run.  Returns the tile
+ * count, or -1 when out of memory; *tile_row receives malloc'd offsets. */
+int skg_plan_tiles(const int *rp, int rows, int **tile_row)
+{
+    int cap = rows / 64 + 16, nt = 0, r = 0;
+    int *t = (int *)malloc(sizeof(int) * (size_t)cap);
+    if (!t) return -1;
+    t[0] = 0;
+    while (r < rows || nt == 0) {
+        int nnz = 0, cnt = 0;
+        if (nt + 2 > cap) {
+            int *g;
+            cap *= 2;
+            g = (int *)realloc(t, sizeof(int) * (size_t)cap);
+            if (!g) { free(t); return -1; }
+            t = g;
+        }
+        if (r < rows && rp[r + 1] - rp[r] > SKK_TILE_NNZ) {
+            ++r;
+        } else {
+            while (r < rows && cnt < SKK_TILE_ROWS && nnz + (rp[r + 1] - rp[r]) <= SKK_TILE_NNZ) {
+                nnz += rp[r + 1] - rp[r];
+                ++r;
+                ++cnt;
+            }
+        }
+        t[++nt] = r;
+    }
+    *tile_row = t;
+    return nt;
+}
+
 void dcsr_free(dcsr *d)
 {
-    if (!d) return;
-    skk_free(d->rp); skk_free(d->ci); skk_free(d->vd); skk_free(d->vf); skk_free(d->rows_list);
+    skk_free(d->rp); skk_free(d->ci); skk_free(d->vd); skk_free(d->vf); skk_free(d->tiles);
     memset(d, 0, sizeof(*d));
 }
 
-/* Upload a host CSR; `order` (may be NULL) is the bin slot -> row list for a
- * matrix kept in caller row order; with order == NULL the rows must already
- * be sorted by class. */
-int dcsr_upload(dcsr *d, const hcsr *a, const skk_bins *bins, const int *order, int with_f32)
+/* Upload a host CSR (rows in the order the products run) with its tiles. */
+int dcsr_upload(dcsr *d, const hcsr *a, int with_f32)
 {
-    int rc;
+    int rc, nt, *tiles = NULL;
     memset(d, 0, sizeof(*d));
     d->k.rows = a->rows; d->k.cols = a->cols; d->k.nnz = a->nnz;
-    d->k.bins = *bins;
+    nt = skg_plan_tiles(a->p, a->rows, &tiles);
+    if (nt < 0) return skg_fail("out of host memory");
+    d->k.ntiles = nt;
+    d->k.grid = nt < SKK_SPMV_GRID ? nt : SKK_SPMV_GRID;
     if ((rc = skk_malloc((void **)&d->rp, sizeof(int) * ((size_t)a->rows + 1))) ||
         (rc = skk_malloc((void **)&d->ci, sizeof(int) * (size_t)(a->nnz ? a->nnz : 1))) ||
-        (rc = skk_malloc((void **)&d->vd, sizeof(double) * (size_t)(a->nnz ? a->nnz : 1))))
+        (rc = skk_malloc((void **)&d->vd, sizeof(double) * (size_t)(a->nnz ? a->nnz : 1))) ||
+        (rc = skk_malloc((void **)&d->tiles, sizeof(int) * ((size_t)nt + 1))))
         goto fail;
     if ((rc = skk_h2d(d->rp, a->p, sizeof(int) * ((size_t)a->rows + 1))) ||
         (rc = skk_h2d(d->ci, a->j, sizeof(int) * (size_t)a->nnz)) ||
-        (rc = skk_h2d(d->vd, a->v, sizeof(double) * (size_t)a->nnz)))
+        (rc = skk_h2d(d->vd, a->v, sizeof(double) * (size_t)a->nnz)) ||
+        (rc = skk_h2d(d->tiles, tiles, sizeof(int) * ((size_t)nt + 1))))
         goto fail;
     if (with_f32) {
         float *tmp = (float *)malloc(sizeof(float) * (size_t)(a->nnz ? a->nnz : 1));
@@ -382,15 +418,11 @@ int dcsr_upload(dcsr *d, const hcsr *a, const skk_bins *bins, const int *order, 
             (rc = skk_h2d(d->vf, tmp, sizeof(float) * (size_t)a->nnz))) { free(tmp); goto fail; }
         free(tmp);
     }
-    if (order) {
-        if ((rc = skk_malloc((void **)&d->rows_list, sizeof(int) * (size_t)(a->rows ? a->rows : 1))) ||
-            (rc = skk_h2d(d->rows_list, order, sizeof(int) * (size_t)a->rows)))
-            goto fail;
-    }
-    d->k.rp = d->rp; d->k.ci = d->ci; d->k.vd = d->vd; d->k.vf = d->vf;
-    d->k.bins.row_list = d->rows_list;
+    free(tiles);
+    d->k.rp = d->rp; d->k.ci = d->ci; d->k.vd = d->vd; d->k.vf = d->vf; d->k.tile_row = d->tiles;
     return 0;
 fail:
+    free(tiles);
     dcsr_free(d);
     return skg_hip_fail("device CSR upload", rc);
 }

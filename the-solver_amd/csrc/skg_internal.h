@@ -42,15 +42,16 @@ int  skg_plan_bins(const int *rp, int rows, const skg_tuning *t, int *order, skk
 int  skg_plan_bins_keyed(const int *rp, int rows, const skg_tuning *t, const int *key, int key_range,
                          int *order, skk_bins *bins);
 
-/* Device CSR with its bin plan. */
+/* Device CSR with its row tiles. */
 typedef struct dcsr {
-    int *rp, *ci, *rows_list;
+    int *rp, *ci, *tiles;
     double *vd;
     float *vf;
     skk_csr k;
 } dcsr;
 
-int  dcsr_upload(dcsr *d, const hcsr *a, const skk_bins *bins, const int *order, int with_f32);
+int  skg_plan_tiles(const int *rp, int rows, int **tile_row);
+int  dcsr_upload(dcsr *d, const hcsr *a, int with_f32);
 void dcsr_free(dcsr *d);
 
 /* Ruiz equilibration (src/lp/Scaling.cpp:11-92) and Pock-Chambolle alpha=1

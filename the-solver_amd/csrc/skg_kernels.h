@@ -43,27 +43,33 @@ void  *skk_stream(void);
 #define SKK_MAX_BINS 12
 #define SKK_BLOCK 256
 
-/* Rows [row_begin,row_end) of a bin are reduced by 2^log2L lanes each
- * (log2L 0..6 inside one wavefront; 8 = one 256-thread workgroup per row with
- * an LDS combine across its four wavefronts).  Bins own consecutive
- * workgroups starting at block_begin. */
+/* Host-side row length classes that order the problem's rows and columns
+ * (class k: rows of about 4 * 2^k entries, k 0..6; 8 above the long-row
+ * threshold), so that rows of similar length are neighbours.  Bins own
+ * consecutive row slots starting at row_begin. */
 typedef struct skk_bin { int row_begin, row_end, log2L, block_begin; } skk_bin;
-/* Workgroups launched for one product: the first SKK_SPMV_GRID of the
- * plan's total_blocks row tiles; workgroup g takes tiles g, g + grid, ...
- * so block partials never exceed SKK_SPMV_GRID per slot. */
-#define SKK_SPMV_GRID 4096
 typedef struct skk_bins {
-    int nbins, total_blocks, grid;
+    int nbins, total_blocks;
     skk_bin b[SKK_MAX_BINS];
-    const int *row_list;     /* NULL: bin slot k is row k; else row_list[k] */
 } skk_bins;
+
+/* CSR-stream row tiles: tile t covers rows [tile_row[t], tile_row[t+1]),
+ * consecutive rows holding at most SKK_TILE_NNZ entries together (a tile
+ * of one longer row is reduced by the whole workgroup) and at most
+ * SKK_TILE_ROWS rows.  A product launches min(ntiles, SKK_SPMV_GRID)
+ * workgroups; workgroup g takes tiles g, g + grid, ..., so block partials
+ * never exceed SKK_SPMV_GRID per slot. */
+#define SKK_TILE_NNZ 2048
+#define SKK_TILE_ROWS 1024
+#define SKK_SPMV_GRID 4096
 
 typedef struct skk_csr {
     int rows, cols, nnz;
     const int *rp, *ci;
     const float *vf;         /* FP32 values (may be NULL) */
     const double *vd;        /* FP64 values               */
-    skk_bins bins;
+    const int *tile_row;     /* ntiles + 1 row offsets    */
+    int ntiles, grid;
 } skk_csr;
 
 /* y = A x in FP64 (plain product, public ABI). */

@@ -3,18 +3,18 @@
 // Design (MI355X-first, not a translation of the reference's CUDA):
 //  * Sparse products are CSR gathers on two stored operators, A and A^T, so no
 //    atomics are needed (the reference's transpose scatters with atomicAdd,
-//    cuda/src/sankhya_cuda.cu:45-53).  Rows are grouped into length bins and
-//    each bin reduces a row with 2^k lanes of a 64-wide wavefront (k = 0..6),
-//    or with a whole 256-thread workgroup combined through LDS for the long
-//    power-law rows.  One launch covers every bin.  Matrix streams use
-//    non-temporal loads so L2 keeps the gathered vector.
+//    cuda/src/sankhya_cuda.cu:45-53).  Rows are cut into tiles of at most
+//    2048 entries; a workgroup streams a tile's products into LDS with
+//    coalesced loads and sums each row from LDS (CSR-stream), and a
+//    power-law row longer than a tile gets a whole workgroup.  Matrix
+//    streams use non-temporal loads so L2 keeps the gathered vector.
 //  * The PDHG updates are epilogues of those products: the dual prox is
 //    fused into A*x, the primal prox + extrapolation + running sums into
 //    A^T*y (src/cuda/PdlpKernels.cu:71-97 are separate kernels there).
 //  * Reductions are two-stage: wavefront butterfly + LDS inside a workgroup,
-//    then the last workgroup to retire combines the per-block partials in
-//    ascending block order, so every sum is bit-reproducible for a fixed
-//    grid (the property PdlpKernels.cu:10-25 asks for).
+//    then a one-workgroup launch combines the per-block partials in a fixed
+//    order, so every sum is bit-reproducible for a fixed grid (the property
+//    PdlpKernels.cu:10-25 asks for).
 //  * The adaptive-step accept/reject decision (PdlpKernels.cu:405-518) is
 //    taken on the device and committed by rotating buffer indices in a
 //    device-resident state struct instead of copying vectors, so a restart
@@ -195,54 +195,78 @@ __device__ __forceinline__ AT row_part(const int *__restrict__ ci, const VT *__r
     return s0 + s1;
 }
 
-// The gathered vector is x0, or -- when `sel` is given -- the candidate
-// buffer of the rotating pair (x0, x1) named by the device state (cur ^ 1).
+// CSR-stream product over row tiles (the CSR-Adaptive scheme of Greathouse
+// and Daga, SC'14): a tile's entries are multiplied by consecutive lanes --
+// fully coalesced value and index loads -- into LDS, then each row of the
+// tile is summed from LDS by 2^k lanes (k chosen from the tile's row count)
+// or, when the tile has more rows than lanes, by one lane per row.  A tile
+// of one row longer than SKK_TILE_NNZ is reduced by the whole workgroup
+// straight from memory.  The gathered vector is x0, or -- when `sel` is
+// given -- the candidate buffer of the rotating pair (x0, x1) named by the
+// device state (cur ^ 1).  Sums are in a fixed order, so the result does
+// not depend on the launch shape.
 template <typename VT, typename XT, typename AT, class Epi>
 __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
                                                  const int *__restrict__ ci,
                                                  const VT *__restrict__ val, const XT *x0,
                                                  const XT *x1, const skk_state *sel,
-                                                 skk_bins bins, Epi epi)
+                                                 const int *__restrict__ tile_row, int ntiles, Epi epi)
 {
+    constexpr int kPer = SKK_TILE_NNZ / kBlock;
+    __shared__ AT prod[SKK_TILE_NNZ];
     Epi e = epi;
     e.begin(epi);
     const XT *__restrict__ x = (sel && (sel->cur ^ 1)) ? x1 : x0;
+    const int tid = threadIdx.x;
     double part[Epi::kSlots > 0 ? Epi::kSlots : 1] = {};
-    for (int blk = blockIdx.x; blk < bins.total_blocks; blk += gridDim.x) {
-        // bin of this row tile, chosen with constant indices only (a dynamic
-        // index into the kernel-argument table would copy it to scratch)
-        skk_bin d = bins.b[0];
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int r0 = tile_row[t], r1 = tile_row[t + 1];
+        const int p0 = rp[r0], p1 = rp[r1];
+        const int nr = r1 - r0;
+        if (p1 - p0 > SKK_TILE_NNZ) {
+            // one long row: workgroup-strided partial sums, wave and LDS combine
+            AT sum = row_part<VT, XT, AT>(ci, val, x, p0 + tid, p1, kBlock);
+            sum = wave_sum(sum);
+            if ((tid & 63) == 0) prod[tid >> 6] = sum;
+            __syncthreads();
+            if (tid == 0) {
+                AT tot = prod[0];
+                for (int w = 1; w < kWaves; ++w) tot += prod[w];
+                e.row(r0, tot, part);
+            }
+            __syncthreads();
+            continue;
+        }
+        // products of the tile's entries, kPer independent loads per lane
 #pragma unroll
-        for (int q = 1; q < SKK_MAX_BINS; ++q)
-            if (q < bins.nbins && blk >= bins.b[q].block_begin) d = bins.b[q];
-        if (d.log2L <= 6) {
-            const int log2L = d.log2L, L = 1 << log2L;
-            const int lane = threadIdx.x & (L - 1);
-            const int k = d.row_begin + ((blk - d.block_begin) << (8 - log2L)) + (threadIdx.x >> log2L);
-            const bool valid = k < d.row_end;
-            int row = 0;
+        for (int k = 0; k < kPer; ++k) {
+            const int p = p0 + k * kBlock + tid;
+            if (p < p1) prod[p - p0] = static_cast<AT>(ntload(val + p)) * static_cast<AT>(x[ntload(ci + p)]);
+        }
+        __syncthreads();
+        if (nr * 2 <= kBlock) {
+            // 2^lg lanes per row, lg = floor(log2(kBlock / nr)) capped at 6
+            int lg = 0;
+            while (lg < 6 && (nr << (lg + 1)) <= kBlock) ++lg;
+            const int L = 1 << lg, g = tid >> lg, lane = tid & (L - 1);
             AT sum = 0;
-            if (valid) {
-                row = bins.row_list ? bins.row_list[k] : k;
-                sum = row_part<VT, XT, AT>(ci, val, x, rp[row] + lane, rp[row + 1], L);
+            int row = r0 + g;
+            if (g < nr) {
+                const int q1 = rp[row + 1] - p0;
+                for (int q = rp[row] - p0 + lane; q < q1; q += L) sum += prod[q];
             }
             for (int off = L >> 1; off > 0; off >>= 1) sum += shfl_xor_d(sum, off);
-            if (valid && lane == 0) e.row(row, sum, part);
+            if (g < nr && lane == 0) e.row(row, sum, part);
         } else {
-            __shared__ AT sh[kWaves];
-            const int k = d.row_begin + (blk - d.block_begin);
-            const int row = bins.row_list ? bins.row_list[k] : k;
-            AT sum = row_part<VT, XT, AT>(ci, val, x, rp[row] + threadIdx.x, rp[row + 1], kBlock);
-            sum = wave_sum(sum);
-            if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = sum;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                AT t = sh[0];
-                for (int w = 1; w < kWaves; ++w) t += sh[w];
-                e.row(row, t, part);
+            for (int g = tid; g < nr; g += kBlock) {
+                const int row = r0 + g;
+                const int q1 = rp[row + 1] - p0;
+                AT sum = 0;
+                for (int q = rp[row] - p0; q < q1; ++q) sum += prod[q];
+                e.row(row, sum, part);
             }
-            __syncthreads();   // sh is reused by the next tile
         }
+        __syncthreads();   // prod is reused by the next tile
     }
     e.end(part);
 }
@@ -259,9 +283,9 @@ template <typename VT, typename XT, typename AT, class Epi>
 int launch_spmv(const skk_csr &A, const VT *vals, const XT *x0, const XT *x1,
                 const skk_state *sel, const Epi &epi)
 {
-    if (A.rows == 0 || A.bins.total_blocks == 0) return 0;
-    hipLaunchKernelGGL((k_spmv<VT, XT, AT, Epi>), dim3(A.bins.grid), dim3(kBlock), 0,
-                       stream(), A.rp, A.ci, vals, x0, x1, sel, A.bins, epi);
+    if (A.ntiles == 0) return 0;
+    hipLaunchKernelGGL((k_spmv<VT, XT, AT, Epi>), dim3(A.grid), dim3(kBlock), 0,
+                       stream(), A.rp, A.ci, vals, x0, x1, sel, A.tile_row, A.ntiles, epi);
     return launch_status();
 }
 
@@ -1002,17 +1026,26 @@ template <typename T> int q_probe(const skk_pdhg &P, const Iter<T> &I)
     return launch_spmv<T, T, T>(P.Q, vals_of<T>(P.Q), I.X[0], I.X[1], P.st, eq);
 }
 
-// A x_cand with the probe epilogue (x_cand is X[cur ^ 1]), then the
-// single-workgroup decision.
-template <typename T> int probe(const skk_pdhg &P, const Iter<T> &I)
+// A x_cand with the probe epilogue (x_cand is X[cur ^ 1]).
+template <typename T> int probe_spmv(const skk_pdhg &P, const Iter<T> &I)
 {
     EpiProbe<T, T> pr{};
     pr.I = I; pr.st = P.st; pr.part = P.part; pr.stride = P.part_capacity;
-    int rc = launch_spmv<T, T, T>(P.A, vals_of<T>(P.A), I.X[0], I.X[1], P.st, pr);
-    if (rc) return rc;
+    return launch_spmv<T, T, T>(P.A, vals_of<T>(P.A), I.X[0], I.X[1], P.st, pr);
+}
+
+// The single-workgroup accept/reject decision.
+int probe_decide(const skk_pdhg &P)
+{
     hipLaunchKernelGGL(k_probe_finalize, dim3(1), dim3(kBlock), 0, stream(), P.st, P.part, P.part_capacity,
-                       P.AT.bins.grid, P.has_q ? P.Q.bins.grid : 0, P.A.bins.grid);
+                       P.AT.grid, P.has_q ? P.Q.grid : 0, P.A.grid);
     return launch_status();
+}
+
+template <typename T> int probe(const skk_pdhg &P, const Iter<T> &I)
+{
+    const int rc = probe_spmv<T>(P, I);
+    return rc ? rc : probe_decide(P);
 }
 
 template <typename T> int adaptive_iter(const skk_pdhg &P)
@@ -1049,9 +1082,10 @@ template <typename T> int adaptive_part(const skk_pdhg &P, int part)
         ep.stride = P.part_capacity;
         return launch_spmv<T, T, T>(P.AT, vals_of<T>(P.AT), I.Y[0], I.Y[1], P.st, ep);
     }
+    if (part == 3) return probe_decide(P);
     int rc;
     if (P.has_q && (rc = q_probe<T>(P, I))) return rc;
-    return probe<T>(P, I);
+    return probe_spmv<T>(P, I);
 }
 
 template <typename T> int fixed_iter(const skk_pdhg &P)
@@ -1432,7 +1466,7 @@ int skk_power_norm(const skk_pdhg *P, double *v, double *w, double *v2, int iter
         EpiNormSq ns{};
         ns.y = v2; ns.part = P->part + (size_t)R_POWER * P->part_capacity;
         if ((rc = launch_spmv<double, double, double>(P->AT, P->AT.vd, w, w, nullptr, ns))) return rc;
-        if ((rc = finalize(ns.part, P->part_capacity, P->AT.bins.grid, 1, 0, out))) return rc;
+        if ((rc = finalize(ns.part, P->part_capacity, P->AT.grid, 1, 0, out))) return rc;
         hipLaunchKernelGGL(k_normalize_from, dim3(grid_for(n, P->elem_grid)), dim3(kBlock), 0,
                            stream(), v, v2, out, n);
         if ((rc = launch_status())) return rc;

@@ -294,18 +294,11 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
         }
     }
     ATp.p[n] = q;
-    /* rows are now sorted by class: the bins are contiguous slot ranges */
-    if (skg_plan_bins(Ap.p, m, &g_skg_tuning, NULL, &bA) ||
-        skg_plan_bins(ATp.p, n, &g_skg_tuning, NULL, &bAT)) { skg_fail("out of host memory"); goto done; }
-    if ((rc = dcsr_upload(&E->A, &Ap, &bA, NULL, E->mixed)) ||
-        (rc = dcsr_upload(&E->AT, &ATp, &bAT, NULL, E->mixed))) goto done;
+    if ((rc = dcsr_upload(&E->A, &Ap, E->mixed)) || (rc = dcsr_upload(&E->AT, &ATp, E->mixed))) goto done;
     if (E->has_q) {
-        /* Q' = C Q C in slot order on both sides; its rows are not sorted by
-           their own length, so its bins address rows through a list */
+        /* Q' = C Q C in slot order on both sides */
         hcsr Qp;
-        skk_bins bQ;
-        int *qorder = (int *)malloc(sizeof(int) * (size_t)(n + 1));
-        if (!qorder || hcsr_alloc(&Qp, n, n, qrp[n])) { free(qorder); skg_fail("out of host memory"); goto done; }
+        if (hcsr_alloc(&Qp, n, n, qrp[n])) { skg_fail("out of host memory"); goto done; }
         q = 0;
         for (k = 0; k < n; ++k) {
             const int r = E->perm_c[k];
@@ -316,9 +309,7 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
             }
         }
         Qp.p[n] = q;
-        rc = skg_plan_bins(Qp.p, n, &g_skg_tuning, qorder, &bQ) ? skg_fail("out of host memory")
-                                                                : dcsr_upload(&E->Q, &Qp, &bQ, qorder, E->mixed);
-        free(qorder);
+        rc = dcsr_upload(&E->Q, &Qp, E->mixed);
         hcsr_free(&Qp);
         if (rc) goto done;
     }
@@ -334,7 +325,7 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
         Q->has_q = E->has_q;
         if (E->has_q) {
             Q->Q = E->Q.k;
-            if (E->Q.k.bins.total_blocks > 0 && (rc = dalloc(E, (void **)&Q->qx_d, 8 * nn))) goto done;
+            if (E->Q.k.ntiles > 0 && (rc = dalloc(E, (void **)&Q->qx_d, 8 * nn))) goto done;
         }
         if ((rc = dalloc(E, (void **)&cd, 8 * nn)) || (rc = dalloc(E, (void **)&ld, 8 * nn)) ||
             (rc = dalloc(E, (void **)&ud, 8 * nn)) || (rc = dalloc(E, (void **)&rld, 8 * mm)) ||
@@ -353,9 +344,9 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
         if (eg > 2048) eg = 2048;
         Q->elem_grid = eg;
         cap = eg;
-        if (bA.grid > cap) cap = bA.grid;
-        if (bAT.grid > cap) cap = bAT.grid;
-        if (E->has_q && E->Q.k.bins.grid > cap) cap = E->Q.k.bins.grid;
+        if (E->A.k.grid > cap) cap = E->A.k.grid;
+        if (E->AT.k.grid > cap) cap = E->AT.k.grid;
+        if (E->has_q && E->Q.k.grid > cap) cap = E->Q.k.grid;
         Q->part_capacity = cap;
         if ((rc = dalloc(E, (void **)&Q->ax_d, 8 * mm)) || (rc = dalloc(E, (void **)&Q->aty_d, 8 * nn)) ||
             (rc = dalloc(E, (void **)&Q->st, sizeof(skk_state))) ||
@@ -610,26 +601,26 @@ int sk_gpu_pdlp_session_iterate(sk_gpu_pdlp_session *S, int iterations, double *
     prec = E->mixed ? SKK_F32 : SKK_F64;
     if (session_start(S, prec, NULL, NULL, &omega)) return -1;
     if (kernel_ms) {
-        const int parts = S->P.adaptive_step ? 3 : 1;
-        if ((rc = skk_timeline_reserve(4 * iterations + 4))) return skg_hip_fail("event timeline", rc);
+        const int parts = S->P.adaptive_step ? 4 : 1;
+        if ((rc = skk_timeline_reserve(5 * iterations + 5))) return skg_hip_fail("event timeline", rc);
         if ((rc = skk_sync())) return skg_hip_fail("sync", rc);
         for (k = 0; k < iterations; ++k) {
-            if ((rc = skk_timeline_mark(4 * k))) return skg_hip_fail("event record", rc);
+            if ((rc = skk_timeline_mark(5 * k))) return skg_hip_fail("event record", rc);
             for (part = 0; part < parts; ++part) {
                 rc = S->P.adaptive_step ? skk_pdhg_adaptive_part(&E->P, prec, part)
                                         : skk_pdhg_fixed_iter(&E->P, prec);
                 if (rc) return skg_hip_fail("PDHG launch", rc);
-                if ((rc = skk_timeline_mark(4 * k + part + 1))) return skg_hip_fail("event record", rc);
+                if ((rc = skk_timeline_mark(5 * k + part + 1))) return skg_hip_fail("event record", rc);
             }
         }
         if ((rc = skk_sync())) return skg_hip_fail("sync", rc);
-        for (part = 0; part < 4; ++part) kernel_ms[part] = 0.0;
+        for (part = 0; part < 5; ++part) kernel_ms[part] = 0.0;
         for (k = 0; k < iterations; ++k) {
             for (part = 0; part < parts; ++part)
-                kernel_ms[part] += skk_timeline_ms(4 * k + part, 4 * k + part + 1);
-            kernel_ms[3] += skk_timeline_ms(4 * k, 4 * k + parts);
+                kernel_ms[part] += skk_timeline_ms(5 * k + part, 5 * k + part + 1);
+            kernel_ms[4] += skk_timeline_ms(5 * k, 5 * k + parts);
         }
-        for (part = 0; part < 4; ++part) kernel_ms[part] /= (iterations > 0 ? iterations : 1);
+        for (part = 0; part < 5; ++part) kernel_ms[part] /= (iterations > 0 ? iterations : 1);
         return 0;
     }
     for (k = 0; k < iterations; ++k) {
