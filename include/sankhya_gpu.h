@@ -282,6 +282,9 @@ typedef struct sk_gpu_pdlp_params {  /* src/cuda/GpuPdlp.hpp:19-67 PdlpParams */
     double ruiz_tol;         /* 1e-2                                        */
     int    use_graph;        /* capture each restart window in a hipGraph   */
     int    verbosity;
+    double primal_weight;    /* initial primal weight; 0: ||c||/||b|| of the
+                                scaled problem (no reference counterpart:
+                                warm starts pass the family's learned value) */
 } sk_gpu_pdlp_params;
 
 typedef struct sk_gpu_pdlp_stats {   /* src/cuda/GpuPdlp.hpp:69-92 PdlpStats */

@@ -883,7 +883,9 @@ int orc_pdlp_solve(const orc_csr *A, const double *c, const double *l, const dou
     eta_global = 0.9 / norm;
     eta_min = P->eta_min_factor / norm;
     eta_max = P->eta_max_factor / norm;
-    if (cost_norm > 0.0 && bound_norm > 0.0)
+    if (P->primal_weight > 0.0)   /* an addition: an explicit starting weight */
+        omega = fmin(1e6, fmax(1e-6, P->primal_weight));
+    else if (cost_norm > 0.0 && bound_norm > 0.0)
         omega = fmin(1e6, fmax(1e-6, cost_norm / bound_norm));
     /* warm start (an addition: the reference always starts at zero) */
     if (x0) for (j = 0; j < n; ++j) x[j] = xb[j] = xr[j] = x0[j];

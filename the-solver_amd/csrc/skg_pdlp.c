@@ -561,7 +561,9 @@ static int session_start(sk_gpu_pdlp_session *S, int prec, const double *x0, con
     const int rows = E->m, cols = E->n;
     double omega = 1.0;
     int k, rc = 0;
-    if (E->cost_norm > 0.0 && E->bound_norm > 0.0)
+    if (S->P.primal_weight > 0.0)
+        omega = fmin(1e6, fmax(1e-6, S->P.primal_weight));
+    else if (E->cost_norm > 0.0 && E->bound_norm > 0.0)
         omega = fmin(1e6, fmax(1e-6, E->cost_norm / E->bound_norm));
     if (x0 || y0) {
         double *tx = (double *)malloc(sizeof(double) * (size_t)(cols + 1));
