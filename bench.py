@@ -99,6 +99,20 @@ class Session:
             self.h = C.c_void_p()
 
 
+def pmc_traffic(name="spmv_lp_1Mx2M_pmc.md"):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes (calibrated against a copy kernel), from the committed summary."""
+    path = ROOT / "profiles" / name
+    out = {}
+    if path.exists():
+        block = path.read_text().split("```traffic", 1)
+        if len(block) == 2:
+            for line in block[1].split("```", 1)[0].strip().splitlines():
+                k, v = line.split()
+                out[k] = int(v)
+    return out
+
+
 def kernel_bytes(m, n, nnz, s, s_v, accept_rate):
     """Algorithmic HBM bytes of the three launches of one adaptive iteration
     (s = iterate element size, s_v = matrix value size, int32 indices)."""
@@ -179,6 +193,16 @@ def workload_lp1(args):
     bytes_dom, ms_dom = kernels[dom]
     achieved = bytes_dom / (ms_dom * 1e-3) / 1e9
     spmv_gbs = {k: kernels[k][0] / (kernels[k][1] * 1e-3) / 1e9 for k in kernels}
+    # the products gather one 4-byte vector element per entry at random
+    # positions: their second ceiling is the device's random-gather rate
+    # for a table of the gathered vector's size
+    gather = {}
+    for name, table in (("AT_spmv_primal", m), ("A_spmv_probe", n)):
+        rate = C.c_double()
+        if lib.sk_gpu_gather_rate(table, nnz, 10, C.byref(rate)) == 0 and rate.value > 0:
+            got = nnz / (kernels[name][1] * 1e-3)
+            gather[name] = {"gathers": nnz, "table_bytes": 4 * table, "achieved_G_per_s": round(got / 1e9, 1),
+                            "probe_G_per_s": round(rate.value / 1e9, 1), "frac": round(got / rate.value, 3)}
     line = {
         "metric": METRIC,
         "value": round(mix["iters_per_s"], 2),
@@ -195,8 +219,10 @@ def workload_lp1(args):
         "fp64_ms_per_step": round(f64["ms_per_iter"], 4),
         "speedup_iters_vs_fp64": round(mix["iters_per_s"] / f64["iters_per_s"], 3),
         "spmv_gbs": {k: round(val, 1) for k, val in spmv_gbs.items()},
+        "gather_roofline": gather,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": pmc_traffic().get(dom) if not args.quick else None,
                      "algorithmic_bytes": int(bytes_dom), "mean_ms": round(ms_dom, 5)},
     }
     if not args.skip_solve:

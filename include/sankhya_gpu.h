@@ -144,6 +144,16 @@ void      sk_gpu_solution_free(sk_solution *s);                 /* sk_core.c:175
 void      sk_gpu_options_default(sk_options *o);                /* sk_core.c:184-207 sk_options_default*/
 const char *sk_gpu_result_name(sk_result r);                    /* sk_core.c:31-43   sk_result_name    */
 const char *sk_gpu_status_name(sk_status s);                    /* sk_core.c:45-58   sk_status_name    */
+/* MPS / QPS reader (c/src/sk_core.c:318-378 sk_read_mps over
+ * c/src/sankhya_mps.c:150-331): free format with a fixed-format fallback;
+ * ROWS N/L/G/E (later N rows are free rows), COLUMNS with MARKER
+ * INTORG/INTEND, RHS (objective RHS -> objshift = -value), RANGES, BOUNDS
+ * LO/UP/FX/FR/MI/PL/BV/LI/UI, OBJSENSE MAX (normalised to minimisation),
+ * QUADOBJ/QMATRIX/QSECTION into the full symmetric Q.  Infinite or
+ * |bound| >= 1e29 becomes +-SK_INFINITY.  The model owns malloc'd arrays
+ * and names: release with sk_gpu_model_free.  SK_ERR_IO when the file
+ * cannot be read as MPS, SK_ERR_STRUCTURE for a bad quadratic section. */
+sk_status sk_gpu_read_mps(const char *path, sk_model *m);
 
 /* ------------------------------------------------------------------------
  * Device-resident sparse matrix.
@@ -191,6 +201,14 @@ int sk_gpu_axpy_f64(int n, double alpha, const double *x, double *y);
 
 /* cuda/include/sankhya_cuda.h:167: thread-local text of the last failure. */
 const char *sk_gpu_last_error(void);
+
+/* Device diagnostic, no reference counterpart: the random-gather issue
+ * rate of the current device -- `count` 4-byte gathers at uniformly random
+ * positions of a table of `table_elems` floats, streamed like a CSR
+ * product's (value, index) pairs.  It bounds every sparse product here,
+ * whose gathered vectors are indexed at random.  *gathers_per_second is
+ * the mean over `reps` timed launches.  0 or -1. */
+int sk_gpu_gather_rate(int table_elems, int count, int reps, double *gathers_per_second);
 
 /* ------------------------------------------------------------------------
  * Plain PDHG with host-visible checkpoints (the reference's sankhya_cuda_*

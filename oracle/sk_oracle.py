@@ -76,8 +76,45 @@ def load_ref():
         _decl(lib, "sk_solution_init", None, [S])
         _decl(lib, "sk_solution_free", None, [S])
         _decl(lib, "sk_model_validate", C.c_int, [M])
+        _decl(lib, "sk_read_mps", C.c_int, [C.c_char_p, M])
+        _decl(lib, "sk_model_init", None, [M])
+        _decl(lib, "sk_model_free", None, [M])
         _ref = lib
     return _ref
+
+
+def model_arrays(m):
+    """Every field of an sk_model (ctypes SkModel) as numpy arrays / lists."""
+    nnz = m.A.p[m.ncol] if m.ncol > 0 else 0
+    out = {"nrow": m.nrow, "ncol": m.ncol, "objshift": m.objshift, "objsense": m.objsense,
+           "Ap": np.array(m.A.p[:m.ncol + 1], np.int32), "Ai": np.array(m.A.i[:nnz], np.int32),
+           "Ax": np.array(m.A.x[:nnz], np.float64)}
+    for k in ("c", "rlow", "rupp", "clow", "cupp"):
+        n = m.ncol if k in ("c", "clow", "cupp") else m.nrow
+        out[k] = np.array(getattr(m, k)[:n], np.float64)
+    out["vartype"] = np.array(m.vartype[:m.ncol], np.uint8)
+    if m.Q:
+        q = m.Q.contents
+        qn = q.p[q.ncol]
+        out.update(Qp=np.array(q.p[:q.ncol + 1], np.int32), Qi=np.array(q.i[:qn], np.int32),
+                   Qx=np.array(q.x[:qn], np.float64))
+    out["rowname"] = np.array([m.rowname[k].decode() for k in range(m.nrow)] if m.rowname else [])
+    out["colname"] = np.array([m.colname[k].decode() for k in range(m.ncol)] if m.colname else [])
+    return out
+
+
+def ref_read_mps(path):
+    """(status, arrays) of the reference C core's sk_read_mps, or None
+    where oracle/_ref was not built."""
+    ref = load_ref()
+    if ref is None:
+        return None
+    m = sg.SkModel()
+    ref.sk_model_init(C.byref(m))
+    st = ref.sk_read_mps(str(path).encode(), C.byref(m))
+    arrays = model_arrays(m) if st == 0 else None
+    ref.sk_model_free(C.byref(m))
+    return st, arrays
 
 
 class Csr:

@@ -129,3 +129,13 @@ def test_axpy():
     assert lib.sk_gpu_axpy_f64(1000, 2.5, sg.dptr(x), sg.dptr(y)) == 0
     assert np.array_equal(y, np.fma(2.5, np.arange(1000.0), 1.0) if hasattr(np, "fma") else 2.5 * np.arange(1000.0) + 1.0)
     assert lib.sk_gpu_axpy_f64(-1, 1.0, sg.dptr(x), sg.dptr(y)) != 0
+
+
+def test_gather_rate_probe():
+    """The device's random-gather rate (the second ceiling of the sparse
+    products) is measured and lands in a physically sensible range."""
+    lib = sg.load_gpu()
+    rate = C.c_double()
+    assert lib.sk_gpu_gather_rate(1 << 20, 1 << 22, 3, C.byref(rate)) == 0, sg.last_error()
+    assert 1e9 < rate.value < 1e13
+    assert lib.sk_gpu_gather_rate(0, 10, 1, C.byref(rate)) == -1

@@ -665,3 +665,14 @@ sk_status sk_gpu_solve(const sk_model *m, const sk_options *o, sk_gpu_options *g
     s->dual_bound = s->objective;
     return SK_OK;
 }
+
+int sk_gpu_gather_rate(int table_elems, int count, int reps, double *gathers_per_second)
+{
+    int rc;
+    skg_clear_error();
+    if (table_elems <= 0 || count <= 0 || reps <= 0 || !gathers_per_second)
+        return skg_fail("invalid gather probe arguments");
+    if (skg_have_device()) return -1;
+    rc = skk_gather_probe(table_elems, count, reps, gathers_per_second);
+    return rc ? skg_hip_fail("gather probe", rc) : 0;
+}

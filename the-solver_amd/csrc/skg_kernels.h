@@ -132,6 +132,9 @@ int skk_pdhg_adaptive_part(const skk_pdhg *P, int prec, int part);
 int    skk_timeline_reserve(int count);
 int    skk_timeline_mark(int k);
 double skk_timeline_ms(int a, int b);
+/* Random-gather issue rate: count 4-byte gathers from a table of `table`
+ * floats, mean of `reps` launches; *rate in gathers per second. */
+int skk_gather_probe(int table, int count, int reps, double *rate);
 int skk_pdhg_flush(const skk_pdhg *P, int prec);
 /* which 0: current iterate, 1: running average (formed here). */
 int skk_pdhg_eval(const skk_pdhg *P, int prec, int which, int adaptive, double fixed_terms);

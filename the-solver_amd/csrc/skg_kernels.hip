@@ -1168,6 +1168,41 @@ template <typename T> int restart(const skk_pdhg &P, int use_avg, int adaptive)
 
 // ======================================================== extern "C" layer
 
+
+// --------------------------------------------------- gather-rate probe
+
+// Index stream for the gather probe: a SplitMix64 hash of the position,
+// reduced to [0, table) -- uniform, reproducible, no host transfer.
+__global__ void k_probe_fill(int *idx, float *val, int count, int table, float *tab)
+{
+    const int stride = gridDim.x * blockDim.x;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < count; p += stride) {
+        uint64_t z = (uint64_t)p * 0x9E3779B97F4A7C15ULL + 0x632BE59BD9B4E019ULL;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        idx[p] = (int)(z % (uint64_t)table);
+        val[p] = 1.0f;
+        if (p < table) tab[p] = 0.5f;
+    }
+}
+
+// The access pattern of a CSR product without its row bookkeeping: each
+// lane streams 8 (value, index) pairs with coalesced non-temporal loads and
+// gathers table[index] for each.
+__global__ __launch_bounds__(kBlock) void k_probe_gather(const int *__restrict__ idx, const float *__restrict__ val,
+                                                         const float *__restrict__ tab, float *out, int count)
+{
+    const int base = blockIdx.x * kBlock * 8 + threadIdx.x;
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int p = base + k * kBlock;
+        if (p < count) s += ntload(val + p) * tab[ntload(idx + p)];
+    }
+    out[blockIdx.x * kBlock + threadIdx.x] = s;
+}
+
 extern "C" {
 
 int skk_device_count(int *count)
@@ -1544,6 +1579,40 @@ int skk_legacy_check(const skk_legacy *L, int have_prev)
     hipLaunchKernelGGL(k_legacy_check, dim3(L->grid), dim3(kBlock), 0, stream(), K);
     if ((rc = launch_status())) return rc;
     return finalize(L->part, L->grid, L->grid, 2, 6, L->out);
+}
+
+/* Random-gather issue rate of this device: `count` 4-byte gathers from a
+ * table of `table` floats (uniform random indices), mean of `reps` timed
+ * launches after one warm-up.  Returns gathers per second in *rate. */
+int skk_gather_probe(int table, int count, int reps, double *rate)
+{
+    if (table <= 0 || count <= 0 || reps <= 0 || !rate) return -1;
+    int *idx = nullptr;
+    float *val = nullptr, *tab = nullptr, *out = nullptr;
+    const int grid = (count + kBlock * 8 - 1) / (kBlock * 8);
+    int rc = skk_malloc((void **)&idx, sizeof(int) * (size_t)count);
+    if (!rc) rc = skk_malloc((void **)&val, sizeof(float) * (size_t)count);
+    if (!rc) rc = skk_malloc((void **)&tab, sizeof(float) * (size_t)table);
+    if (!rc) rc = skk_malloc((void **)&out, sizeof(float) * (size_t)grid * kBlock);
+    if (!rc) {
+        hipLaunchKernelGGL(k_probe_fill, dim3(1024), dim3(kBlock), 0, stream(), idx, val, count, table, tab);
+        hipLaunchKernelGGL(k_probe_gather, dim3(grid), dim3(kBlock), 0, stream(), idx, val, tab, out, count);
+        rc = launch_status();
+    }
+    if (!rc) rc = skk_timeline_reserve(2);
+    if (!rc) rc = skk_timeline_mark(0);
+    for (int r = 0; r < reps && !rc; ++r) {
+        hipLaunchKernelGGL(k_probe_gather, dim3(grid), dim3(kBlock), 0, stream(), idx, val, tab, out, count);
+        rc = launch_status();
+    }
+    if (!rc) rc = skk_timeline_mark(1);
+    if (!rc) rc = skk_sync();
+    if (!rc) {
+        const double ms = skk_timeline_ms(0, 1) / reps;
+        *rate = ms > 0.0 ? (double)count / (ms * 1e-3) : 0.0;
+    }
+    skk_free(idx); skk_free(val); skk_free(tab); skk_free(out);
+    return rc;
 }
 
 }  // extern "C"

@@ -266,6 +266,9 @@ def load_gpu():
         _declare(lib, "sk_gpu_pdlp_session_solve", C.c_int,
                  [vp, c_double_p, c_double_p, c_double_p, c_double_p, C.POINTER(PdlpStats)])
         _declare(lib, "sk_gpu_pdlp_session_destroy", None, [vp])
+        _declare(lib, "sk_gpu_gather_rate", C.c_int, [C.c_int, C.c_int, C.c_int, c_double_p])
+        _declare(lib, "sk_gpu_read_mps", C.c_int, [C.c_char_p, C.POINTER(SkModel)])
+        _declare(lib, "sk_gpu_model_free", None, [C.POINTER(SkModel)])
         _declare(lib, "sk_gpu_batch_options_default", None, [C.POINTER(BatchOptions)])
         _declare(lib, "sk_gpu_batch_solve", C.c_int,
                  [C.c_int, M, C.POINTER(BatchOptions), c_double_p, c_double_p,
@@ -354,10 +357,24 @@ class Model:
         self._generated = True
         return self
 
+    @classmethod
+    def read_mps(cls, path):
+        """An MPS / QPS file through the library's reader (sk_gpu_read_mps)."""
+        self = cls()
+        lib = load_gpu()
+        rc = lib.sk_gpu_read_mps(str(path).encode(), C.byref(self.m))
+        if rc != 0:
+            raise RuntimeError(f"sk_gpu_read_mps({path}) = {rc}: {last_error()}")
+        self._owned_by_lib = True
+        return self
+
     def __del__(self):
         if self._generated and _work is not None:
             _work.skw_model_free(C.byref(self.m))
             self._generated = False
+        if getattr(self, "_owned_by_lib", False) and _gpu is not None:
+            _gpu.sk_gpu_model_free(C.byref(self.m))
+            self._owned_by_lib = False
 
     # numpy views ---------------------------------------------------
     @property
