@@ -10,7 +10,8 @@ import mlp_oracle as mo
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("shape,B", [((16, 64, 100), 32), ((16, 512, 700), 64), ((5, 40, 3000), 70)])
+@pytest.mark.parametrize("shape,B", [((16, 64, 100), 32), ((16, 512, 700), 64), ((5, 40, 3000), 70),
+                                     ((16, 512, 1000), 200), ((16, 128, 20000), 64)])
 def test_gradients_match_oracle(shape, B):
     n_in, hid, n_out = shape
     rng = np.random.default_rng(1)

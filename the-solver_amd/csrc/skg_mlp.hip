@@ -5,10 +5,10 @@
 // (c/src/sk_solve.c:871, cuda/src/sankhya_cuda_lp.cu:205-213).
 //
 // GEMM: C[M x N] = op(A)[M x K] * op(B)[K x N] with op = identity or
-// transpose, 64 x 64 x 32 tiles, 256 threads = 4 wavefronts each owning a
-// 32 x 32 quadrant computed with v_mfma_f32_32x32x8bf16_1k.  Both LDS tiles
+// transpose, 128 x 128 x 32 tiles, 256 threads = 4 wavefronts each owning a
+// 64 x 64 quadrant computed with v_mfma_f32_32x32x16_bf16.  Both LDS tiles
 // are stored K-contiguous ([row][k] for A, [col][k] for B) so every MFMA
-// operand is one 8-byte LDS read.  Long-K products (the gradient through
+// operand is one 16-byte LDS read.  Long-K products (the gradient through
 // the 70,000-wide output layer) split K over blockIdx.z and finish in a
 // reduction kernel that applies the same epilogue.
 #include <hip/hip_runtime.h>
@@ -22,10 +22,10 @@ namespace {
 
 hipStream_t mstream() { return static_cast<hipStream_t>(skk_stream()); }
 
-typedef short short4_t __attribute__((ext_vector_type(4)));
-typedef float float16_t __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int BM = 64, BN = 64, BK = 32, PAD = 8, THREADS = 256;
+constexpr int BM = 128, BN = 128, BK = 32, PAD = 8, THREADS = 256;
 
 __device__ __forceinline__ uint16_t f2bf(float f)
 {
@@ -56,16 +56,75 @@ __device__ __forceinline__ void epilogue(const Epi &e, int r, int c, float v)
     if (e.c16) e.c16[(size_t)r * e.ldc + c] = f2bf(v);
 }
 
-// op(A)[i][k]: TA ? A[k*lda + i] : A[i*lda + k]; op(B)[k][j]: TB ? B[j*ldb + k] : B[k*ldb + j].
-// With kchunk > 0 the z-slice z covers k in [z*kchunk, (z+1)*kchunk) and the
-// raw sums go to part[z][M][N] instead of the epilogue.
+// One operand tile of ROWS x BK elements into LDS as [row][k] (K-contiguous,
+// the MFMA operand layout).  Source element (r, k) lives at
+//   src[r * ld + k]  when KC (K-contiguous source), else src[k * ld + r].
+// Loads are 16-byte vectors of 8 elements where the whole vector is in range
+// and aligned (vec), otherwise element by element with zero fill.  The
+// values travel through registers so the next tile's loads overlap the
+// current tile's MFMAs.
+template <bool KC> struct TileLoader {
+    static constexpr int kVec = (BM * BK) / (8 * THREADS);   // vectors per thread (2)
+    uint16_t v[kVec][8];
+    __device__ void load(const uint16_t *__restrict__ src, int ld, int r0, int rows, int k0, int kend, bool vec)
+    {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int q = 0; q < kVec; ++q) {
+            // KC: vector along k of one row; else: vector along rows of one k
+            const int r = KC ? r0 + (t >> 2) + q * (THREADS / 4) : r0 + ((t & 15) << 3);
+            const int k = KC ? k0 + ((t & 3) << 3) : k0 + (t >> 4) + q * (THREADS / 16);
+            const bool full = KC ? (r < rows && k + 8 <= kend) : (k < kend && r + 8 <= rows);
+            if (vec && full) {
+                const uint4 w = *reinterpret_cast<const uint4 *>(KC ? src + (size_t)r * ld + k
+                                                                     : src + (size_t)k * ld + r);
+                const uint16_t *h = reinterpret_cast<const uint16_t *>(&w);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) v[q][e] = h[e];
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int rr = KC ? r : r + e, kk = KC ? k + e : k;
+                    v[q][e] = (rr < rows && kk < kend) ? src[KC ? (size_t)rr * ld + kk : (size_t)kk * ld + rr]
+                                                       : (uint16_t)0;
+                }
+            }
+        }
+    }
+    __device__ void store(uint16_t (*lds)[BK + PAD]) const
+    {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int q = 0; q < kVec; ++q) {
+            if (KC) {
+                const int r = (t >> 2) + q * (THREADS / 4), k = (t & 3) << 3;
+                uint4 w;
+                uint16_t *h = reinterpret_cast<uint16_t *>(&w);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) h[e] = v[q][e];
+                *reinterpret_cast<uint4 *>(&lds[r][k]) = w;
+            } else {
+                const int r = (t & 15) << 3, k = (t >> 4) + q * (THREADS / 16);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) lds[r + e][k] = v[q][e];
+            }
+        }
+    }
+};
+
+// C[M x N] = op(A)[M x K] op(B)[K x N], op(A)[i][k] = TA ? A[k*lda + i] : A[i*lda + k],
+// op(B)[k][j] = TB ? B[j*ldb + k] : B[k*ldb + j].  128 x 128 x 32 tiles,
+// four wavefronts of 64 x 64, each a 2 x 2 grid of v_mfma_f32_32x32x16_bf16
+// (the CDNA4 double-rate BF16 MFMA); LDS double-buffered with one barrier
+// per K step.  With kchunk > 0 the z-slice covers k in [z*kchunk,
+// (z+1)*kchunk) and writes raw sums to part[z][M][N].
 template <bool TA, bool TB>
 __global__ __launch_bounds__(THREADS) void k_gemm(int M, int N, int K, const uint16_t *__restrict__ A,
                                                   int lda, const uint16_t *__restrict__ B, int ldb,
-                                                  Epi e, float *part, int kchunk)
+                                                  Epi e, float *part, int kchunk, int vec)
 {
-    __shared__ uint16_t As[BM][BK + PAD];
-    __shared__ uint16_t Bs[BN][BK + PAD];
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][BM][BK + PAD];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][BN][BK + PAD];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int row0 = blockIdx.y * BM, col0 = blockIdx.x * BN;
@@ -74,59 +133,64 @@ __global__ __launch_bounds__(THREADS) void k_gemm(int M, int N, int K, const uin
         kbeg = blockIdx.z * kchunk;
         kend = min(K, kbeg + kchunk);
     }
-    float16_t acc;
-    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-    for (int k0 = kbeg; k0 < kend; k0 += BK) {
-        // ---- A tile: BM x BK, 8 elements per thread
-        if (!TA) {
-            const int r = t >> 2, kk = (t & 3) * 8;
-            const int gi = row0 + r;
-            for (int q = 0; q < 8; ++q) {
-                const int gk = k0 + kk + q;
-                As[r][kk + q] = (gi < M && gk < kend) ? A[(size_t)gi * lda + gk] : (uint16_t)0;
-            }
-        } else {
-            const int kk = t >> 3, i0 = (t & 7) * 8;
-            const int gk = k0 + kk;
-            for (int q = 0; q < 8; ++q) {
-                const int gi = row0 + i0 + q;
-                As[i0 + q][kk] = (gi < M && gk < kend) ? A[(size_t)gk * lda + gi] : (uint16_t)0;
-            }
-        }
-        // ---- B tile: BK x BN stored [n][k]
-        if (!TB) {
-            const int kk = t >> 3, j0 = (t & 7) * 8;
-            const int gk = k0 + kk;
-            for (int q = 0; q < 8; ++q) {
-                const int gj = col0 + j0 + q;
-                Bs[j0 + q][kk] = (gj < N && gk < kend) ? B[(size_t)gk * ldb + gj] : (uint16_t)0;
-            }
-        } else {
-            const int j = t >> 2, kk = (t & 3) * 8;
-            const int gj = col0 + j;
-            for (int q = 0; q < 8; ++q) {
-                const int gk = k0 + kk + q;
-                Bs[j][kk + q] = (gj < N && gk < kend) ? B[(size_t)gj * ldb + gk] : (uint16_t)0;
-            }
-        }
-        __syncthreads();
+    f32x16 acc[2][2];
 #pragma unroll
-        for (int ks = 0; ks < BK; ks += 8) {
-            const int kq = ks + (lane >> 5) * 4;
-            const short4_t a = *reinterpret_cast<const short4_t *>(&As[wm * 32 + (lane & 31)][kq]);
-            const short4_t b = *reinterpret_cast<const short4_t *>(&Bs[wn * 32 + (lane & 31)][kq]);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[bi][bj][r] = 0.0f;
+    TileLoader<!TA> la;   // A is K-contiguous unless transposed
+    TileLoader<TB> lb;    // B is K-contiguous only when transposed
+    const bool v = vec != 0;
+    la.load(A, lda, row0, M, kbeg, kend, v);
+    lb.load(B, ldb, col0, N, kbeg, kend, v);
+    la.store(As[0]);
+    lb.store(Bs[0]);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        const bool more = k0 + BK < kend;
+        if (more) {
+            la.load(A, lda, row0, M, k0 + BK, kend, v);
+            lb.load(B, ldb, col0, N, k0 + BK, kend, v);
+        }
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 16) {
+            const int kq = ks + ((lane >> 5) << 3);
+            bf16x8 a[2], b[2];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                a[q] = *reinterpret_cast<const bf16x8 *>(&As[buf][wm * 64 + q * 32 + (lane & 31)][kq]);
+                b[q] = *reinterpret_cast<const bf16x8 *>(&Bs[buf][wn * 64 + q * 32 + (lane & 31)][kq]);
+            }
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj)
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[bi], b[bj], acc[bi][bj], 0, 0, 0);
+        }
+        if (more) {
+            la.store(As[buf ^ 1]);
+            lb.store(Bs[buf ^ 1]);
         }
         __syncthreads();
+        buf ^= 1;
     }
-    // D[i][j]: j = lane % 32, i = 8 * (r / 4) + 4 * (lane / 32) + r % 4
-    const int j = col0 + wn * 32 + (lane & 31);
-    for (int r = 0; r < 16; ++r) {
-        const int i = row0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        if (i >= M || j >= N) continue;
-        if (kchunk > 0) part[((size_t)blockIdx.z * M + i) * N + j] = acc[r];
-        else epilogue(e, i, j, acc[r]);
-    }
+    // D[i][j] of one 32 x 32 block: j = lane % 32, i = 8 * (r / 4) + 4 * (lane / 32) + r % 4
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            const int j = col0 + wn * 64 + bj * 32 + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = row0 + wm * 64 + bi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (i >= M || j >= N) continue;
+                if (kchunk > 0) part[((size_t)blockIdx.z * M + i) * N + j] = acc[bi][bj][r];
+                else epilogue(e, i, j, acc[bi][bj][r]);
+            }
+        }
 }
 
 __global__ void k_splitk_reduce(int M, int N, int splits, const float *part, Epi e)
@@ -254,7 +318,9 @@ int skm_gemm(int M, int N, int K, const uint16_t *A, int lda, int trans_a, const
     const int kchunk = splits > 1 ? ((K + splits - 1) / splits + BK - 1) / BK * BK : 0;
     const int zs = splits > 1 ? (K + kchunk - 1) / kchunk : 1;
     const dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, zs);
-#define GO(ta, tb) hipLaunchKernelGGL((k_gemm<ta, tb>), grid, dim3(THREADS), 0, mstream(), M, N, K, A, lda, B, ldb, e, splitk_buf, kchunk)
+    // 16-byte operand vectors need 16-byte aligned bases and leading dimensions
+    const int vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && lda % 8 == 0 && ldb % 8 == 0;
+#define GO(ta, tb) hipLaunchKernelGGL((k_gemm<ta, tb>), grid, dim3(THREADS), 0, mstream(), M, N, K, A, lda, B, ldb, e, splitk_buf, kchunk, vec)
     if (!trans_a && !trans_b) GO(false, false);
     else if (!trans_a && trans_b) GO(false, true);
     else if (trans_a && !trans_b) GO(true, false);
