@@ -303,6 +303,11 @@ typedef struct sk_gpu_pdlp_params {  /* src/cuda/GpuPdlp.hpp:19-67 PdlpParams */
     double primal_weight;    /* initial primal weight; 0: ||c||/||b|| of the
                                 scaled problem (no reference counterpart:
                                 warm starts pass the family's learned value) */
+    int    unfused;          /* 1: never fuse the step decision into the next
+                                dual prox (the result is identical either way;
+                                fusion saves a launch per iteration when the
+                                products have few workgroups)             */
+    int    pad;
 } sk_gpu_pdlp_params;
 
 typedef struct sk_gpu_pdlp_stats {   /* src/cuda/GpuPdlp.hpp:69-92 PdlpStats */

@@ -101,6 +101,21 @@ def test_graph_window_matches_direct_launches():
     assert np.array_equal(a[1], b[1])
 
 
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_fused_decision_matches_separate_launch(prec, use_graph):
+    """The step decision taken inside the next dual prox (every workgroup
+    reducing the probe partials itself) gives the same iterates, bit for bit,
+    as the one-workgroup decision launch."""
+    model = sg.Model.generated("skw_bounded_lp", 400, 900, 5, 17)
+    a = gpu_pdlp(model, eps_optimal=1e-7, precision=prec, use_graph=use_graph, unfused=1)
+    b = gpu_pdlp(model, eps_optimal=1e-7, precision=prec, use_graph=use_graph, unfused=0)
+    assert a[0] == 0 and b[0] == 0 and a[3].converged and b[3].converged
+    assert a[3].iterations == b[3].iterations and a[3].restarts == b[3].restarts
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    assert b[3].kernel_launches < a[3].kernel_launches
+
+
 def test_fixed_step_path():
     model = sg.Model.generated("skw_bounded_lp", 50, 100, 5, 102)
     rc, x, y, st = gpu_pdlp(model, eps_optimal=1e-6, adaptive_step=0)

@@ -111,6 +111,11 @@ typedef struct skk_pdhg {
     skk_iter it[2];                          /* [0] FP64, [1] FP32          */
     double *ax_d, *aty_d, *qx_d;             /* FP64 products for checks    */
     skk_state *st;
+    skk_state *snap;                         /* copy of st taken by the step
+                                                probe, read by the fused
+                                                decision of the next launch  */
+    int fuse_decision;                       /* decide inside the next dual
+                                                prox (few block partials)    */
     double *part;                            /* block partials              */
     unsigned *counters;                      /* self-resetting tickets      */
     double *eval_out;                        /* [2][8] residual sums        */
@@ -123,6 +128,11 @@ enum { SKK_F64 = 0, SKK_F32 = 1 };
 int skk_pdhg_init(const skk_pdhg *P, int prec, double eta0, double omega, double eta_min,
                   double eta_max, int warm_x, int warm_y);
 int skk_pdhg_adaptive_iter(const skk_pdhg *P, int prec);
+/* A run of `count` adaptive iterations ending with the step decision
+ * applied.  With P->fuse_decision the decision of iteration k is taken
+ * inside the dual prox of iteration k + 1 (three launches per iteration
+ * instead of four) and a closing one-workgroup decision follows the last. */
+int skk_pdhg_adaptive_run(const skk_pdhg *P, int prec, int count);
 int skk_pdhg_fixed_iter(const skk_pdhg *P, int prec);
 /* One of the three launches of an adaptive iteration: 0 dual prox,
  * 1 A^T y_cand + primal prox, 2 A x_cand + step decision. */

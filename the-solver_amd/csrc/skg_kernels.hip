@@ -449,6 +449,7 @@ template <typename T, typename AT> struct EpiProbe {
     static constexpr int kSlots = 2;
     Iter<T> I;
     const skk_state *st;
+    skk_state *snap;    // or null
     double *part;       // region base
     int stride;         // part_capacity
     const T *y, *yc, *ax;
@@ -473,43 +474,94 @@ template <typename T, typename AT> struct EpiProbe {
     {
         double v[2] = {p[0], p[1]};
         store_partials<2>(v, part + (size_t)R_DY * stride, stride);
+        // the state this launch ran with, for a decision fused into the
+        // next launch (st itself is rewritten there)
+        if (snap && blockIdx.x == 0 && threadIdx.x == 0) *snap = *st;
     }
 };
 
-// One workgroup after the A x_cand launch: combine the partials, take the
-// accept/reject decision and the next step size (PdlpKernels.cu:454-493)
-// and commit by rotating buffer indices.
-__global__ __launch_bounds__(kBlock) void k_probe_finalize(skk_state *st, const double *part, int stride,
-                                                           int dx_blocks, int dq_blocks, int dy_blocks)
+// Accept/reject decision and next step size (PdlpKernels.cu:454-493) from
+// the reduced probe sums, committed by rotating buffer indices: `out` is
+// `in` after the decision.
+__device__ void decide(const skk_state &in, double dx2, double dq, double dy2, double dya, skk_state &out)
 {
-    const double dx2 = combine(part + (size_t)R_DX * stride, dx_blocks, false);
-    const double dq = dq_blocks > 0 ? combine(part + (size_t)R_DQ * stride, dq_blocks, false) : 0.0;
-    const double dy2 = combine(part + (size_t)R_DY * stride, dy_blocks, false);
-    const double dya = combine(part + (size_t)R_INTER * stride, dy_blocks, false);
-    if (threadIdx.x != 0) return;
+    out = in;
     // with a Hessian the local bound also covers dx'Q dx (>= 0 for convex Q)
     const double inter = fabs(dya) + 0.5 * fmax(dq, 0.0);
-    const double omega = st->omega, eta = st->eta;
+    const double omega = in.omega, eta = in.eta;
     const double movement = 0.5 * (omega * dx2 + dy2 / omega);
-    const double eta_bar = (inter > 0.0 && movement > 0.0) ? movement / inter : st->eta_max;
+    const double eta_bar = (inter > 0.0 && movement > 0.0) ? movement / inter : in.eta_max;
     const bool accept = eta <= eta_bar;
-    const double k = st->updates + 1.0;
+    const double k = in.updates + 1.0;
     const double shrink = 1.0 - pow(k + 1.0, -0.3);
     const double grow = 1.0 + pow(k + 1.0, -0.6);
     double next = fmin(shrink * eta_bar, grow * eta);
     if (!(next > 0.0) || !isfinite(next)) next = eta;
-    next = fmin(fmax(next, st->eta_min), st->eta_max);
-    st->eta = next;
-    st->eta_bar = eta_bar;
-    st->dx2 = dx2; st->dy2 = dy2; st->inter = inter;
-    st->updates = k;
-    st->accepted = accept ? 1 : 0;
-    st->pending = accept ? 1 : 0;
+    next = fmin(fmax(next, in.eta_min), in.eta_max);
+    out.eta = next;
+    out.eta_bar = eta_bar;
+    out.dx2 = dx2; out.dy2 = dy2; out.inter = inter;
+    out.updates = k;
+    out.accepted = accept ? 1 : 0;
+    out.pending = accept ? 1 : 0;
     if (accept) {
-        st->accept_count += 1.0;
-        st->cur ^= 1;
-        const int a0 = st->a0, a1 = st->a1, a2 = st->a2;
-        st->a0 = a2; st->a1 = a0; st->a2 = a1;   // x_prev <- x, x <- cand
+        out.accept_count += 1.0;
+        out.cur ^= 1;
+        out.a0 = in.a2; out.a1 = in.a0; out.a2 = in.a1;   // x_prev <- x, x <- cand
+    }
+}
+
+// The decision's four sums, reduced by one workgroup in a fixed order.
+__device__ void probe_sums(const double *part, int stride, int dx_blocks, int dq_blocks, int dy_blocks,
+                           double &dx2, double &dq, double &dy2, double &dya)
+{
+    dx2 = combine(part + (size_t)R_DX * stride, dx_blocks, false);
+    dq = dq_blocks > 0 ? combine(part + (size_t)R_DQ * stride, dq_blocks, false) : 0.0;
+    dy2 = combine(part + (size_t)R_DY * stride, dy_blocks, false);
+    dya = combine(part + (size_t)R_INTER * stride, dy_blocks, false);
+}
+
+// One workgroup after the A x_cand launch: the decision, committed to st.
+__global__ __launch_bounds__(kBlock) void k_probe_finalize(skk_state *st, const double *part, int stride,
+                                                           int dx_blocks, int dq_blocks, int dy_blocks)
+{
+    double dx2, dq, dy2, dya;
+    probe_sums(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya);
+    if (threadIdx.x != 0) return;
+    skk_state next;
+    decide(*st, dx2, dq, dy2, dya, next);
+    *st = next;
+}
+
+// The previous iteration's decision (from the snapshot its step probe took
+// and the probe partials; every workgroup takes the same decision in the
+// same order, workgroup 0 commits it to st) followed by this iteration's
+// dual prox with the decided step and buffers.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dual_decide(Iter<T> I, Data<T> D, const skk_state *__restrict__ snap,
+                                                       skk_state *st, const double *part, int stride,
+                                                       int dx_blocks, int dq_blocks, int dy_blocks, int m)
+{
+    __shared__ skk_state s_next;
+    double dx2, dq, dy2, dya;
+    probe_sums(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya);
+    if (threadIdx.x == 0) {
+        decide(*snap, dx2, dq, dy2, dya, s_next);
+        if (blockIdx.x == 0) *st = s_next;
+    }
+    __syncthreads();
+    const int cur = s_next.cur, pending = s_next.pending;
+    const T sigma = static_cast<T>(s_next.eta * s_next.omega);
+    const T *__restrict__ y = I.y(cur);
+    const T *__restrict__ ax = I.ax(s_next.a0);
+    const T *__restrict__ axp = I.ax(s_next.a1);
+    T *__restrict__ yc = I.y(cur ^ 1);
+    const int stride_e = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride_e) {
+        const T yi = y[i];
+        const T v = yi + sigma * (T(2) * ax[i] - axp[i]);
+        yc[i] = v - sigma * clampv(v / sigma, D.rl[i], D.ru[i]);
+        if (pending) I.ysum[i] += yi;
     }
 }
 
@@ -1027,10 +1079,11 @@ template <typename T> int q_probe(const skk_pdhg &P, const Iter<T> &I)
 }
 
 // A x_cand with the probe epilogue (x_cand is X[cur ^ 1]).
-template <typename T> int probe_spmv(const skk_pdhg &P, const Iter<T> &I)
+template <typename T> int probe_spmv(const skk_pdhg &P, const Iter<T> &I, bool snapshot = false)
 {
     EpiProbe<T, T> pr{};
     pr.I = I; pr.st = P.st; pr.part = P.part; pr.stride = P.part_capacity;
+    pr.snap = snapshot ? P.snap : nullptr;
     return launch_spmv<T, T, T>(P.A, vals_of<T>(P.A), I.X[0], I.X[1], P.st, pr);
 }
 
@@ -1064,6 +1117,39 @@ template <typename T> int adaptive_iter(const skk_pdhg &P)
     if ((rc = launch_spmv<T, T, T>(P.AT, vals_of<T>(P.AT), I.Y[0], I.Y[1], P.st, ep))) return rc;
     if (P.has_q && (rc = q_probe<T>(P, I))) return rc;
     return probe<T>(P, I);
+}
+
+// count iterations; with the fused decision: dual (first) or dual+decision,
+// A^T + primal, [Q], A + probe (snapshot) per iteration, one closing decision.
+template <typename T> int adaptive_run(const skk_pdhg &P, int count)
+{
+    if (!P.fuse_decision || !P.snap) {
+        for (int k = 0; k < count; ++k) {
+            const int rc = adaptive_iter<T>(P);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    const Iter<T> I(P.it[prec_index<T>()]);
+    const Data<T> D = data_of<T>(P);
+    int rc;
+    for (int k = 0; k < count; ++k) {
+        if (k == 0) {
+            hipLaunchKernelGGL(k_dual_adaptive<T>, dim3(P.elem_grid), dim3(kBlock), 0, stream(), I, D, P.st, P.m);
+        } else {
+            hipLaunchKernelGGL(k_dual_decide<T>, dim3(P.elem_grid), dim3(kBlock), 0, stream(), I, D, P.snap, P.st,
+                               P.part, P.part_capacity, P.AT.grid, P.has_q ? P.Q.grid : 0, P.A.grid, P.m);
+        }
+        if ((rc = launch_status())) return rc;
+        EpiPrimalAdaptive<T, T> ep{};
+        ep.I = I; ep.D = D; ep.st = P.st;
+        ep.part = P.part + (size_t)R_DX * P.part_capacity;
+        ep.stride = P.part_capacity;
+        if ((rc = launch_spmv<T, T, T>(P.AT, vals_of<T>(P.AT), I.Y[0], I.Y[1], P.st, ep))) return rc;
+        if (P.has_q && (rc = q_probe<T>(P, I))) return rc;
+        if ((rc = probe_spmv<T>(P, I, true))) return rc;
+    }
+    return count > 0 ? probe_decide(P) : 0;
 }
 
 template <typename T> int adaptive_part(const skk_pdhg &P, int part)
@@ -1384,6 +1470,11 @@ int skk_pdhg_init(const skk_pdhg *P, int prec, double eta0, double omega, double
 int skk_pdhg_adaptive_iter(const skk_pdhg *P, int prec)
 {
     return prec == SKK_F32 ? adaptive_iter<float>(*P) : adaptive_iter<double>(*P);
+}
+
+int skk_pdhg_adaptive_run(const skk_pdhg *P, int prec, int count)
+{
+    return prec == SKK_F32 ? adaptive_run<float>(*P, count) : adaptive_run<double>(*P, count);
 }
 
 int skk_pdhg_adaptive_part(const skk_pdhg *P, int prec, int part)

@@ -26,9 +26,12 @@ int skm_gemm(int M, int N, int K, const uint16_t *A, int lda, int trans_a, const
              int trans_b, const skm_epilogue *ep, float *splitk_buf, int splits);
 int skm_mse_grad(int count, const float *P, const float *T, float scale, float *dP, uint16_t *dP16,
                  double *part, double *loss_sum);
-int skm_colsum(int rows, int cols, const float *dZ, float *db);
+/* db = column sums of dZ; scratch holds max_slices * cols floats. */
+int skm_colsum(int rows, int cols, const float *dZ, float *db, float *scratch, int max_slices);
+/* Adam step; also writes the BF16 copy of each weight block q (offset
+ * w_off[q], length w_len[q] in p) to w_dst[q]. */
 int skm_adam(long long n, float *p, const float *g, float *m, float *v, float lr, float b1, float b2,
-             float eps, int step);
+             float eps, int step, const long long *w_off, const long long *w_len, uint16_t *const *w_dst);
 int skm_to_bf16(long long n, const float *src, uint16_t *dst);
 int skm_gather_std(int rows, int cols, const int *idx, const float *src, const float *mean,
                    const float *stdv, float *dst32, uint16_t *dst16);

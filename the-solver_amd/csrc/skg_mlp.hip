@@ -241,19 +241,36 @@ __global__ void k_sum_partials(const double *part, int count, double *out)
     if (threadIdx.x == 0) *out = sh[0];
 }
 
-// db[c] = sum_r dZ[r][c]
-__global__ void k_colsum(int rows, int cols, const float *dZ, float *db)
+// db[c] = sum_r dZ[r][c] in two fixed-order stages: slice s of the rows
+// (blockIdx.y) gives part[s][c], then the slices are added in order.
+__global__ void k_colsum_part(int rows, int cols, int slices, const float *dZ, float *part)
 {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < cols; c += gridDim.x * blockDim.x) {
-        float s = 0.0f;
-        for (int r = 0; r < rows; ++r) s += dZ[(size_t)r * cols + c];
-        db[c] = s;
-    }
+    const int c = blockIdx.x * blockDim.x + threadIdx.x, s = blockIdx.y;
+    if (c >= cols) return;
+    const int r0 = (int)((long long)rows * s / slices), r1 = (int)((long long)rows * (s + 1) / slices);
+    float acc = 0.0f;
+    for (int r = r0; r < r1; ++r) acc += dZ[(size_t)r * cols + c];
+    part[(size_t)s * cols + c] = acc;
 }
 
-// Adam on the flat FP32 parameter vector; refresh the BF16 weight copies.
+__global__ void k_colsum_fin(int cols, int slices, const float *part, float *db)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    float acc = 0.0f;
+    for (int s = 0; s < slices; ++s) acc += part[(size_t)s * cols + c];
+    db[c] = acc;
+}
+
+// Adam on the flat FP32 parameter vector; the BF16 copies of the three
+// weight blocks are refreshed in the same pass (no second read of p).
+struct Bf16Copies {
+    long long o[3], n[3];    // offset and length of W1, W2, W3 in p
+    uint16_t *dst[3];
+};
+
 __global__ void k_adam(long long n, float *p, const float *g, float *m, float *v, float lr, float b1,
-                       float b2, float eps, float c1, float c2)
+                       float b2, float eps, float c1, float c2, Bf16Copies w)
 {
     for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
          k += (long long)gridDim.x * blockDim.x) {
@@ -262,7 +279,11 @@ __global__ void k_adam(long long n, float *p, const float *g, float *m, float *v
         const float vk = b2 * v[k] + (1.0f - b2) * gk * gk;
         m[k] = mk;
         v[k] = vk;
-        p[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+        const float pk = p[k] - lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+        p[k] = pk;
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            if (k >= w.o[q] && k < w.o[q] + w.n[q]) w.dst[q][k - w.o[q]] = f2bf(pk);
     }
 }
 
@@ -347,18 +368,27 @@ int skm_mse_grad(int count, const float *P, const float *T, float scale, float *
     return (int)hipGetLastError();
 }
 
-int skm_colsum(int rows, int cols, const float *dZ, float *db)
+int skm_colsum(int rows, int cols, const float *dZ, float *db, float *scratch, int max_slices)
 {
-    hipLaunchKernelGGL(k_colsum, dim3(grid1d((size_t)cols)), dim3(THREADS), 0, mstream(), rows, cols, dZ, db);
+    int slices = rows / 8;
+    if (slices > max_slices) slices = max_slices;
+    if (slices < 1) slices = 1;
+    const int gx = (cols + THREADS - 1) / THREADS;
+    hipLaunchKernelGGL(k_colsum_part, dim3(gx, slices), dim3(THREADS), 0, mstream(), rows, cols, slices, dZ, scratch);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return (int)err;
+    hipLaunchKernelGGL(k_colsum_fin, dim3(gx), dim3(THREADS), 0, mstream(), cols, slices, scratch, db);
     return (int)hipGetLastError();
 }
 
 int skm_adam(long long n, float *p, const float *g, float *m, float *v, float lr, float b1, float b2, float eps,
-             int step)
+             int step, const long long *w_off, const long long *w_len, uint16_t *const *w_dst)
 {
     const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+    Bf16Copies w;
+    for (int q = 0; q < 3; ++q) { w.o[q] = w_off[q]; w.n[q] = w_len[q]; w.dst[q] = w_dst[q]; }
     hipLaunchKernelGGL(k_adam, dim3(grid1d((size_t)n)), dim3(THREADS), 0, mstream(), n, p, g, m, v, lr, b1, b2,
-                       eps, c1, c2);
+                       eps, c1, c2, w);
     return (int)hipGetLastError();
 }
 

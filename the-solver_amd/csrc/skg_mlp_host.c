@@ -24,8 +24,13 @@ struct sk_gpu_mlp {
     float *x32, *t32, *h1, *h2, *pred, *dp, *dz2, *dz1, *ybuf, *splitk;
     uint16_t *x16, *h1b, *h2b, *dp16, *dz2b, *dz1b;
     double *part, *loss;
+    float *colsum;               /* COLSUM_SLICES * max(hid, out) floats     */
+    double *loss_dst;            /* where backward leaves the loss sum (NULL: loss) */
+    const int *gidx;             /* batch row indices on the device (NULL: idx)   */
     long long splitk_floats;
 };
+
+#define COLSUM_SLICES 32
 
 static uint64_t sm64(uint64_t *x)
 {
@@ -42,10 +47,11 @@ static void free_work(sk_gpu_mlp *M)
     skk_free(M->idx); skk_free(M->x32); skk_free(M->t32); skk_free(M->h1); skk_free(M->h2);
     skk_free(M->pred); skk_free(M->dp); skk_free(M->dz2); skk_free(M->dz1); skk_free(M->ybuf);
     skk_free(M->splitk); skk_free(M->x16); skk_free(M->h1b); skk_free(M->h2b); skk_free(M->dp16);
-    skk_free(M->dz2b); skk_free(M->dz1b); skk_free(M->part); skk_free(M->loss);
+    skk_free(M->dz2b); skk_free(M->dz1b); skk_free(M->part); skk_free(M->loss); skk_free(M->colsum);
     M->idx = NULL; M->x32 = M->t32 = M->h1 = M->h2 = M->pred = M->dp = M->dz2 = M->dz1 = M->ybuf = NULL;
     M->splitk = NULL; M->x16 = M->h1b = M->h2b = M->dp16 = M->dz2b = M->dz1b = NULL;
     M->part = M->loss = NULL;
+    M->colsum = NULL;
     M->cap = 0;
 }
 
@@ -68,6 +74,7 @@ static int ensure_work(sk_gpu_mlp *M, int rows)
     A(M->dz1, sizeof(float) * r * M->hid); A(M->dz1b, 2 * r * M->hid);
     A(M->splitk, sizeof(float) * (size_t)M->splitk_floats);
     A(M->part, sizeof(double) * 2048); A(M->loss, sizeof(double));
+    A(M->colsum, sizeof(float) * COLSUM_SLICES * (size_t)(M->out > M->hid ? M->out : M->hid));
 #undef A
     if (rc) { free_work(M); return skg_hip_fail("MLP work buffers", rc); }
     M->cap = rows;
@@ -188,7 +195,8 @@ static int forward(sk_gpu_mlp *M, int rows, const float *dev_x)
 {
     skm_epilogue e;
     int rc;
-    if ((rc = skm_gather_std(rows, M->in, M->idx, dev_x, M->in_mean, M->in_std, NULL, M->x16))) return rc;
+    if ((rc = skm_gather_std(rows, M->in, M->gidx ? M->gidx : M->idx, dev_x, M->in_mean, M->in_std, NULL,
+                             M->x16))) return rc;
     memset(&e, 0, sizeof(e));
     e.bias = M->p + M->o_b1; e.relu = 1; e.c32 = M->h1; e.c16 = M->h1b; e.ldc = M->hid;
     if ((rc = skm_gemm(rows, M->hid, M->in, M->x16, M->in, 0, M->w1b, M->hid, 0, &e, NULL, 1))) return rc;
@@ -205,13 +213,15 @@ static int backward(sk_gpu_mlp *M, int rows, const float *dev_y)
     skm_epilogue e;
     int rc, splits;
     const float scale = 1.0f / ((float)rows * (float)M->out);
-    if ((rc = skm_gather_std(rows, M->out, M->idx, dev_y, M->out_mean, M->out_std, M->t32, NULL))) return rc;
-    if ((rc = skm_mse_grad(rows * M->out, M->pred, M->t32, scale, M->dp, M->dp16, M->part, M->loss))) return rc;
+    if ((rc = skm_gather_std(rows, M->out, M->gidx ? M->gidx : M->idx, dev_y, M->out_mean, M->out_std, M->t32,
+                             NULL))) return rc;
+    if ((rc = skm_mse_grad(rows * M->out, M->pred, M->t32, scale, M->dp, M->dp16, M->part,
+                           M->loss_dst ? M->loss_dst : M->loss))) return rc;
     memset(&e, 0, sizeof(e));
     /* dW3 = H2' dP, db3 = colsum dP */
     e.c32 = M->g + M->o_w3; e.ldc = M->out;
     if ((rc = skm_gemm(M->hid, M->out, rows, M->h2b, M->hid, 1, M->dp16, M->out, 0, &e, NULL, 1))) return rc;
-    if ((rc = skm_colsum(rows, M->out, M->dp, M->g + M->o_b3))) return rc;
+    if ((rc = skm_colsum(rows, M->out, M->dp, M->g + M->o_b3, M->colsum, COLSUM_SLICES))) return rc;
     /* dZ2 = (dP W3') * [H2 > 0]; K = out is long: split it */
     splits = (int)(M->splitk_floats / ((long long)rows * M->hid));
     if (splits > 32) splits = 32;
@@ -224,7 +234,7 @@ static int backward(sk_gpu_mlp *M, int rows, const float *dev_y)
     memset(&e, 0, sizeof(e));
     e.c32 = M->g + M->o_w2; e.ldc = M->hid;
     if ((rc = skm_gemm(M->hid, M->hid, rows, M->h1b, M->hid, 1, M->dz2b, M->hid, 0, &e, NULL, 1))) return rc;
-    if ((rc = skm_colsum(rows, M->hid, M->dz2, M->g + M->o_b2))) return rc;
+    if ((rc = skm_colsum(rows, M->hid, M->dz2, M->g + M->o_b2, M->colsum, COLSUM_SLICES))) return rc;
     /* dZ1 = (dZ2 W2') * [H1 > 0] */
     memset(&e, 0, sizeof(e));
     e.mask = M->h1; e.ldm = M->hid; e.c32 = M->dz1; e.c16 = M->dz1b; e.ldc = M->hid;
@@ -233,7 +243,7 @@ static int backward(sk_gpu_mlp *M, int rows, const float *dev_y)
     memset(&e, 0, sizeof(e));
     e.c32 = M->g + M->o_w1; e.ldc = M->hid;
     if ((rc = skm_gemm(M->in, M->hid, rows, M->x16, M->in, 1, M->dz1b, M->hid, 0, &e, NULL, 1))) return rc;
-    return skm_colsum(rows, M->hid, M->dz1, M->g + M->o_b1);
+    return skm_colsum(rows, M->hid, M->dz1, M->g + M->o_b1, M->colsum, COLSUM_SLICES);
 }
 
 static int upload_rows(float **dst, const float *src, long long count)
@@ -271,36 +281,53 @@ int sk_gpu_mlp_train(sk_gpu_mlp *M, const float *X, const float *Y, int count, i
                      unsigned long long seed, float *epoch_loss)
 {
     float *dx = NULL, *dy = NULL;
-    int *perm = NULL, rc = 0, ep, k;
+    double *dloss = NULL, *hloss = NULL;
+    int *perm = NULL, *dperm = NULL, rc = 0, ep, k, steps;
     uint64_t s = seed;
+    const long long off[3] = {M ? M->o_w1 : 0, M ? M->o_w2 : 0, M ? M->o_w3 : 0};
     skg_clear_error();
     if (!M || !X || !Y || count <= 0 || epochs < 0 || batch <= 0 || !(lr > 0.0f)) return skg_fail("invalid arguments");
     if (batch > count) batch = count;
     if (ensure_work(M, batch)) return -1;
+    steps = count / batch;
     perm = (int *)malloc(sizeof(int) * (size_t)count);
-    if (!perm) return skg_fail("out of host memory");
+    hloss = (double *)malloc(sizeof(double) * (size_t)steps);
+    if (!perm || !hloss) { free(perm); free(hloss); return skg_fail("out of host memory"); }
     rc = upload_rows(&dx, X, (long long)count * M->in);
     if (!rc) rc = upload_rows(&dy, Y, (long long)count * M->out);
+    if (!rc) rc = skk_malloc((void **)&dloss, sizeof(double) * (size_t)steps);
+    if (!rc) rc = skk_malloc((void **)&dperm, sizeof(int) * (size_t)count);
     for (k = 0; k < count; ++k) perm[k] = k;
     for (ep = 0; ep < epochs && !rc; ++ep) {
-        double sum = 0.0, l = 0.0;
+        double sum = 0.0;
         int b, nb = 0;
         for (k = count - 1; k > 0; --k) {       /* Fisher-Yates */
             const int j = (int)(u01(&s) * (double)(k + 1));
             const int t = perm[k]; perm[k] = perm[j]; perm[j] = t;
         }
+        /* the epoch's order goes up once and the losses stay on the device
+           until the end of the epoch: no host round trip per step */
+        if (!rc) rc = skk_h2d(dperm, perm, sizeof(int) * (size_t)count);
         for (b = 0; b + batch <= count && !rc; b += batch, ++nb) {
-            if ((rc = skk_h2d(M->idx, perm + b, sizeof(int) * (size_t)batch))) break;
+            const long long len[3] = {(long long)M->in * M->hid, (long long)M->hid * M->hid,
+                                      (long long)M->hid * M->out};
+            uint16_t *const dst[3] = {M->w1b, M->w2b, M->w3b};
+            M->gidx = dperm + b;
+            M->loss_dst = dloss + nb;
             if ((rc = forward(M, batch, dx)) || (rc = backward(M, batch, dy))) break;
             ++M->step;
-            if ((rc = skm_adam(M->np, M->p, M->g, M->m, M->v, lr, 0.9f, 0.999f, 1e-8f, M->step))) break;
-            if (refresh_bf16(M)) { rc = -1; break; }
-            if ((rc = skk_d2h(&l, M->loss, sizeof(double)))) break;
-            sum += l / ((double)batch * M->out);
+            if ((rc = skm_adam(M->np, M->p, M->g, M->m, M->v, lr, 0.9f, 0.999f, 1e-8f, M->step, off, len, dst)))
+                break;
         }
+        M->loss_dst = NULL;
+        M->gidx = NULL;
+        if (!rc && nb) rc = skk_d2h(hloss, dloss, sizeof(double) * (size_t)nb);
+        for (k = 0; k < nb && !rc; ++k) sum += hloss[k] / ((double)batch * M->out);
         if (epoch_loss) epoch_loss[ep] = nb ? (float)(sum / nb) : 0.0f;
     }
-    free(perm); skk_free(dx); skk_free(dy);
+    M->loss_dst = NULL;
+    M->gidx = NULL;
+    free(perm); free(hloss); skk_free(dx); skk_free(dy); skk_free(dloss); skk_free(dperm);
     return rc ? skg_hip_fail("MLP training", rc) : 0;
 }
 

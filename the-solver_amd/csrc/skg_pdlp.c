@@ -18,6 +18,10 @@
 
 skg_tuning g_skg_tuning = {4, 512};
 
+/* Block partials (A, A^T and Q products together) up to which the step
+ * decision is fused into the next dual prox. */
+#define FUSE_MAX_PARTIALS 1024
+
 void sk_gpu_pdlp_params_default(sk_gpu_pdlp_params *p)
 {
     memset(p, 0, sizeof(*p));
@@ -348,8 +352,13 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
         if (E->AT.k.grid > cap) cap = E->AT.k.grid;
         if (E->has_q && E->Q.k.grid > cap) cap = E->Q.k.grid;
         Q->part_capacity = cap;
+        /* with few block partials every dual-prox workgroup can afford to
+           reduce them itself: the step decision then rides in that launch */
+        Q->fuse_decision = !P->unfused &&
+                           E->A.k.grid + E->AT.k.grid + (E->has_q ? E->Q.k.grid : 0) <= FUSE_MAX_PARTIALS;
         if ((rc = dalloc(E, (void **)&Q->ax_d, 8 * mm)) || (rc = dalloc(E, (void **)&Q->aty_d, 8 * nn)) ||
             (rc = dalloc(E, (void **)&Q->st, sizeof(skk_state))) ||
+            (rc = dalloc(E, (void **)&Q->snap, sizeof(skk_state))) ||
             (rc = dalloc(E, (void **)&Q->part, sizeof(double) * 16 * (size_t)cap)) ||
             (rc = dalloc(E, (void **)&Q->counters, sizeof(unsigned) * 16)) ||
             (rc = dalloc(E, (void **)&Q->eval_out, sizeof(double) * 16)) ||
@@ -404,11 +413,14 @@ static kkt kkt_from(const double *s, double cost_norm, double bound_norm)
 static int queue_window(engine *E, int prec, int period, int adaptive, double fixed_terms)
 {
     int k, rc;
-    for (k = 0; k < period; ++k) {
-        rc = adaptive ? skk_pdhg_adaptive_iter(&E->P, prec) : skk_pdhg_fixed_iter(&E->P, prec);
-        if (rc) return skg_hip_fail("PDHG iteration launch", rc);
+    if (adaptive) {
+        if ((rc = skk_pdhg_adaptive_run(&E->P, prec, period))) return skg_hip_fail("PDHG iteration launch", rc);
+        E->launches += (long long)period * (E->P.fuse_decision ? 3 : 4) + (E->P.fuse_decision ? 1 : 0);
+    } else {
+        for (k = 0; k < period; ++k)
+            if ((rc = skk_pdhg_fixed_iter(&E->P, prec))) return skg_hip_fail("PDHG iteration launch", rc);
+        E->launches += (long long)period * 2;
     }
-    E->launches += (long long)period * (adaptive ? 3 : 2);
     if (adaptive && (rc = skk_pdhg_flush(&E->P, prec))) return skg_hip_fail("flush launch", rc);
     if ((rc = skk_pdhg_eval(&E->P, prec, 0, adaptive, fixed_terms))) return skg_hip_fail("evaluation launch", rc);
     if ((rc = skk_pdhg_eval(&E->P, prec, 1, adaptive, fixed_terms))) return skg_hip_fail("evaluation launch", rc);
@@ -625,9 +637,15 @@ int sk_gpu_pdlp_session_iterate(sk_gpu_pdlp_session *S, int iterations, double *
         for (part = 0; part < 5; ++part) kernel_ms[part] /= (iterations > 0 ? iterations : 1);
         return 0;
     }
-    for (k = 0; k < iterations; ++k) {
-        rc = S->P.adaptive_step ? skk_pdhg_adaptive_iter(&E->P, prec) : skk_pdhg_fixed_iter(&E->P, prec);
-        if (rc) return skg_hip_fail("PDHG launch", rc);
+    if (S->P.adaptive_step) {
+        /* windows of restart_period iterations, as the solve queues them */
+        for (k = 0; k < iterations; k += S->P.restart_period) {
+            const int cnt = iterations - k < S->P.restart_period ? iterations - k : S->P.restart_period;
+            if ((rc = skk_pdhg_adaptive_run(&E->P, prec, cnt))) return skg_hip_fail("PDHG launch", rc);
+        }
+    } else {
+        for (k = 0; k < iterations; ++k)
+            if ((rc = skk_pdhg_fixed_iter(&E->P, prec))) return skg_hip_fail("PDHG launch", rc);
     }
     if ((rc = skk_sync())) return skg_hip_fail("sync", rc);
     return 0;

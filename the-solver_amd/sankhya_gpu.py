@@ -97,7 +97,8 @@ class PdlpParams(C.Structure):
                 ("precision", C.c_int), ("tail_switch", C.c_double),
                 ("tail_stall_windows", C.c_int), ("scaling", C.c_int),
                 ("ruiz_iterations", C.c_int), ("ruiz_tol", C.c_double),
-                ("use_graph", C.c_int), ("verbosity", C.c_int), ("primal_weight", C.c_double)]
+                ("use_graph", C.c_int), ("verbosity", C.c_int), ("primal_weight", C.c_double),
+                ("unfused", C.c_int), ("pad", C.c_int)]
 
 
 class PdlpStats(C.Structure):
