@@ -294,7 +294,7 @@ typedef struct sk_gpu_pdlp_params {  /* src/cuda/GpuPdlp.hpp:19-67 PdlpParams */
     int    precision;        /* SK_GPU_PREC_*                               */
     double tail_switch;      /* MIXED: KKT at which FP64 takes over (1e-5)  */
     int    tail_stall_windows; /* MIXED: windows without progress before the
-                                  FP64 tail starts early (8)                */
+                                  FP64 tail starts early (32)               */
     int    scaling;          /* 1: Ruiz then Pock-Chambolle, 0: none        */
     int    ruiz_iterations;  /* 100                                         */
     double ruiz_tol;         /* 1e-2                                        */

@@ -35,7 +35,7 @@ void sk_gpu_pdlp_params_default(sk_gpu_pdlp_params *p)
     p->eta_max_factor = 1e4;
     p->precision = SK_GPU_PREC_FP64;
     p->tail_switch = 1e-5;
-    p->tail_stall_windows = 8;
+    p->tail_stall_windows = 32;
     p->scaling = 1;
     p->ruiz_iterations = 100;
     p->ruiz_tol = 1e-2;

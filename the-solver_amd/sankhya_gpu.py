@@ -151,7 +151,7 @@ def default_pdlp_params(**overrides) -> PdlpParams:
     p = PdlpParams(eps_optimal=1e-6, max_iterations=500000, restart_period=64,
                    time_limit_seconds=0.0, power_iterations=30, adaptive_step=1,
                    eta_min_factor=1e-6, eta_max_factor=1e4, precision=PREC_FP64,
-                   tail_switch=1e-5, tail_stall_windows=8, scaling=1,
+                   tail_switch=1e-5, tail_stall_windows=32, scaling=1,
                    ruiz_iterations=100, ruiz_tol=1e-2, use_graph=0, verbosity=0)
     for k, v in overrides.items():
         setattr(p, k, v)
