@@ -33,6 +33,8 @@ int skm_colsum(int rows, int cols, const float *dZ, float *db, float *scratch, i
 int skm_adam(long long n, float *p, const float *g, float *m, float *v, float lr, float b1, float b2,
              float eps, int step, const long long *w_off, const long long *w_len, uint16_t *const *w_dst);
 int skm_to_bf16(long long n, const float *src, uint16_t *dst);
+/* dst (cols x rows, leading dimension ld_dst) = transpose of src (rows x cols). */
+int skm_transpose_bf16(int rows, int cols, const uint16_t *src, int ld_src, uint16_t *dst, int ld_dst);
 int skm_gather_std(int rows, int cols, const int *idx, const float *src, const float *mean,
                    const float *stdv, float *dst32, uint16_t *dst16);
 int skm_destandardize(int rows, int cols, const float *p, const float *mean, const float *stdv, float *y);

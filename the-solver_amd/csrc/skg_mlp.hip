@@ -287,6 +287,25 @@ __global__ void k_adam(long long n, float *p, const float *g, float *m, float *v
     }
 }
 
+// dst[c * ld_dst + r] = src[r * ld_src + c] for r < rows, c < cols: 64 x 64
+// tiles through LDS so both the reads and the writes are row-contiguous.
+__global__ __launch_bounds__(THREADS) void k_transpose_bf16(int rows, int cols, const uint16_t *__restrict__ src,
+                                                            int ld_src, uint16_t *__restrict__ dst, int ld_dst)
+{
+    __shared__ uint16_t tile[64][65];
+    const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;    // 64 x 4
+    for (int k = ty; k < 64; k += 4) {
+        const int r = r0 + k, c = c0 + tx;
+        tile[k][tx] = (r < rows && c < cols) ? src[(size_t)r * ld_src + c] : (uint16_t)0;
+    }
+    __syncthreads();
+    for (int k = ty; k < 64; k += 4) {
+        const int c = c0 + k, r = r0 + tx;
+        if (c < cols && r < rows) dst[(size_t)c * ld_dst + r] = tile[tx][k];
+    }
+}
+
 __global__ void k_to_bf16(long long n, const float *src, uint16_t *dst)
 {
     for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
@@ -389,6 +408,14 @@ int skm_adam(long long n, float *p, const float *g, float *m, float *v, float lr
     for (int q = 0; q < 3; ++q) { w.o[q] = w_off[q]; w.n[q] = w_len[q]; w.dst[q] = w_dst[q]; }
     hipLaunchKernelGGL(k_adam, dim3(grid1d((size_t)n)), dim3(THREADS), 0, mstream(), n, p, g, m, v, lr, b1, b2,
                        eps, c1, c2, w);
+    return (int)hipGetLastError();
+}
+
+int skm_transpose_bf16(int rows, int cols, const uint16_t *src, int ld_src, uint16_t *dst, int ld_dst)
+{
+    if (rows <= 0 || cols <= 0) return 0;
+    hipLaunchKernelGGL(k_transpose_bf16, dim3((cols + 63) / 64, (rows + 63) / 64), dim3(THREADS), 0, mstream(),
+                       rows, cols, src, ld_src, dst, ld_dst);
     return (int)hipGetLastError();
 }
 

@@ -15,7 +15,9 @@ struct sk_gpu_mlp {
     long long np;                /* parameter count                          */
     long long o_w1, o_b1, o_w2, o_b2, o_w3, o_b3;
     float *p, *g, *m, *v;        /* device: params, grads, Adam moments      */
-    uint16_t *w1b, *w2b, *w3b;   /* device BF16 weight copies                */
+    uint16_t *w1b, *w2b, *w3b;   /* device BF16 weight copies [in][out]      */
+    uint16_t *w1t, *w2t, *w3t;   /* and transposed [out][in]: every GEMM reads
+                                    both operands K-contiguous               */
     float *in_mean, *in_std, *out_mean, *out_std;
     int step;
     /* work buffers for batches of up to `cap` rows */
@@ -23,6 +25,7 @@ struct sk_gpu_mlp {
     int *idx;
     float *x32, *t32, *h1, *h2, *pred, *dp, *dz2, *dz1, *ybuf, *splitk;
     uint16_t *x16, *h1b, *h2b, *dp16, *dz2b, *dz1b;
+    uint16_t *x16t, *h1t, *h2t, *dpt, *dz2t, *dz1t;   /* [cols][rows] copies */
     double *part, *loss;
     float *colsum;               /* COLSUM_SLICES * max(hid, out) floats     */
     double *loss_dst;            /* where backward leaves the loss sum (NULL: loss) */
@@ -48,6 +51,8 @@ static void free_work(sk_gpu_mlp *M)
     skk_free(M->pred); skk_free(M->dp); skk_free(M->dz2); skk_free(M->dz1); skk_free(M->ybuf);
     skk_free(M->splitk); skk_free(M->x16); skk_free(M->h1b); skk_free(M->h2b); skk_free(M->dp16);
     skk_free(M->dz2b); skk_free(M->dz1b); skk_free(M->part); skk_free(M->loss); skk_free(M->colsum);
+    skk_free(M->x16t); skk_free(M->h1t); skk_free(M->h2t); skk_free(M->dpt); skk_free(M->dz2t); skk_free(M->dz1t);
+    M->x16t = M->h1t = M->h2t = M->dpt = M->dz2t = M->dz1t = NULL;
     M->idx = NULL; M->x32 = M->t32 = M->h1 = M->h2 = M->pred = M->dp = M->dz2 = M->dz1 = M->ybuf = NULL;
     M->splitk = NULL; M->x16 = M->h1b = M->h2b = M->dp16 = M->dz2b = M->dz1b = NULL;
     M->part = M->loss = NULL;
@@ -72,6 +77,8 @@ static int ensure_work(sk_gpu_mlp *M, int rows)
     A(M->dp, sizeof(float) * r * M->out); A(M->dp16, 2 * r * M->out);
     A(M->dz2, sizeof(float) * r * M->hid); A(M->dz2b, 2 * r * M->hid);
     A(M->dz1, sizeof(float) * r * M->hid); A(M->dz1b, 2 * r * M->hid);
+    A(M->x16t, 2 * r * M->in); A(M->h1t, 2 * r * M->hid); A(M->h2t, 2 * r * M->hid);
+    A(M->dpt, 2 * r * M->out); A(M->dz2t, 2 * r * M->hid); A(M->dz1t, 2 * r * M->hid);
     A(M->splitk, sizeof(float) * (size_t)M->splitk_floats);
     A(M->part, sizeof(double) * 2048); A(M->loss, sizeof(double));
     A(M->colsum, sizeof(float) * COLSUM_SLICES * (size_t)(M->out > M->hid ? M->out : M->hid));
@@ -89,8 +96,20 @@ void sk_gpu_mlp_destroy(sk_gpu_mlp *M)
     free_work(M);
     skk_free(M->p); skk_free(M->g); skk_free(M->m); skk_free(M->v);
     skk_free(M->w1b); skk_free(M->w2b); skk_free(M->w3b);
+    skk_free(M->w1t); skk_free(M->w2t); skk_free(M->w3t);
     skk_free(M->in_mean); skk_free(M->in_std); skk_free(M->out_mean); skk_free(M->out_std);
     free(M);
+}
+
+/* The [out][in] BF16 weight copies from the [in][out] ones. */
+static int transpose_weights(sk_gpu_mlp *M)
+{
+    int rc;
+    if ((rc = skm_transpose_bf16(M->in, M->hid, M->w1b, M->hid, M->w1t, M->in)) ||
+        (rc = skm_transpose_bf16(M->hid, M->hid, M->w2b, M->hid, M->w2t, M->hid)) ||
+        (rc = skm_transpose_bf16(M->hid, M->out, M->w3b, M->out, M->w3t, M->hid)))
+        return rc;
+    return 0;
 }
 
 static int refresh_bf16(sk_gpu_mlp *M)
@@ -98,7 +117,8 @@ static int refresh_bf16(sk_gpu_mlp *M)
     int rc;
     if ((rc = skm_to_bf16((long long)M->in * M->hid, M->p + M->o_w1, M->w1b)) ||
         (rc = skm_to_bf16((long long)M->hid * M->hid, M->p + M->o_w2, M->w2b)) ||
-        (rc = skm_to_bf16((long long)M->hid * M->out, M->p + M->o_w3, M->w3b)))
+        (rc = skm_to_bf16((long long)M->hid * M->out, M->p + M->o_w3, M->w3b)) ||
+        (rc = transpose_weights(M)))
         return skg_hip_fail("weight conversion", rc);
     return 0;
 }
@@ -140,6 +160,8 @@ int sk_gpu_mlp_create(int inputs, int hidden, int outputs, unsigned long long se
     A(M->m, sizeof(float) * (size_t)M->np); A(M->v, sizeof(float) * (size_t)M->np);
     A(M->w1b, 2 * (size_t)inputs * hidden); A(M->w2b, 2 * (size_t)hidden * hidden);
     A(M->w3b, 2 * (size_t)hidden * outputs);
+    A(M->w1t, 2 * (size_t)inputs * hidden); A(M->w2t, 2 * (size_t)hidden * hidden);
+    A(M->w3t, 2 * (size_t)hidden * outputs);
     A(M->in_mean, sizeof(float) * inputs); A(M->in_std, sizeof(float) * inputs);
     A(M->out_mean, sizeof(float) * outputs); A(M->out_std, sizeof(float) * outputs);
 #undef A
@@ -199,12 +221,12 @@ static int forward(sk_gpu_mlp *M, int rows, const float *dev_x)
                              M->x16))) return rc;
     memset(&e, 0, sizeof(e));
     e.bias = M->p + M->o_b1; e.relu = 1; e.c32 = M->h1; e.c16 = M->h1b; e.ldc = M->hid;
-    if ((rc = skm_gemm(rows, M->hid, M->in, M->x16, M->in, 0, M->w1b, M->hid, 0, &e, NULL, 1))) return rc;
+    if ((rc = skm_gemm(rows, M->hid, M->in, M->x16, M->in, 0, M->w1t, M->in, 1, &e, NULL, 1))) return rc;
     e.bias = M->p + M->o_b2; e.c32 = M->h2; e.c16 = M->h2b;
-    if ((rc = skm_gemm(rows, M->hid, M->hid, M->h1b, M->hid, 0, M->w2b, M->hid, 0, &e, NULL, 1))) return rc;
+    if ((rc = skm_gemm(rows, M->hid, M->hid, M->h1b, M->hid, 0, M->w2t, M->hid, 1, &e, NULL, 1))) return rc;
     memset(&e, 0, sizeof(e));
     e.bias = M->p + M->o_b3; e.c32 = M->pred; e.ldc = M->out;
-    return skm_gemm(rows, M->out, M->hid, M->h2b, M->hid, 0, M->w3b, M->out, 0, &e, NULL, 1);
+    return skm_gemm(rows, M->out, M->hid, M->h2b, M->hid, 0, M->w3t, M->hid, 1, &e, NULL, 1);
 }
 
 /* Loss and gradients of one batch (targets gathered from dev_y). */
@@ -220,7 +242,10 @@ static int backward(sk_gpu_mlp *M, int rows, const float *dev_y)
     memset(&e, 0, sizeof(e));
     /* dW3 = H2' dP, db3 = colsum dP */
     e.c32 = M->g + M->o_w3; e.ldc = M->out;
-    if ((rc = skm_gemm(M->hid, M->out, rows, M->h2b, M->hid, 1, M->dp16, M->out, 0, &e, NULL, 1))) return rc;
+    if ((rc = skm_transpose_bf16(rows, M->hid, M->h2b, M->hid, M->h2t, rows)) ||
+        (rc = skm_transpose_bf16(rows, M->out, M->dp16, M->out, M->dpt, rows)))
+        return rc;
+    if ((rc = skm_gemm(M->hid, M->out, rows, M->h2t, rows, 0, M->dpt, rows, 1, &e, NULL, 1))) return rc;
     if ((rc = skm_colsum(rows, M->out, M->dp, M->g + M->o_b3, M->colsum, COLSUM_SLICES))) return rc;
     /* dZ2 = (dP W3') * [H2 > 0]; K = out is long: split it */
     splits = (int)(M->splitk_floats / ((long long)rows * M->hid));
@@ -233,7 +258,10 @@ static int backward(sk_gpu_mlp *M, int rows, const float *dev_y)
     /* dW2 = H1' dZ2, db2 */
     memset(&e, 0, sizeof(e));
     e.c32 = M->g + M->o_w2; e.ldc = M->hid;
-    if ((rc = skm_gemm(M->hid, M->hid, rows, M->h1b, M->hid, 1, M->dz2b, M->hid, 0, &e, NULL, 1))) return rc;
+    if ((rc = skm_transpose_bf16(rows, M->hid, M->h1b, M->hid, M->h1t, rows)) ||
+        (rc = skm_transpose_bf16(rows, M->hid, M->dz2b, M->hid, M->dz2t, rows)))
+        return rc;
+    if ((rc = skm_gemm(M->hid, M->hid, rows, M->h1t, rows, 0, M->dz2t, rows, 1, &e, NULL, 1))) return rc;
     if ((rc = skm_colsum(rows, M->hid, M->dz2, M->g + M->o_b2, M->colsum, COLSUM_SLICES))) return rc;
     /* dZ1 = (dZ2 W2') * [H1 > 0] */
     memset(&e, 0, sizeof(e));
@@ -242,7 +270,10 @@ static int backward(sk_gpu_mlp *M, int rows, const float *dev_y)
     /* dW1 = X' dZ1, db1 */
     memset(&e, 0, sizeof(e));
     e.c32 = M->g + M->o_w1; e.ldc = M->hid;
-    if ((rc = skm_gemm(M->in, M->hid, rows, M->x16, M->in, 1, M->dz1b, M->hid, 0, &e, NULL, 1))) return rc;
+    if ((rc = skm_transpose_bf16(rows, M->in, M->x16, M->in, M->x16t, rows)) ||
+        (rc = skm_transpose_bf16(rows, M->hid, M->dz1b, M->hid, M->dz1t, rows)))
+        return rc;
+    if ((rc = skm_gemm(M->in, M->hid, rows, M->x16t, rows, 0, M->dz1t, rows, 1, &e, NULL, 1))) return rc;
     return skm_colsum(rows, M->hid, M->dz1, M->g + M->o_b1, M->colsum, COLSUM_SLICES);
 }
 
@@ -318,6 +349,7 @@ int sk_gpu_mlp_train(sk_gpu_mlp *M, const float *X, const float *Y, int count, i
             ++M->step;
             if ((rc = skm_adam(M->np, M->p, M->g, M->m, M->v, lr, 0.9f, 0.999f, 1e-8f, M->step, off, len, dst)))
                 break;
+            if ((rc = transpose_weights(M))) break;
         }
         M->loss_dst = NULL;
         M->gidx = NULL;
