@@ -525,7 +525,7 @@ def main():
     ap.add_argument("--quick", action="store_true", help="small instance (smoke)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch restart windows directly instead of replaying hipGraphs (profilers)")
-    ap.add_argument("--solve-time-limit", type=float, default=150.0)
+    ap.add_argument("--solve-time-limit", type=float, default=60.0)
     ap.add_argument("--train", type=int, default=16000, help="warm4: training instances")
     ap.add_argument("--test", type=int, default=4000, help="warm4: held-out instances")
     ap.add_argument("--compare", type=int, default=100, help="warm4: held-out instances solved cold and warm")
