@@ -288,12 +288,33 @@ __global__ void k_adam(long long n, float *p, const float *g, float *m, float *v
 }
 
 // dst[c * ld_dst + r] = src[r * ld_src + c] for r < rows, c < cols: 64 x 64
-// tiles through LDS so both the reads and the writes are row-contiguous.
+// tiles through LDS so both the reads and the writes are row-contiguous;
+// with even extents and leading dimensions (pairs) each lane moves two
+// elements per access.
 __global__ __launch_bounds__(THREADS) void k_transpose_bf16(int rows, int cols, const uint16_t *__restrict__ src,
-                                                            int ld_src, uint16_t *__restrict__ dst, int ld_dst)
+                                                            int ld_src, uint16_t *__restrict__ dst, int ld_dst,
+                                                            int pairs)
 {
-    __shared__ uint16_t tile[64][65];
+    __shared__ uint16_t tile[64][66];
     const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+    if (pairs) {
+        const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 lanes, 2 elements each
+        for (int k = ty; k < 64; k += 8) {
+            const int r = r0 + k, c = c0 + 2 * tx;
+            uint32_t w = 0;
+            if (r < rows && c < cols) w = *reinterpret_cast<const uint32_t *>(src + (size_t)r * ld_src + c);
+            tile[k][2 * tx] = (uint16_t)(w & 0xffffu);
+            tile[k][2 * tx + 1] = (uint16_t)(w >> 16);
+        }
+        __syncthreads();
+        for (int k = ty; k < 64; k += 8) {
+            const int c = c0 + k, r = r0 + 2 * tx;
+            if (c < cols && r < rows)
+                *reinterpret_cast<uint32_t *>(dst + (size_t)c * ld_dst + r) =
+                    (uint32_t)tile[2 * tx][k] | ((uint32_t)tile[2 * tx + 1][k] << 16);
+        }
+        return;
+    }
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;    // 64 x 4
     for (int k = ty; k < 64; k += 4) {
         const int r = r0 + k, c = c0 + tx;
@@ -414,8 +435,11 @@ int skm_adam(long long n, float *p, const float *g, float *m, float *v, float lr
 int skm_transpose_bf16(int rows, int cols, const uint16_t *src, int ld_src, uint16_t *dst, int ld_dst)
 {
     if (rows <= 0 || cols <= 0) return 0;
+    // pairs: both extents and leading dimensions even, 4-byte aligned bases
+    const int pairs = rows % 2 == 0 && cols % 2 == 0 && ld_src % 2 == 0 && ld_dst % 2 == 0 &&
+                      (uintptr_t)src % 4 == 0 && (uintptr_t)dst % 4 == 0;
     hipLaunchKernelGGL(k_transpose_bf16, dim3((cols + 63) / 64, (rows + 63) / 64), dim3(THREADS), 0, mstream(),
-                       rows, cols, src, ld_src, dst, ld_dst);
+                       rows, cols, src, ld_src, dst, ld_dst, pairs);
     return (int)hipGetLastError();
 }
 

@@ -34,6 +34,9 @@ struct sk_gpu_mlp {
 };
 
 #define COLSUM_SLICES 32
+/* K splits of the long-K gradient product dP W3' (K = outputs): enough
+ * 128 x 128 tiles to cover the device several times over */
+#define MAX_SPLITS 128
 
 static uint64_t sm64(uint64_t *x)
 {
@@ -66,7 +69,7 @@ static int ensure_work(sk_gpu_mlp *M, int rows)
     int rc = 0;
     if (rows <= M->cap) return 0;
     free_work(M);
-    M->splitk_floats = 32LL * rows * M->hid;
+    M->splitk_floats = (long long)MAX_SPLITS * rows * M->hid;
 #define A(p, bytes) if (!rc) rc = skk_malloc((void **)&(p), (bytes))
     A(M->idx, sizeof(int) * r);
     A(M->x32, sizeof(float) * r * M->in); A(M->x16, 2 * r * M->in);
@@ -249,7 +252,7 @@ static int backward(sk_gpu_mlp *M, int rows, const float *dev_y)
     if ((rc = skm_colsum(rows, M->out, M->dp, M->g + M->o_b3, M->colsum, COLSUM_SLICES))) return rc;
     /* dZ2 = (dP W3') * [H2 > 0]; K = out is long: split it */
     splits = (int)(M->splitk_floats / ((long long)rows * M->hid));
-    if (splits > 32) splits = 32;
+    if (splits > MAX_SPLITS) splits = MAX_SPLITS;
     if (splits < 1) splits = 1;
     memset(&e, 0, sizeof(e));
     e.mask = M->h2; e.ldm = M->hid; e.c32 = M->dz2; e.c16 = M->dz2b; e.ldc = M->hid;
