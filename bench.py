@@ -553,7 +553,43 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-WORKLOADS = {"lp1": workload_lp1, "lp0": workload_lp0, "qp2": workload_qp2, "batch3": workload_batch3, "warm4": workload_warm4}
+def workload_mlp4(args):
+    """The warm-start MLP's kernels alone at the config-4 shape
+    (16 -> 512 -> 512 -> 70,000, batch 256) on random standardised data:
+    training and inference throughput against the BF16 MFMA peak."""
+    n_in, hid, out = 16, 512, 70000
+    rows = max(args.train if args.train != 16000 else 4096, 512)
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((rows, n_in)).astype(np.float32)
+    Y = rng.standard_normal((rows, out)).astype(np.float32)
+    net = sg.Mlp(n_in, hid, out, seed=1)
+    net.set_scaling(X.mean(0), X.std(0), Y.mean(0), Y.std(0))
+    net.train(X[:512], Y[:512], epochs=1, batch=256, lr=1e-3)      # warm-up
+    epochs = max(args.epochs if args.epochs != 40 else 2, 1)
+    t = time.perf_counter()
+    net.train(X, Y, epochs=epochs, batch=256, lr=1e-3)
+    train_s = time.perf_counter() - t
+    steps = epochs * (rows // 256)
+    per_sample = 2 * (n_in * hid + hid * hid + hid * out)             # forward FLOP
+    train_tf = steps * 256 * 3 * per_sample / train_s / 1e12
+    t = time.perf_counter()
+    net.predict(X)
+    infer_s = time.perf_counter() - t
+    extra = {"mlp": {"params": net.n_params, "rows": rows, "epochs": epochs, "steps": steps,
+                     "ms_per_step": round(1e3 * train_s / steps, 3), "train_TFLOP_s": round(train_tf, 1),
+                     "predict_s": round(infer_s, 4),
+                     "predict_TFLOP_s_incl_transfers": round(rows * per_sample / infer_s / 1e12, 2)},
+             "roofline": {"bound": "mfma", "achieved": round(train_tf, 1), "peak": 2500.0, "unit": "TFLOP/s",
+                          "frac": round(train_tf / 2500.0, 4), "traffic": None,
+                          "note": "whole training step (GEMMs, transposes, Adam) over dense BF16 peak"}}
+    line = _line(round(train_tf, 1), "TFLOP/s (MLP training step)", args,
+                 "MLP 16-512-512-70000 BF16 MFMA, batch 256, random data (config-4 shape)", extra)
+    line["dtype"] = "bf16 MFMA, fp32 accumulate"
+    return line
+
+
+WORKLOADS = {"lp1": workload_lp1, "lp0": workload_lp0, "qp2": workload_qp2, "batch3": workload_batch3,
+             "warm4": workload_warm4, "mlp4": workload_mlp4}
 
 if __name__ == "__main__":
     main()
