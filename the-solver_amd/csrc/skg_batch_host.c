@@ -15,6 +15,9 @@ void sk_gpu_batch_options_default(sk_gpu_batch_options *o)
     sk_gpu_pdlp_params_default(&o->pdlp);
     o->pdlp.max_iterations = 100000;
     o->pdlp.precision = SK_GPU_PREC_MIXED;
+    /* small instances: an earlier FP64 handoff measured faster for the whole
+       batch (0.79 s against 0.86 s with 32 windows, 4,096 instances) */
+    o->pdlp.tail_stall_windows = 8;
     o->block_threads = 0;       /* autotune */
     o->use_graph = 1;
 }
