@@ -297,12 +297,17 @@ def _kkt(lib, model, arrays, q, x, y):
     return out
 
 
-def _engines(lib, model, arrays, q, args, use_graph=0):
-    """Iterations/s and cold time to 1e-6 for the mixed and the FP64 engine."""
+def _engines(lib, model, arrays, q, args, use_graph=0, repeats=3):
+    """Iterations/s and cold time to 1e-6 for the mixed and the FP64 engine:
+    the median of `repeats` cold solves of the resident session (the loop on
+    the device) and, end to end, the session set-up (scaling, permutation,
+    upload, ||A||) plus one solve."""
     res = {}
     for name, prec in (("mixed", sg.PREC_MIXED), ("fp64", sg.PREC_FP64)):
+        t = time.perf_counter()
         S = _session_qp(lib, model, arrays, q, precision=prec, use_graph=use_graph,
                         time_limit_seconds=args.solve_time_limit)
+        setup_s = time.perf_counter() - t
         S.iterate(args.warmup)
         t = time.perf_counter()
         S.iterate(args.steps)
@@ -310,10 +315,16 @@ def _engines(lib, model, arrays, q, args, use_graph=0):
         ms = S.iterate(min(args.steps, 200), timed=True)
         r = dict(iters_per_s=args.steps / wall, kernel_ms=ms.tolist())
         if not args.skip_solve:
-            rc, x, y, st = S.solve()
+            loops, solves = [], []
+            for _ in range(repeats):
+                t = time.perf_counter()
+                rc, x, y, st = S.solve()
+                solves.append(time.perf_counter() - t)
+                loops.append(st.loop_seconds)
             kkt = _kkt(lib, model, arrays, q, x, y)
             r.update(converged=bool(st.converged), iterations=st.iterations, fp32_iterations=st.fp32_iterations,
-                     fp64_iterations=st.fp64_iterations, loop_s=st.loop_seconds, restarts=st.restarts,
+                     fp64_iterations=st.fp64_iterations, loop_s=float(np.median(loops)), loop_s_all=loops,
+                     end_to_end_s=setup_s + float(np.median(solves)), setup_s=setup_s, restarts=st.restarts,
                      original_kkt=kkt[:3].tolist(), objective=float(kkt[3]))
             log(f"{name}: {r['iters_per_s']:.0f} it/s, solve conv={st.converged} it={st.iterations} "
                 f"loop={st.loop_seconds:.3f}s kkt={kkt[:3]}")
