@@ -195,29 +195,6 @@ __device__ __forceinline__ AT row_part(const int *__restrict__ ci, const VT *__r
     return s0 + s1;
 }
 
-// Epilogues that define `Pre` can have a row's operands loaded before the
-// product phase (load(row)) and handed to row(row, sum, pre, partials), so
-// those loads overlap the gathers instead of following them.
-template <class E, class = void> struct HasPre : std::false_type {};
-template <class E> struct HasPre<E, std::void_t<typename E::Pre>> : std::true_type {};
-template <class E, bool = HasPre<E>::value> struct PreOf { struct type {}; };
-template <class E> struct PreOf<E, true> { using type = typename E::Pre; };
-
-template <class Epi>
-__device__ __forceinline__ typename PreOf<Epi>::type epi_load(const Epi &e, int row)
-{
-    if constexpr (HasPre<Epi>::value) return e.load(row);
-    else return {};
-}
-
-template <class Epi, typename AT>
-__device__ __forceinline__ void epi_row(const Epi &e, int row, AT s, const typename PreOf<Epi>::type &pre,
-                                        double *part)
-{
-    if constexpr (HasPre<Epi>::value) e.row(row, s, pre, part);
-    else e.row(row, s, part);
-}
-
 // CSR-stream product over row tiles (the CSR-Adaptive scheme of Greathouse
 // and Daga, SC'14): a tile's entries are multiplied by consecutive lanes --
 // fully coalesced value and index loads -- into LDS, then each row of the
@@ -255,38 +232,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
             if (tid == 0) {
                 AT tot = prod[0];
                 for (int w = 1; w < kWaves; ++w) tot += prod[w];
-                epi_row(e, r0, tot, epi_load(e, r0), part);
+                e.row(r0, tot, part);
             }
             __syncthreads();
             continue;
-        }
-        // the rows this lane will finish: their bounds and epilogue operands
-        // are loaded now, so they arrive while the products are gathered
-        int lg = 0;
-        while (lg < 6 && (nr << (lg + 1)) <= kBlock) ++lg;
-        const bool grouped = nr * 2 <= kBlock;
-        // (the first row of each lane only: more would cost occupancy)
-        typename PreOf<Epi>::type pre0;
-        int q0[SKK_TILE_ROWS / kBlock], q1[SKK_TILE_ROWS / kBlock];
-        if (grouped) {
-            const int g = tid >> lg, lane = tid & ((1 << lg) - 1);
-            q0[0] = q1[0] = 0;
-            if (g < nr) {
-                q0[0] = rp[r0 + g] - p0 + lane;
-                q1[0] = rp[r0 + g + 1] - p0;
-                if (lane == 0) pre0 = epi_load(e, r0 + g);
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < SKK_TILE_ROWS / kBlock; ++k) {
-                const int g = tid + k * kBlock;
-                q0[k] = q1[k] = 0;
-                if (g < nr) {
-                    q0[k] = rp[r0 + g] - p0;
-                    q1[k] = rp[r0 + g + 1] - p0;
-                }
-            }
-            if (tid < nr) pre0 = epi_load(e, r0 + tid);
         }
         // products of the tile's entries, kPer independent loads per lane
 #pragma unroll
@@ -295,22 +244,26 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
             if (p < p1) prod[p - p0] = static_cast<AT>(ntload(val + p)) * static_cast<AT>(x[ntload(ci + p)]);
         }
         __syncthreads();
-        if (grouped) {
+        if (nr * 2 <= kBlock) {
             // 2^lg lanes per row, lg = floor(log2(kBlock / nr)) capped at 6
+            int lg = 0;
+            while (lg < 6 && (nr << (lg + 1)) <= kBlock) ++lg;
             const int L = 1 << lg, g = tid >> lg, lane = tid & (L - 1);
             AT sum = 0;
-            for (int q = q0[0]; q < q1[0]; q += L) sum += prod[q];
+            int row = r0 + g;
+            if (g < nr) {
+                const int q1 = rp[row + 1] - p0;
+                for (int q = rp[row] - p0 + lane; q < q1; q += L) sum += prod[q];
+            }
             for (int off = L >> 1; off > 0; off >>= 1) sum += shfl_xor_d(sum, off);
-            if (g < nr && lane == 0) epi_row(e, r0 + g, sum, pre0, part);
+            if (g < nr && lane == 0) e.row(row, sum, part);
         } else {
-#pragma unroll
-            for (int k = 0; k < SKK_TILE_ROWS / kBlock; ++k) {
-                const int g = tid + k * kBlock;
-                if (g < nr) {
-                    AT sum = 0;
-                    for (int q = q0[k]; q < q1[k]; ++q) sum += prod[q];
-                    epi_row(e, r0 + g, sum, k == 0 ? pre0 : epi_load(e, r0 + g), part);
-                }
+            for (int g = tid; g < nr; g += kBlock) {
+                const int row = r0 + g;
+                const int q1 = rp[row + 1] - p0;
+                AT sum = 0;
+                for (int q = rp[row] - p0; q < q1; ++q) sum += prod[q];
+                e.row(row, sum, part);
             }
         }
         __syncthreads();   // prod is reused by the next tile
@@ -474,14 +427,14 @@ template <typename T, typename AT> struct EpiPrimalAdaptive {
         tau = static_cast<T>(st->eta / st->omega);
         pending = st->pending;
     }
-    struct Pre { T x, c, l, u; };
-    __device__ Pre load(int j) const { return Pre{x[j], D.c[j], D.l[j], D.u[j]}; }
-    __device__ void row(int j, AT s, const Pre &r, double *p) const
+    __device__ void row(int j, AT s, double *p) const
     {
-        const T xn = clampv(r.x - tau * (r.c + static_cast<T>(s) + (qx ? qx[j] : T(0))), r.l, r.u);
+        const T xj = x[j];
+        const T g = D.c[j] + static_cast<T>(s) + (qx ? qx[j] : T(0));
+        const T xn = clampv(xj - tau * g, D.l[j], D.u[j]);
         xc[j] = xn;
-        if (pending) I.xsum[j] += r.x;
-        const double d = static_cast<double>(xn) - static_cast<double>(r.x);
+        if (pending) I.xsum[j] += xj;
+        const double d = static_cast<double>(xn) - static_cast<double>(xj);
         p[0] += d * d;
     }
     __device__ void end(double *p) const
@@ -509,15 +462,13 @@ template <typename T, typename AT> struct EpiProbe {
         const int cur = st->cur;
         y = k.I.y(cur); yc = k.I.y(cur ^ 1); ax = k.I.ax(st->a0); axc = k.I.ax(st->a2);
     }
-    struct Pre { T y, yc, ax; };
-    __device__ Pre load(int i) const { return Pre{y[i], yc[i], ax[i]}; }
-    __device__ void row(int i, AT s, const Pre &r, double *p) const
+    __device__ void row(int i, AT s, double *p) const
     {
         const T a = static_cast<T>(s);
         axc[i] = a;
-        const double dy = static_cast<double>(r.yc) - static_cast<double>(r.y);
+        const double dy = static_cast<double>(yc[i]) - static_cast<double>(y[i]);
         p[0] += dy * dy;
-        p[1] += dy * (static_cast<double>(a) - static_cast<double>(r.ax));
+        p[1] += dy * (static_cast<double>(a) - static_cast<double>(ax[i]));
     }
     __device__ void end(double *p) const
     {
@@ -627,14 +578,12 @@ template <typename T, typename AT> struct EpiQProbe {
         const int cur = st->cur;
         x = I.x(cur); xc = I.x(cur ^ 1); qx = I.qx(cur); qxc = I.qx(cur ^ 1);
     }
-    struct Pre { T x, xc, qx; };
-    __device__ Pre load(int j) const { return Pre{x[j], xc[j], qx[j]}; }
-    __device__ void row(int j, AT s, const Pre &r, double *p) const
+    __device__ void row(int j, AT s, double *p) const
     {
         const T q = static_cast<T>(s);
         qxc[j] = q;
-        p[0] += (static_cast<double>(r.xc) - static_cast<double>(r.x)) *
-                (static_cast<double>(q) - static_cast<double>(r.qx));
+        p[0] += (static_cast<double>(xc[j]) - static_cast<double>(x[j])) *
+                (static_cast<double>(q) - static_cast<double>(qx[j]));
     }
     __device__ void end(double *p) const
     {
