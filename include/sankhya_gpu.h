@@ -424,7 +424,7 @@ typedef struct sk_gpu_batch_stats {
     double seconds;
     int    block_threads, grid, lds_bytes, converged;
     long long total_iterations;
-    double tune_ms[3];        /* autotuner timings for 64/128/256 threads  */
+    double tune_ms[3];        /* autotuner timings for 128/256/512 threads */
     long long fp32_iterations;
     int    handoffs;          /* instances finished by the FP64 tail        */
     int    grid_fp64, lds_bytes_fp64;   /* launch shape of the FP64 phase   */

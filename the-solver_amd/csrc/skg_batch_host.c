@@ -306,20 +306,31 @@ int sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_opt
         B.params.tail_stall_windows = 0x7fffffff;
     }
     stats->setup_seconds = skg_now() - t0;
-    /* autotune the workgroup size on a two-window run of the whole batch */
+    /* autotune the workgroup size: time four restart windows of a sample of
+       instances spread over the size order, as many as one wave of the
+       grid holds -- long enough that the per-iteration rate, not the
+       start-up (power iteration, first windows), decides */
     bt = o.block_threads;
     if (bt == 0) {
-        const int cand[3] = {64, 128, 256};
+        const int cand[3] = {128, 256, 512};
+        const int sample = count < 4 * skb_cu_count() ? count : 4 * skb_cu_count();
         double best = 1e300;
         skb_batch T = B;
-        T.params.max_iterations = 2 * o.pdlp.restart_period;
+        int *horder = (int *)malloc(sizeof(int) * (size_t)sample);
+        if (!horder) { rc = skg_fail("out of host memory"); goto fail; }
+        for (k = 0; k < sample; ++k) horder[k] = h.order[(int)(((long long)k * count) / sample)];
+        rc = dput(&d, (const void **)&T.order, horder, sizeof(int) * (size_t)sample);
+        free(horder);
+        if (rc) goto fail;
+        T.count = sample;
+        T.params.max_iterations = 4 * o.pdlp.restart_period;
         for (k = 0; k < 3; ++k) {
             void *none = NULL;
             phases tp = ph;
             double ms;
             tp.bt = cand[k];
-            tp.grid32 = grid_for(count, cand[k], lds32);
-            tp.grid64 = grid_for(count, cand[k], lds64);
+            tp.grid32 = grid_for(sample, cand[k], lds32);
+            tp.grid64 = grid_for(sample, cand[k], lds64);
             ms = timed_launch(&T, &tp, &none, 0);
             if (ms < 0.0) { rc = skk_last_launch_error(); goto fail; }
             stats->tune_ms[k] = ms;
