@@ -59,9 +59,15 @@ typedef struct skk_bins {
  * SKK_TILE_ROWS rows.  A product launches min(ntiles, SKK_SPMV_GRID)
  * workgroups; workgroup g takes tiles g, g + grid, ..., so block partials
  * never exceed SKK_SPMV_GRID per slot. */
+#ifndef SKK_TILE_NNZ
 #define SKK_TILE_NNZ 2048
+#endif
+#ifndef SKK_TILE_ROWS
 #define SKK_TILE_ROWS 1024
+#endif
+#ifndef SKK_SPMV_GRID
 #define SKK_SPMV_GRID 4096
+#endif
 
 typedef struct skk_csr {
     int rows, cols, nnz;
