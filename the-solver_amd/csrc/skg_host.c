@@ -348,15 +348,29 @@ int skg_plan_bins(const int *rp, int rows, const skg_tuning *t, int *order, skk_
 
 /* ----------------------------------------------------------- device CSR */
 
+/* Entry cap of the tiles of a matrix with nnz entries: the largest power
+ * of two from SKK_TILE_NNZ down to SKK_TILE_MIN_NNZ that still leaves at
+ * least SKK_MIN_TILES tiles, so that a small matrix spreads over enough
+ * workgroups to occupy the chip while a large one keeps full tiles. */
+int skg_tile_nnz(long long nnz)
+{
+    int t = SKK_TILE_NNZ;
+    while (t > SKK_TILE_MIN_NNZ && nnz / t < SKK_MIN_TILES) t >>= 1;
+    return t;
+}
+
 /* CSR-stream tiles of consecutive rows (greedy, in row order): a tile
- * closes before it would exceed SKK_TILE_NNZ entries or SKK_TILE_ROWS rows;
- * a row longer than SKK_TILE_NNZ is a tile of its own.  At least one
- * (possibly empty) tile, so fused epilogues always run.  Returns the tile
- * count, or -1 when out of memory; *tile_row receives malloc'd offsets. */
-int skg_plan_tiles(const int *rp, int rows, int **tile_row)
+ * closes before it would exceed tile_nnz (<= SKK_TILE_NNZ) entries or
+ * SKK_TILE_ROWS rows; a row longer than tile_nnz is a tile of its own.  At
+ * least one (possibly empty) tile, so fused epilogues always run.  Returns
+ * the tile count, or -1 when out of memory or tile_nnz is out of range;
+ * *tile_row receives malloc'd offsets. */
+int skg_plan_tiles(const int *rp, int rows, int tile_nnz, int **tile_row)
 {
     int cap = rows / 64 + 16, nt = 0, r = 0;
-    int *t = (int *)malloc(sizeof(int) * (size_t)cap);
+    int *t;
+    if (tile_nnz < 1 || tile_nnz > SKK_TILE_NNZ) return -1;
+    t = (int *)malloc(sizeof(int) * (size_t)cap);
     if (!t) return -1;
     t[0] = 0;
     while (r < rows || nt == 0) {
@@ -368,10 +382,10 @@ int skg_plan_tiles(const int *rp, int rows, int **tile_row)
             if (!g) { free(t); return -1; }
             t = g;
         }
-        if (r < rows && rp[r + 1] - rp[r] > SKK_TILE_NNZ) {
+        if (r < rows && rp[r + 1] - rp[r] > tile_nnz) {
             ++r;
         } else {
-            while (r < rows && cnt < SKK_TILE_ROWS && nnz + (rp[r + 1] - rp[r]) <= SKK_TILE_NNZ) {
+            while (r < rows && cnt < SKK_TILE_ROWS && nnz + (rp[r + 1] - rp[r]) <= tile_nnz) {
                 nnz += rp[r + 1] - rp[r];
                 ++r;
                 ++cnt;
@@ -395,7 +409,7 @@ int dcsr_upload(dcsr *d, const hcsr *a, int with_f32)
     int rc, nt, *tiles = NULL;
     memset(d, 0, sizeof(*d));
     d->k.rows = a->rows; d->k.cols = a->cols; d->k.nnz = a->nnz;
-    nt = skg_plan_tiles(a->p, a->rows, &tiles);
+    nt = skg_plan_tiles(a->p, a->rows, skg_tile_nnz(a->nnz), &tiles);
     if (nt < 0) return skg_fail("out of host memory");
     d->k.ntiles = nt;
     d->k.grid = nt < SKK_SPMV_GRID ? nt : SKK_SPMV_GRID;

@@ -50,7 +50,8 @@ typedef struct dcsr {
     skk_csr k;
 } dcsr;
 
-int  skg_plan_tiles(const int *rp, int rows, int **tile_row);
+int  skg_tile_nnz(long long nnz);
+int  skg_plan_tiles(const int *rp, int rows, int tile_nnz, int **tile_row);
 int  dcsr_upload(dcsr *d, const hcsr *a, int with_f32);
 void dcsr_free(dcsr *d);
 

@@ -54,13 +54,21 @@ typedef struct skk_bins {
 } skk_bins;
 
 /* CSR-stream row tiles: tile t covers rows [tile_row[t], tile_row[t+1]),
- * consecutive rows holding at most SKK_TILE_NNZ entries together (a tile
- * of one longer row is reduced by the whole workgroup) and at most
- * SKK_TILE_ROWS rows.  A product launches min(ntiles, SKK_SPMV_GRID)
- * workgroups; workgroup g takes tiles g, g + grid, ..., so block partials
- * never exceed SKK_SPMV_GRID per slot. */
+ * consecutive rows holding at most a per-matrix entry cap together (a tile
+ * of one row longer than SKK_TILE_NNZ is reduced by the whole workgroup)
+ * and at most SKK_TILE_ROWS rows.  The cap is SKK_TILE_NNZ (the LDS
+ * staging size), halved down to SKK_TILE_MIN_NNZ while the matrix would
+ * have fewer than SKK_MIN_TILES tiles.  A product launches
+ * min(ntiles, SKK_SPMV_GRID) workgroups; workgroup g takes tiles g,
+ * g + grid, ..., so block partials never exceed SKK_SPMV_GRID per slot. */
 #ifndef SKK_TILE_NNZ
 #define SKK_TILE_NNZ 2048
+#endif
+#ifndef SKK_TILE_MIN_NNZ
+#define SKK_TILE_MIN_NNZ 256
+#endif
+#ifndef SKK_MIN_TILES
+#define SKK_MIN_TILES 1024
 #endif
 #ifndef SKK_TILE_ROWS
 #define SKK_TILE_ROWS 1024
