@@ -506,6 +506,101 @@ int sk_gpu_relative_kkt_qp(int rows, int cols, int nnz, const int *row_offsets,
                            const double *col_lower, const double *col_upper, const double *x,
                            const double *y, double out[5]);
 
+/* ------------------------------------------------------------------------
+ * LP pipeline of the C++ solver: presolve -> first-order GPU solve of the
+ * reduced model -> postsolve -> original-space primal gate
+ * (src/lp/LpSolver.cpp:127-333 solve_lp with LpMethod::FIRST_ORDER).
+ *
+ * The problem is in the form of src/lp/LpProblem.hpp:15-28: A x + s = rhs
+ * with slack_lower <= s <= slack_upper (so rhs - slack_upper <= A x <=
+ * rhs - slack_lower) and lower <= x <= upper.  Absent bounds are IEEE
+ * +-infinity: presolve and the gate read a finite 1e30 as a finite bound,
+ * as the reference does (the GPU engine reads |b| >= 1e30 as absent).
+ * A is CSR; row_types ('L', 'G', 'E', 'N') is diagnostic and may be NULL
+ * on input.  Presolve and postsolve run on the host (their passes are
+ * sequential by definition: each reduction sees the bounds the previous
+ * one left); the reduced model goes to the GPU engine.
+ * ---------------------------------------------------------------------- */
+typedef struct sk_gpu_lp_problem {  /* src/lp/LpProblem.hpp:15-28 LpProblem */
+    int     rows, cols, nnz;
+    int    *row_ptr;                 /* [rows + 1]                           */
+    int    *col_idx;                 /* [nnz]                                */
+    double *values;                  /* [nnz]                                */
+    double *obj;                     /* [cols]                               */
+    double *rhs;                     /* [rows]                               */
+    char   *row_types;               /* [rows] or NULL                       */
+    double *lower, *upper;           /* [cols]                               */
+    double *slack_lower, *slack_upper; /* [rows]                             */
+} sk_gpu_lp_problem;
+
+enum { SK_GPU_PRESOLVE_OK = 0, SK_GPU_PRESOLVE_INFEASIBLE = 1, SK_GPU_PRESOLVE_UNBOUNDED = 2 };
+
+typedef struct sk_gpu_presolve_result {  /* src/lp/Presolve.hpp:12-55 PresolveResult */
+    int    status;                   /* SK_GPU_PRESOLVE_*                    */
+    const char *reason;              /* which test concluded, or NULL        */
+    int    reason_row, reason_col, reason_pass;
+    sk_gpu_lp_problem reduced;       /* owned by the result                  */
+    int   *kept_columns;             /* reduced column k is original kept_columns[k] */
+    int   *kept_rows;                /* reduced row r is original kept_rows[r] */
+    double *fixed_value;             /* [original_n_cols]: value of a removed column */
+    char  *column_removed;           /* [original_n_cols]                    */
+    int    original_n_rows, original_n_cols;
+    int    removed_rows, removed_cols;
+} sk_gpu_presolve_result;
+
+/* src/lp/Presolve.hpp:92 presolve (max_passes <= 0 means 20).  Returns 0
+ * (read out->status), -1 on invalid input or allocation failure.  Release
+ * with sk_gpu_presolve_free. */
+int  sk_gpu_presolve(const sk_gpu_lp_problem *problem, int max_passes, sk_gpu_presolve_result *out);
+/* src/lp/Presolve.hpp:95 postsolve: x[original_n_cols] from the reduced
+ * solution (reduced_len entries; extra kept columns read as 0). */
+int  sk_gpu_postsolve(const sk_gpu_presolve_result *result, const double *reduced_x, int reduced_len,
+                      double *x);
+void sk_gpu_presolve_free(sk_gpu_presolve_result *result);
+
+/* An sk_model (continuous LP) in LpProblem form: rows with a finite upper
+ * bound anchor rhs there (slack in [0, upper - lower]), rows with only a
+ * lower bound anchor it at the lower bound (slack in [-inf, 0]), free rows
+ * get rhs 0; the +-1e30 sentinel becomes IEEE infinity.  Owned arrays:
+ * release with sk_gpu_lp_problem_free. */
+int  sk_gpu_lp_problem_from_model(const sk_model *m, sk_gpu_lp_problem *p);
+void sk_gpu_lp_problem_free(sk_gpu_lp_problem *p);
+
+enum {  /* src/lp/Simplex.hpp:19 LpStatus */
+    SK_GPU_LP_OPTIMAL = 0, SK_GPU_LP_INFEASIBLE = 1, SK_GPU_LP_UNBOUNDED = 2,
+    SK_GPU_LP_ITERATION_LIMIT = 3, SK_GPU_LP_NUMERICAL_FAILURE = 4
+};
+
+typedef struct sk_gpu_lp_solver_options {  /* src/lp/LpSolver.hpp:46-106 (FIRST_ORDER) */
+    int    use_presolve;             /* 1                                    */
+    int    presolve_passes;          /* 20                                   */
+    sk_gpu_pdlp_params pdlp;
+} sk_gpu_lp_solver_options;
+
+typedef struct sk_gpu_lp_solution {  /* src/lp/LpSolver.hpp:117-165 LpSolution */
+    int    status;                   /* SK_GPU_LP_*                          */
+    int    has_x;                    /* x[] was written (OPTIMAL or a failed gate) */
+    double objective_value;          /* original costs . postsolved x        */
+    int    iterations;
+    double presolve_seconds;
+    double solve_seconds;
+    int    presolve_removed_rows;
+    int    presolve_removed_cols;
+    int    used_first_order;
+    double primal_residual;          /* original-space gate value            */
+    sk_gpu_pdlp_stats pdlp;
+} sk_gpu_lp_solution;
+
+void sk_gpu_lp_solver_options_default(sk_gpu_lp_solver_options *o);
+/* src/lp/LpSolver.cpp:127 solve_lp with method FIRST_ORDER.  x[cols]
+ * receives the solution in the original column space.  Returns 0 (read
+ * solution->status) or -1 on invalid input / device failure. */
+int  sk_gpu_solve_lp(const sk_gpu_lp_problem *problem, const sk_gpu_lp_solver_options *options,
+                     double *x, sk_gpu_lp_solution *solution);
+/* src/lp/LpSolver.cpp:23-50: max(row violation / (1 + ||rhs||_inf), bound
+ * violation) of x against the problem, rows summed in CSR order. */
+double sk_gpu_lp_primal_residual(const sk_gpu_lp_problem *problem, const double *x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,7 +3,9 @@
 Imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg,
 always as the checker.  ``liboracle.so`` is the FP64 C restatement
 (sk_oracle.c); ``_ref/libsankhya_ref.so`` is the reference C core compiled
-from its own sources (oracle/Makefile), present only where it was built.
+from its own sources (oracle/Makefile) and ``_ref/libsihps_presolve_ref.so``
+the reference's C++ presolve and MPS reader behind ref_presolve.cpp, both
+present only where they were built.
 """
 from __future__ import annotations
 
@@ -19,6 +21,7 @@ import sankhya_gpu as sg  # noqa: E402  (ctypes structure definitions only)
 
 ORACLE_LIB = HERE / "liboracle.so"
 REF_LIB = HERE / "_ref" / "libsankhya_ref.so"
+PRESOLVE_REF_LIB = HERE / "_ref" / "libsihps_presolve_ref.so"
 
 
 class OrcCsr(C.Structure):
@@ -28,6 +31,7 @@ class OrcCsr(C.Structure):
 
 _orc = None
 _ref = None
+_psref = None
 
 
 def _decl(lib, name, restype, argtypes):
@@ -228,3 +232,57 @@ def first_order_pdlp(model, params):
     rc = orc.orc_first_order(A.ref(), sg.dptr(c), sg.dptr(l), sg.dptr(u), sg.dptr(rl), sg.dptr(ru),
                              C.byref(params), sg.dptr(x), sg.dptr(y), C.byref(st))
     return rc, x[:model.ncol], y[:model.nrow], st
+
+
+def load_presolve_ref():
+    """The reference's C++ presolve (ref_presolve.cpp), or None where
+    oracle/_ref was not built."""
+    global _psref
+    if _psref is None and PRESOLVE_REF_LIB.exists():
+        lib = C.CDLL(str(PRESOLVE_REF_LIB))
+        LP, PR = C.POINTER(sg.SkLpProblem), C.POINTER(sg.SkPresolveResult)
+        _decl(lib, "refps_read_mps", C.c_int, [C.c_char_p, LP])
+        _decl(lib, "refps_presolve", C.c_int, [LP, C.c_int, PR])
+        _decl(lib, "refps_postsolve", C.c_int, [LP, C.c_int, sg.c_double_p, C.c_int, sg.c_double_p])
+        _decl(lib, "refps_problem_free", None, [LP])
+        _decl(lib, "refps_result_free", None, [PR])
+        _psref = lib
+    return _psref
+
+
+def ref_lp_from_mps(path):
+    """The reference's read_mps_file + lp_problem_from_mps as an LpProblem
+    (None where the reference presolve library was not built)."""
+    lib = load_presolve_ref()
+    if lib is None:
+        return None
+    s = sg.SkLpProblem()
+    if lib.refps_read_mps(str(path).encode(), C.byref(s)) != 0:
+        raise RuntimeError(f"reference MPS reader rejected {path}")
+    out = sg.LpProblem.from_struct(s)
+    lib.refps_problem_free(C.byref(s))
+    return out
+
+
+def ref_presolve(problem, max_passes=20):
+    """The reference's presolve on an LpProblem, as sankhya_gpu.presolve's dict."""
+    lib = load_presolve_ref()
+    if lib is None:
+        return None
+    s = problem.struct()
+    r = sg.SkPresolveResult()
+    if lib.refps_presolve(C.byref(s), max_passes, C.byref(r)) != 0:
+        raise RuntimeError("reference presolve raised")
+    return sg.presolve_result_dict(r, lib.refps_result_free)
+
+
+def ref_postsolve(problem, reduced_x, max_passes=20):
+    lib = load_presolve_ref()
+    if lib is None:
+        return None
+    s = problem.struct()
+    rx = np.ascontiguousarray(reduced_x, np.float64)
+    x = np.zeros(max(problem.cols, 1))
+    if lib.refps_postsolve(C.byref(s), max_passes, sg.dptr(rx) if rx.size else None, rx.size, sg.dptr(x)) != 0:
+        raise RuntimeError("reference postsolve raised")
+    return x[:problem.cols]

@@ -298,6 +298,16 @@ def load_gpu():
         _declare(lib, "sk_gpu_solution_free", None, [S])
         _declare(lib, "sk_gpu_options_default", None, [C.POINTER(SkOptions)])
         _declare(lib, "sk_gpu_result_name", C.c_char_p, [C.c_int])
+        LP, PR = C.POINTER(SkLpProblem), C.POINTER(SkPresolveResult)
+        _declare(lib, "sk_gpu_presolve", C.c_int, [LP, C.c_int, PR])
+        _declare(lib, "sk_gpu_postsolve", C.c_int, [PR, c_double_p, C.c_int, c_double_p])
+        _declare(lib, "sk_gpu_presolve_free", None, [PR])
+        _declare(lib, "sk_gpu_lp_problem_from_model", C.c_int, [M, LP])
+        _declare(lib, "sk_gpu_lp_problem_free", None, [LP])
+        _declare(lib, "sk_gpu_lp_solver_options_default", None, [C.POINTER(LpSolverOptions)])
+        _declare(lib, "sk_gpu_solve_lp", C.c_int, [LP, C.POINTER(LpSolverOptions), c_double_p,
+                                                   C.POINTER(LpSolution)])
+        _declare(lib, "sk_gpu_lp_primal_residual", C.c_double, [LP, c_double_p])
         _declare(lib, "sk_gpu_status_name", C.c_char_p, [C.c_int])
         _gpu = lib
     return _gpu
@@ -571,3 +581,166 @@ class Mlp:
         if getattr(self, "h", None):
             self.lib.sk_gpu_mlp_destroy(self.h)
             self.h = C.c_void_p()
+
+
+# ---------------------------------------------------------------- LP pipeline
+
+class SkLpProblem(C.Structure):
+    """sk_gpu_lp_problem: the reference's LpProblem (src/lp/LpProblem.hpp:15-28)."""
+    _fields_ = [("rows", C.c_int), ("cols", C.c_int), ("nnz", C.c_int),
+                ("row_ptr", c_int_p), ("col_idx", c_int_p), ("values", c_double_p),
+                ("obj", c_double_p), ("rhs", c_double_p), ("row_types", C.POINTER(C.c_char)),
+                ("lower", c_double_p), ("upper", c_double_p),
+                ("slack_lower", c_double_p), ("slack_upper", c_double_p)]
+
+
+class SkPresolveResult(C.Structure):
+    """sk_gpu_presolve_result: PresolveResult (src/lp/Presolve.hpp:12-55)."""
+    _fields_ = [("status", C.c_int), ("reason", C.c_char_p), ("reason_row", C.c_int),
+                ("reason_col", C.c_int), ("reason_pass", C.c_int), ("reduced", SkLpProblem),
+                ("kept_columns", c_int_p), ("kept_rows", c_int_p), ("fixed_value", c_double_p),
+                ("column_removed", C.POINTER(C.c_char)), ("original_n_rows", C.c_int),
+                ("original_n_cols", C.c_int), ("removed_rows", C.c_int), ("removed_cols", C.c_int)]
+
+
+class LpSolverOptions(C.Structure):
+    _fields_ = [("use_presolve", C.c_int), ("presolve_passes", C.c_int), ("pdlp", PdlpParams)]
+
+
+class LpSolution(C.Structure):
+    _fields_ = [("status", C.c_int), ("has_x", C.c_int), ("objective_value", C.c_double),
+                ("iterations", C.c_int), ("presolve_seconds", C.c_double),
+                ("solve_seconds", C.c_double), ("presolve_removed_rows", C.c_int),
+                ("presolve_removed_cols", C.c_int), ("used_first_order", C.c_int),
+                ("primal_residual", C.c_double), ("pdlp", PdlpStats)]
+
+
+PRESOLVE_OK, PRESOLVE_INFEASIBLE, PRESOLVE_UNBOUNDED = 0, 1, 2
+LP_OPTIMAL, LP_INFEASIBLE, LP_UNBOUNDED, LP_ITERATION_LIMIT, LP_NUMERICAL_FAILURE = 0, 1, 2, 3, 4
+LP_STATUS_NAMES = ["optimal", "infeasible", "unbounded", "iteration_limit", "numerical_failure"]
+_LP_FIELDS = ("row_ptr", "col_idx", "values", "obj", "rhs", "row_types", "lower", "upper",
+              "slack_lower", "slack_upper")
+_LP_DTYPES = {"row_ptr": np.int32, "col_idx": np.int32, "row_types": np.uint8}
+
+
+class LpProblem:
+    """An LpProblem held in numpy arrays: A x + s = rhs, slack_lower <= s <=
+    slack_upper, lower <= x <= upper, A in CSR, absent bounds IEEE inf."""
+
+    def __init__(self, rows, cols, row_ptr, col_idx, values, obj, rhs, lower, upper,
+                 slack_lower, slack_upper, row_types=None):
+        self.rows, self.cols = int(rows), int(cols)
+        a = {"row_ptr": row_ptr, "col_idx": col_idx, "values": values, "obj": obj, "rhs": rhs,
+             "lower": lower, "upper": upper, "slack_lower": slack_lower, "slack_upper": slack_upper,
+             "row_types": row_types if row_types is not None else np.full(self.rows, ord("L"), np.uint8)}
+        for k, v in a.items():
+            if k == "row_types" and isinstance(v, (bytes, str)):
+                v = np.frombuffer(v.encode() if isinstance(v, str) else v, np.uint8)
+            setattr(self, k, np.ascontiguousarray(v, _LP_DTYPES.get(k, np.float64)))
+        self.nnz = int(self.row_ptr[-1]) if len(self.row_ptr) else 0
+
+    @classmethod
+    def from_struct(cls, s):
+        """Copy an SkLpProblem (library- or reference-owned) into numpy arrays."""
+        m, n, nnz = s.rows, s.cols, s.nnz
+        def arr(ptr, k, dt):
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(dt)), shape=(k,)).copy() if k and ptr else np.zeros(0, dt)
+        return cls(m, n, arr(s.row_ptr, m + 1, C.c_int) if s.row_ptr else np.zeros(m + 1, np.int32),
+                   arr(s.col_idx, nnz, C.c_int), arr(s.values, nnz, C.c_double), arr(s.obj, n, C.c_double),
+                   arr(s.rhs, m, C.c_double), arr(s.lower, n, C.c_double), arr(s.upper, n, C.c_double),
+                   arr(s.slack_lower, m, C.c_double), arr(s.slack_upper, m, C.c_double),
+                   arr(s.row_types, m, C.c_uint8) if s.row_types else None)
+
+    @classmethod
+    def from_model(cls, model):
+        """sk_gpu_lp_problem_from_model of a Model (continuous LP)."""
+        lib = load_gpu()
+        s = SkLpProblem()
+        if lib.sk_gpu_lp_problem_from_model(model.ptr(), C.byref(s)) != 0:
+            raise RuntimeError(last_error())
+        out = cls.from_struct(s)
+        lib.sk_gpu_lp_problem_free(C.byref(s))
+        return out
+
+    def struct(self):
+        s = SkLpProblem(self.rows, self.cols, self.nnz)
+        for k in _LP_FIELDS:
+            v = getattr(self, k)
+            ct = {"row_ptr": C.c_int, "col_idx": C.c_int, "row_types": C.c_char}.get(k, C.c_double)
+            setattr(s, k, v.ctypes.data_as(C.POINTER(ct)) if v.size else None)
+        return s
+
+    def arrays(self):
+        return {k: getattr(self, k) for k in _LP_FIELDS} | {"rows": self.rows, "cols": self.cols}
+
+
+def presolve_result_dict(r, free=None):
+    """Every field of an SkPresolveResult as numpy arrays (then freed with
+    `free`, the owning library's release function)."""
+    n = r.original_n_cols
+    ok = r.status == PRESOLVE_OK
+    kc = r.original_n_cols - r.removed_cols if ok else 0
+    kr = r.original_n_rows - r.removed_rows if ok else 0
+    def arr(ptr, k, dt):
+        return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(dt)), shape=(k,)).copy() if k and ptr else np.zeros(0, dt)
+    d = {"status": r.status, "reason": r.reason.decode() if r.reason else None,
+         "reason_row": r.reason_row, "reason_col": r.reason_col, "reason_pass": r.reason_pass,
+         "original_n_rows": r.original_n_rows, "original_n_cols": n,
+         "removed_rows": r.removed_rows, "removed_cols": r.removed_cols,
+         "kept_columns": arr(r.kept_columns, kc, C.c_int), "kept_rows": arr(r.kept_rows, kr, C.c_int),
+         "fixed_value": arr(r.fixed_value, n, C.c_double),
+         "column_removed": arr(r.column_removed, n, C.c_uint8),
+         "reduced": LpProblem.from_struct(r.reduced) if ok else None}
+    if free is not None:
+        free(C.byref(r))
+    return d
+
+
+def presolve(problem: LpProblem, max_passes=20):
+    """sk_gpu_presolve (src/lp/Presolve.hpp:92) as a dict of arrays."""
+    lib = load_gpu()
+    s = problem.struct()
+    r = SkPresolveResult()
+    if lib.sk_gpu_presolve(C.byref(s), max_passes, C.byref(r)) != 0:
+        raise RuntimeError(last_error())
+    return presolve_result_dict(r, lib.sk_gpu_presolve_free)
+
+
+def postsolve(problem: LpProblem, reduced_x, max_passes=20):
+    """sk_gpu_postsolve of sk_gpu_presolve(problem) applied to reduced_x."""
+    lib = load_gpu()
+    s = problem.struct()
+    r = SkPresolveResult()
+    if lib.sk_gpu_presolve(C.byref(s), max_passes, C.byref(r)) != 0:
+        raise RuntimeError(last_error())
+    rx = np.ascontiguousarray(reduced_x, np.float64)
+    x = np.zeros(max(problem.cols, 1))
+    rc = lib.sk_gpu_postsolve(C.byref(r), dptr(rx) if rx.size else None, rx.size, dptr(x))
+    lib.sk_gpu_presolve_free(C.byref(r))
+    if rc != 0:
+        raise RuntimeError(last_error())
+    return x[:problem.cols]
+
+
+def lp_solver_options(use_presolve=1, presolve_passes=20, **pdlp) -> LpSolverOptions:
+    return LpSolverOptions(use_presolve=use_presolve, presolve_passes=presolve_passes,
+                           pdlp=default_pdlp_params(**pdlp))
+
+
+def solve_lp(problem: LpProblem, options: LpSolverOptions | None = None):
+    """sk_gpu_solve_lp (src/lp/LpSolver.cpp:127, FIRST_ORDER): (solution, x)."""
+    lib = load_gpu()
+    s = problem.struct()
+    o = options if options is not None else lp_solver_options()
+    x = np.zeros(max(problem.cols, 1))
+    sol = LpSolution()
+    if lib.sk_gpu_solve_lp(C.byref(s), C.byref(o), dptr(x), C.byref(sol)) != 0:
+        raise RuntimeError(last_error())
+    return sol, x[:problem.cols]
+
+
+def primal_residual(problem: LpProblem, x):
+    lib = load_gpu()
+    s = problem.struct()
+    xs = np.ascontiguousarray(x, np.float64)
+    return lib.sk_gpu_lp_primal_residual(C.byref(s), dptr(xs) if xs.size else None)
