@@ -10,8 +10,11 @@
  * relaxation of every derived bound, 1e-11 to call a column fixed.  The
  * reduced model is emitted in the reference's order and row encoding, so
  * it is the reference's bit for bit.  Presolve stays on the host: every
- * reduction reads the bounds the previous one left (a Gauss-Seidel sweep),
- * and each pass is a few O(nnz) scans, small next to the solve.
+ * reduction reads the bounds the previous one left (a Gauss-Seidel sweep).
+ * A row whose tests or propagation changed nothing, and none of whose
+ * inputs changed since, is skipped: the result would again be no change,
+ * so the outcome is unchanged while later passes touch only the rows near
+ * the last reductions.
  *
  * sk_gpu_solve_lp is src/lp/LpSolver.cpp:127-333 with FIRST_ORDER:
  * presolve, the GPU PDLP solve (sk_gpu_pdlp_solve: Ruiz + Pock-Chambolle
@@ -31,6 +34,8 @@
 #define FIX_TOL 1e-11
 #define FINAL_PRIMAL_TOL 1e-6      /* src/lp/LpSolver.cpp:15 */
 
+typedef struct bounds { double lo, hi; } bounds;
+
 /* Working state of one presolve call.  Rows are lo <= A x <= hi. */
 typedef struct ps {
     int m, n;
@@ -38,8 +43,16 @@ typedef struct ps {
     const double *rv;
     hcsr csc;                      /* CSR of A^T: the columns of A          */
     const double *obj;
-    double *row_lo, *row_hi, *col_lo, *col_hi;
+    double *row_lo, *row_hi;
+    bounds *cb;                    /* column bounds, lo and hi side by side */
     char *row_active, *col_active;
+    /* Change stamps.  Every change to a column (bounds, or fixed) takes the
+     * next stamp; rt_seen[i] / pg_seen[i] is the first stamp after the start
+     * of row i's last tests / propagation that changed nothing (0: none).
+     * When no column of the row has a newer stamp, nothing the row reads
+     * has changed (its bounds change only when one of its columns is
+     * fixed), so redoing it would again change nothing and it is skipped. */
+    unsigned *col_stamp, *rt_seen, *pg_seen, stamp;
     int *row_count, *col_count;
     double *fixed;
     char *removed;
@@ -56,6 +69,29 @@ typedef struct activity { double fmin, fmax; int nmin_inf, nmax_inf; } activity;
 static double act_min(const activity *a) { return a->nmin_inf > 0 ? -INFINITY : a->fmin; }
 static double act_max(const activity *a) { return a->nmax_inf > 0 ? INFINITY : a->fmax; }
 
+/* Column j changed.  Should the stamps run out, every row is marked as
+ * needing work again and counting restarts. */
+static void touch_column(ps *P, int j)
+{
+    if (P->stamp == 0xfffffff0u) {
+        memset(P->rt_seen, 0, sizeof(unsigned) * ((size_t)P->m + 1));
+        memset(P->pg_seen, 0, sizeof(unsigned) * ((size_t)P->m + 1));
+        memset(P->col_stamp, 0, sizeof(unsigned) * ((size_t)P->n + 1));
+        P->stamp = 1;
+    }
+    P->col_stamp[j] = ++P->stamp;
+}
+
+/* Whether row i is unchanged since `seen` (see the stamps above). */
+static int row_unchanged(const ps *P, int i, unsigned seen)
+{
+    int k;
+    if (seen == 0) return 0;
+    for (k = P->rp[i]; k < P->rp[i + 1]; ++k)
+        if (P->rv[k] != 0.0 && P->col_stamp[P->rj[k]] >= seen) return 0;
+    return 1;
+}
+
 /* Presolve.cpp:111-130: substitute the value into every active row. */
 static void fix_column(ps *P, int j, double value)
 {
@@ -64,6 +100,7 @@ static void fix_column(ps *P, int j, double value)
     P->col_active[j] = 0;
     P->removed[j] = 1;
     P->fixed[j] = value;
+    touch_column(P, j);
     for (k = P->csc.p[j]; k < P->csc.p[j + 1]; ++k) {
         const int i = P->csc.j[k];
         const double a = P->csc.v[k];
@@ -98,8 +135,8 @@ static activity row_activity(const ps *P, int i)
         const double a = P->rv[k];
         double mn, mx;
         if (a == 0.0 || !P->col_active[j]) continue;
-        mn = a > 0.0 ? P->col_lo[j] : P->col_hi[j];
-        mx = a > 0.0 ? P->col_hi[j] : P->col_lo[j];
+        mn = a > 0.0 ? P->cb[j].lo : P->cb[j].hi;
+        mx = a > 0.0 ? P->cb[j].hi : P->cb[j].lo;
         if (isfinite(mn)) act.fmin += a * mn; else ++act.nmin_inf;
         if (isfinite(mx)) act.fmax += a * mx; else ++act.nmax_inf;
     }
@@ -114,10 +151,11 @@ static int tighten_upper(ps *P, int j, double limit)
     double relaxed, cur;
     if (!isfinite(limit)) return 0;
     relaxed = limit + BOUND_RELAX * (1.0 + fabs(limit));
-    if (relaxed < P->col_lo[j]) return 0;
-    cur = P->col_hi[j];
+    if (relaxed < P->cb[j].lo) return 0;
+    cur = P->cb[j].hi;
     if (isfinite(cur) && !(relaxed < cur - BOUND_IMPROVE_TOL * (1.0 + fabs(cur)))) return 0;
-    P->col_hi[j] = relaxed;
+    P->cb[j].hi = relaxed;
+    touch_column(P, j);
     return 1;
 }
 
@@ -126,10 +164,11 @@ static int tighten_lower(ps *P, int j, double limit)
     double relaxed, cur;
     if (!isfinite(limit)) return 0;
     relaxed = limit - BOUND_RELAX * (1.0 + fabs(limit));
-    if (relaxed > P->col_hi[j]) return 0;
-    cur = P->col_lo[j];
+    if (relaxed > P->cb[j].hi) return 0;
+    cur = P->cb[j].lo;
     if (isfinite(cur) && !(relaxed > cur + BOUND_IMPROVE_TOL * (1.0 + fabs(cur)))) return 0;
-    P->col_lo[j] = relaxed;
+    P->cb[j].lo = relaxed;
+    touch_column(P, j);
     return 1;
 }
 
@@ -149,7 +188,7 @@ static int fixed_columns(ps *P)
 {
     int j, changed = 0;
     for (j = 0; j < P->n; ++j) {
-        const double lo = P->col_lo[j], hi = P->col_hi[j];
+        const double lo = P->cb[j].lo, hi = P->cb[j].hi;
         if (!P->col_active[j]) continue;
         if (lo > hi + INFEAS_TOL * (1.0 + fabs(hi))) {
             fail(P, "column bounds crossed", -1, j);
@@ -169,7 +208,7 @@ static int empty_columns(ps *P)
 {
     int j, changed = 0;
     for (j = 0; j < P->n; ++j) {
-        const double lo = P->col_lo[j], hi = P->col_hi[j], c = P->obj[j];
+        const double lo = P->cb[j].lo, hi = P->cb[j].hi, c = P->obj[j];
         double v;
         if (!P->col_active[j] || P->col_count[j] != 0) continue;
         if (c > 0.0) {
@@ -197,7 +236,7 @@ static void force_row(ps *P, int i, int at_upper)
         const double a = P->rv[k];
         double pin;
         if (a == 0.0 || !P->col_active[j]) continue;
-        pin = (a > 0.0) == (at_upper != 0) ? P->col_lo[j] : P->col_hi[j];
+        pin = (a > 0.0) == (at_upper != 0) ? P->cb[j].lo : P->cb[j].hi;
         if (!isfinite(pin)) continue;
         P->buf_j[cnt] = j;
         P->buf_v[cnt++] = pin;
@@ -225,8 +264,8 @@ static int singleton_row(ps *P, int i)
     if (isfinite(hi_b)) ihi = hi_b / a;
     took_lo = tighten_lower(P, j, ilo);
     took_hi = tighten_upper(P, j, ihi);
-    if (!(took_lo || !isfinite(ilo) || ilo <= P->col_lo[j] + BOUND_RELAX * (1.0 + fabs(ilo)))) return 0;
-    if (!(took_hi || !isfinite(ihi) || ihi >= P->col_hi[j] - BOUND_RELAX * (1.0 + fabs(ihi)))) return 0;
+    if (!(took_lo || !isfinite(ilo) || ilo <= P->cb[j].lo + BOUND_RELAX * (1.0 + fabs(ilo)))) return 0;
+    if (!(took_hi || !isfinite(ihi) || ihi >= P->cb[j].hi - BOUND_RELAX * (1.0 + fabs(ihi)))) return 0;
     drop_row(P, i);
     return 1;
 }
@@ -239,7 +278,8 @@ static int row_tests(ps *P)
         double lo, hi, scale, ascale, mn, mx;
         activity act;
         int at_upper, at_lower;
-        if (!P->row_active[i]) continue;
+        if (!P->row_active[i] || row_unchanged(P, i, P->rt_seen[i])) continue;
+        P->rt_seen[i] = P->stamp + 1;   /* stale again if this test changes a column */
         lo = P->row_lo[i];
         hi = P->row_hi[i];
         scale = 1.0 + fmax(isfinite(lo) ? fabs(lo) : 0.0, isfinite(hi) ? fabs(hi) : 0.0);
@@ -288,15 +328,16 @@ static int propagate(ps *P)
         activity act;
         int nl = 0, nu = 0;
         const double lo = P->row_lo[i], hi = P->row_hi[i];
-        if (!P->row_active[i]) continue;
+        if (!P->row_active[i] || row_unchanged(P, i, P->pg_seen[i])) continue;
+        P->pg_seen[i] = P->stamp + 1;   /* stale again if this row moves a bound */
         act = row_activity(P, i);
         for (k = P->rp[i]; k < P->rp[i + 1]; ++k) {
             const int j = P->rj[k];
             const double a = P->rv[k];
             double mnb, mxb, rest_min = -INFINITY, rest_max = INFINITY;
             if (a == 0.0 || !P->col_active[j]) continue;
-            mnb = a > 0.0 ? P->col_lo[j] : P->col_hi[j];
-            mxb = a > 0.0 ? P->col_hi[j] : P->col_lo[j];
+            mnb = a > 0.0 ? P->cb[j].lo : P->cb[j].hi;
+            mxb = a > 0.0 ? P->cb[j].hi : P->cb[j].lo;
             if (act.nmin_inf == 0) rest_min = act.fmin - a * mnb;
             else if (act.nmin_inf == 1 && !isfinite(mnb)) rest_min = act.fmin;
             if (act.nmax_inf == 0) rest_max = act.fmax - a * mxb;
@@ -428,8 +469,9 @@ void sk_gpu_presolve_free(sk_gpu_presolve_result *r)
 static void ps_free(ps *P)
 {
     hcsr_free(&P->csc);
-    free(P->row_lo); free(P->row_hi); free(P->col_lo); free(P->col_hi);
+    free(P->row_lo); free(P->row_hi); free(P->cb);
     free(P->row_active); free(P->col_active); free(P->row_count); free(P->col_count);
+    free(P->col_stamp); free(P->rt_seen); free(P->pg_seen);
     free(P->buf_j); free(P->low_j); free(P->up_j); free(P->buf_v); free(P->low_v); free(P->up_v);
 }
 
@@ -491,11 +533,14 @@ int sk_gpu_presolve(const sk_gpu_lp_problem *problem, int max_passes, sk_gpu_pre
     for (i = 0; i < P.m; ++i) if (P.rp[i + 1] - P.rp[i] > maxrow) maxrow = P.rp[i + 1] - P.rp[i];
     P.row_lo = (double *)malloc(sizeof(double) * ((size_t)P.m + 1));
     P.row_hi = (double *)malloc(sizeof(double) * ((size_t)P.m + 1));
-    P.col_lo = (double *)malloc(sizeof(double) * ((size_t)P.n + 1));
-    P.col_hi = (double *)malloc(sizeof(double) * ((size_t)P.n + 1));
+    P.cb = (bounds *)malloc(sizeof(bounds) * ((size_t)P.n + 1));
     P.row_active = (char *)malloc((size_t)P.m + 1);
     P.col_active = (char *)malloc((size_t)P.n + 1);
     P.row_count = (int *)calloc((size_t)P.m + 1, sizeof(int));
+    P.rt_seen = (unsigned *)calloc((size_t)P.m + 1, sizeof(unsigned));
+    P.pg_seen = (unsigned *)calloc((size_t)P.m + 1, sizeof(unsigned));
+    P.col_stamp = (unsigned *)calloc((size_t)P.n + 1, sizeof(unsigned));
+    P.stamp = 1;
     P.col_count = (int *)calloc((size_t)P.n + 1, sizeof(int));
     P.buf_j = (int *)malloc(sizeof(int) * (size_t)maxrow);
     P.buf_v = (double *)malloc(sizeof(double) * (size_t)maxrow);
@@ -506,8 +551,8 @@ int sk_gpu_presolve(const sk_gpu_lp_problem *problem, int max_passes, sk_gpu_pre
     out->fixed_value = (double *)calloc((size_t)P.n + 1, sizeof(double));
     out->column_removed = (char *)calloc((size_t)P.n + 1, 1);
     newc = (int *)malloc(sizeof(int) * ((size_t)P.n + 1));
-    if (!P.row_lo || !P.row_hi || !P.col_lo || !P.col_hi || !P.row_active || !P.col_active ||
-        !P.row_count || !P.col_count || !P.buf_j || !P.buf_v || !P.low_j || !P.low_v || !P.up_j ||
+    if (!P.row_lo || !P.row_hi || !P.cb || !P.row_active || !P.col_active ||
+        !P.row_count || !P.col_count || !P.rt_seen || !P.pg_seen || !P.col_stamp || !P.buf_j || !P.buf_v || !P.low_j || !P.low_v || !P.up_j ||
         !P.up_v || !out->fixed_value || !out->column_removed || !newc || hcsr_transpose(&A, &P.csc)) {
         skg_fail("out of host memory");
         goto done;
@@ -520,8 +565,8 @@ int sk_gpu_presolve(const sk_gpu_lp_problem *problem, int max_passes, sk_gpu_pre
         P.row_active[i] = 1;
     }
     for (j = 0; j < P.n; ++j) {
-        P.col_lo[j] = problem->lower[j];
-        P.col_hi[j] = problem->upper[j];
+        P.cb[j].lo = problem->lower[j];
+        P.cb[j].hi = problem->upper[j];
         P.col_active[j] = 1;
     }
     for (i = 0; i < P.m; ++i)
@@ -592,8 +637,8 @@ int sk_gpu_presolve(const sk_gpu_lp_problem *problem, int max_passes, sk_gpu_pre
         for (k = 0; k < rn; ++k) {
             const int o = out->kept_columns[k];
             R->obj[k] = problem->obj[o];
-            R->lower[k] = P.col_lo[o];
-            R->upper[k] = P.col_hi[o];
+            R->lower[k] = P.cb[o].lo;
+            R->upper[k] = P.cb[o].hi;
         }
     }
     out->status = SK_GPU_PRESOLVE_OK;
