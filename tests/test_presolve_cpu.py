@@ -247,3 +247,33 @@ def test_postsolve_matches_reference():
             continue
         rx = rng.normal(size=len(r["kept_columns"]))
         assert sg.postsolve(p, rx).tobytes() == so.ref_postsolve(p, rx).tobytes()
+
+
+# -------------------------------------------------- pipeline paths that never reach the device
+
+def test_solve_lp_ends_at_presolve_verdicts():
+    """LpSolver.cpp:138-145: INFEASIBLE / UNBOUNDED from presolve end the
+    solve with no solution."""
+    inf = pc.lp(2, 1, [(0, 0, 1.0), (1, 0, 1.0)], [0.0], [3.0, 5.0], "LG", [0.0], [INF])
+    unb = pc.lp(1, 2, [(0, 0, 1.0)], [0.0, -1.0], [1.0], "L", [0.0, 0.0], [INF, INF])
+    for p, status in ((inf, sg.LP_INFEASIBLE), (unb, sg.LP_UNBOUNDED)):
+        sol, _ = sg.solve_lp(p)
+        assert sol.status == status and not sol.has_x and not sol.used_first_order
+
+
+def test_solve_lp_fully_presolved_model():
+    """LpSolver.cpp:157-164: nothing left to optimise is OPTIMAL once the
+    postsolved point passes the original-space gate."""
+    p = pc.lp(1, 2, [(0, 0, 1.0), (0, 1, 1.0)], [0.0, 1.0], [10.0], "E", [4.0, 0.0], [4.0, INF])
+    sol, x = sg.solve_lp(p)
+    assert sol.status == sg.LP_OPTIMAL and sol.has_x and not sol.used_first_order
+    assert (sol.presolve_removed_rows, sol.presolve_removed_cols) == (1, 2)
+    assert x[0] == 4.0 and abs(x[1] - 6.0) < 1e-8 and sol.objective_value == x[1]
+    assert sol.primal_residual == sg.primal_residual(p, x) <= 1e-6
+
+
+def test_solve_lp_rejects_invalid_input():
+    p = pc.tiny_lp()
+    p.row_ptr[1] = 5
+    with pytest.raises(RuntimeError):
+        sg.solve_lp(p)

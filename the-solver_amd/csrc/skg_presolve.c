@@ -761,7 +761,7 @@ int sk_gpu_solve_lp(const sk_gpu_lp_problem *problem, const sk_gpu_lp_solver_opt
     sol->solve_seconds = skg_now() - t1;
     if (sol->status == SK_GPU_LP_OPTIMAL) {
         if (options->use_presolve) sk_gpu_postsolve(&red, rx, target->cols, x);
-        else memcpy(x, rx, sizeof(double) * (size_t)problem->cols);
+        else if (problem->cols > 0) memcpy(x, rx, sizeof(double) * (size_t)problem->cols);
         sol->has_x = 1;
         sol->objective_value = 0.0;
         for (j = 0; j < problem->cols; ++j) sol->objective_value += problem->obj[j] * x[j];
