@@ -413,6 +413,7 @@ int dcsr_upload(dcsr *d, const hcsr *a, int with_f32)
     if (nt < 0) return skg_fail("out of host memory");
     d->k.ntiles = nt;
     d->k.grid = nt < SKK_SPMV_GRID ? nt : SKK_SPMV_GRID;
+    d->k.stream = (long long)a->nnz * 12 > SKK_STREAM_BYTES;
     if ((rc = skk_malloc((void **)&d->rp, sizeof(int) * ((size_t)a->rows + 1))) ||
         (rc = skk_malloc((void **)&d->ci, sizeof(int) * (size_t)(a->nnz ? a->nnz : 1))) ||
         (rc = skk_malloc((void **)&d->vd, sizeof(double) * (size_t)(a->nnz ? a->nnz : 1))) ||

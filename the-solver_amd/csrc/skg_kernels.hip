@@ -77,6 +77,13 @@ template <typename T> __device__ __forceinline__ T ntload(const T *p)
     return __builtin_nontemporal_load(p);
 }
 
+// Matrix entry load: non-temporal for matrices too large to stay cached.
+template <bool NT, typename T> __device__ __forceinline__ T mload(const T *p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
 // Sum over lanes of one wavefront (butterfly, fixed order).
 template <typename T> __device__ __forceinline__ T wave_sum(T v)
 {
@@ -178,20 +185,19 @@ int finalize(const double *part, int stride, int count, int nsum, int nmax, doub
 
 // ----------------------------------------------------------- binned SpMV
 
-// Lane-strided partial dot product of one row; two accumulators and
-// non-temporal matrix loads.
-template <typename VT, typename XT, typename AT>
+// Lane-strided partial dot product of one row; two accumulators.
+template <typename VT, typename XT, typename AT, bool NT = true>
 __device__ __forceinline__ AT row_part(const int *__restrict__ ci, const VT *__restrict__ val,
                                        const XT *__restrict__ x, int p, int e, int L)
 {
     AT s0 = 0, s1 = 0;
     for (; p + L < e; p += 2 * L) {
-        const int c0 = ntload(ci + p), c1 = ntload(ci + p + L);
-        const VT v0 = ntload(val + p), v1 = ntload(val + p + L);
+        const int c0 = mload<NT>(ci + p), c1 = mload<NT>(ci + p + L);
+        const VT v0 = mload<NT>(val + p), v1 = mload<NT>(val + p + L);
         s0 += static_cast<AT>(v0) * static_cast<AT>(x[c0]);
         s1 += static_cast<AT>(v1) * static_cast<AT>(x[c1]);
     }
-    if (p < e) s0 += static_cast<AT>(ntload(val + p)) * static_cast<AT>(x[ntload(ci + p)]);
+    if (p < e) s0 += static_cast<AT>(mload<NT>(val + p)) * static_cast<AT>(x[mload<NT>(ci + p)]);
     return s0 + s1;
 }
 
@@ -205,7 +211,7 @@ __device__ __forceinline__ AT row_part(const int *__restrict__ ci, const VT *__r
 // given -- the candidate buffer of the rotating pair (x0, x1) named by the
 // device state (cur ^ 1).  Sums are in a fixed order, so the result does
 // not depend on the launch shape.
-template <typename VT, typename XT, typename AT, class Epi>
+template <typename VT, typename XT, typename AT, class Epi, bool NT>
 __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
                                                  const int *__restrict__ ci,
                                                  const VT *__restrict__ val, const XT *x0,
@@ -225,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
         const int nr = r1 - r0;
         if (p1 - p0 > SKK_TILE_NNZ) {
             // one long row: workgroup-strided partial sums, wave and LDS combine
-            AT sum = row_part<VT, XT, AT>(ci, val, x, p0 + tid, p1, kBlock);
+            AT sum = row_part<VT, XT, AT, NT>(ci, val, x, p0 + tid, p1, kBlock);
             sum = wave_sum(sum);
             if ((tid & 63) == 0) prod[tid >> 6] = sum;
             __syncthreads();
@@ -241,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int p = p0 + k * kBlock + tid;
-            if (p < p1) prod[p - p0] = static_cast<AT>(ntload(val + p)) * static_cast<AT>(x[ntload(ci + p)]);
+            if (p < p1) prod[p - p0] = static_cast<AT>(mload<NT>(val + p)) * static_cast<AT>(x[mload<NT>(ci + p)]);
         }
         __syncthreads();
         if (nr * 2 <= kBlock) {
@@ -284,8 +290,12 @@ int launch_spmv(const skk_csr &A, const VT *vals, const XT *x0, const XT *x1,
                 const skk_state *sel, const Epi &epi)
 {
     if (A.ntiles == 0) return 0;
-    hipLaunchKernelGGL((k_spmv<VT, XT, AT, Epi>), dim3(A.grid), dim3(kBlock), 0,
-                       stream(), A.rp, A.ci, vals, x0, x1, sel, A.tile_row, A.ntiles, epi);
+    if (A.stream)
+        hipLaunchKernelGGL((k_spmv<VT, XT, AT, Epi, true>), dim3(A.grid), dim3(kBlock), 0,
+                           stream(), A.rp, A.ci, vals, x0, x1, sel, A.tile_row, A.ntiles, epi);
+    else
+        hipLaunchKernelGGL((k_spmv<VT, XT, AT, Epi, false>), dim3(A.grid), dim3(kBlock), 0,
+                           stream(), A.rp, A.ci, vals, x0, x1, sel, A.tile_row, A.ntiles, epi);
     return launch_status();
 }
 
