@@ -529,7 +529,7 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--workload", default="lp1", choices=sorted(WORKLOADS),
-                    help="lp1 (default, headline) | lp0 | qp2 | batch3 | warm4")
+                    help="lp1 (default, headline) | lp0 | qp2 | batch3 | warm4 | mlp4 | pipe1")
     ap.add_argument("--skip-solve", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=100)
@@ -599,8 +599,52 @@ def workload_mlp4(args):
     return line
 
 
+def workload_pipe1(args):
+    """The headline LP through the LP pipeline (sk_gpu_solve_lp: host presolve,
+    GPU PDLP with FP32 iterates on the reduced model, postsolve, original-space
+    gate), with the reference's own presolve (oracle/_ref) timed beside ours on
+    the same problem on the host."""
+    t = time.perf_counter()
+    model, _ = make_workload(args.quick)
+    problem = sg.LpProblem.from_model(model)
+    log(f"LpProblem built in {time.perf_counter() - t:.1f}s ({problem.rows} x {problem.cols}, nnz {problem.nnz})")
+    opts = sg.lp_solver_options(use_presolve=1, eps_optimal=1e-6, time_limit_seconds=args.solve_time_limit,
+                                precision=sg.PREC_MIXED, use_graph=0 if args.no_graph else 1)
+    t = time.perf_counter()
+    sol, _x = sg.solve_lp(problem, opts)
+    wall = time.perf_counter() - t
+    st = sol.pdlp
+    log(f"pipeline: {sg.LP_STATUS_NAMES[sol.status]} presolve {sol.presolve_seconds:.2f}s "
+        f"solve {sol.solve_seconds:.2f}s iterations {sol.iterations}")
+    extra = {"pipeline": {"status": sg.LP_STATUS_NAMES[sol.status], "converged": bool(st.converged),
+                          "presolve_s": round(sol.presolve_seconds, 3), "solve_s": round(sol.solve_seconds, 3),
+                          "wall_s": round(wall, 3), "iterations": sol.iterations,
+                          "fp32_iterations": st.fp32_iterations, "fp64_iterations": st.fp64_iterations,
+                          "removed_rows": sol.presolve_removed_rows, "removed_cols": sol.presolve_removed_cols,
+                          "kkt_reduced": [st.relative_primal_residual, st.relative_dual_residual, st.relative_gap],
+                          "original_space_primal_residual": sol.primal_residual,
+                          "time_limit_s": args.solve_time_limit}}
+    if not args.skip_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import sk_oracle as so
+        if so.load_presolve_ref() is not None:
+            t = time.perf_counter()
+            ref = so.ref_presolve(problem)
+            dt = time.perf_counter() - t
+            extra["cpu_baseline"] = {"value": round(dt, 3), "unit": "s (presolve)", "cores": 1, "kind": "reference",
+                                     "sample": "the reference's presolve (src/lp/Presolve.cpp, compiled from its "
+                                               f"sources) on the same problem, whole run; removed "
+                                               f"{ref['removed_rows']} rows and {ref['removed_cols']} columns"}
+            extra["presolve_speedup_vs_reference"] = round(dt / max(sol.presolve_seconds, 1e-9), 3)
+    rate = sol.iterations / max(st.loop_seconds, 1e-9) if sol.iterations else 0.0
+    line = _line(round(rate, 1), "PDHG iterations/s", args,
+                 "LP m=1000000 n=2000000 through presolve + PDLP + postsolve (config 1 pipeline)", extra)
+    line["higher_is_better"] = True
+    return line
+
+
 WORKLOADS = {"lp1": workload_lp1, "lp0": workload_lp0, "qp2": workload_qp2, "batch3": workload_batch3,
-             "warm4": workload_warm4, "mlp4": workload_mlp4}
+             "warm4": workload_warm4, "mlp4": workload_mlp4, "pipe1": workload_pipe1}
 
 if __name__ == "__main__":
     main()

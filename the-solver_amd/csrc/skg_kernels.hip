@@ -167,6 +167,47 @@ __device__ double combine(const double *part, int count, bool is_max)
     return out[0];
 }
 
+// Sums of S slots at once: the same fixed-order per-thread chains and
+// workgroup tree as combine() (so the totals are bit-identical to it), but
+// every partial of every slot is loaded up front, so the loads of all slots
+// are in flight together instead of one slot's after another's.
+template <int S, int kPer>
+__device__ void combine_sums(const double *const (&part)[S], const int (&count)[S], double (&out)[S])
+{
+    const int tid = threadIdx.x;
+    bool fits = true;
+#pragma unroll
+    for (int s = 0; s < S; ++s) fits = fits && count[s] <= kPer * kBlock;
+    if (!fits) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) out[s] = combine(part[s], count[s], false);
+        return;
+    }
+    double v[S][kPer];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) v[s][k] = tid + k * kBlock < count[s] ? part[s][tid + k * kBlock] : 0.0;
+    double acc[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int next = 0;   // first element the four-chain loop of combine() leaves
+#pragma unroll
+        for (int q = 0; q + 3 < kPer; q += 4) {
+            if (tid + (q + 3) * kBlock < count[s]) {
+                a0 += v[s][q]; a1 += v[s][q + 1]; a2 += v[s][q + 2]; a3 += v[s][q + 3];
+                next = q + 4;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+            if (k >= next && tid + k * kBlock < count[s]) a0 += v[s][k];
+        acc[s] = (a0 + a1) + (a2 + a3);
+    }
+    block_reduce<S>(acc, out);
+}
+
 // out[s] for slots [0, nsum) (sums) then [nsum, nsum + nmax) (maxima).
 __global__ __launch_bounds__(kBlock) void k_finalize(const double *part, int stride, int count,
                                                      int nsum, int nmax, double *out)
@@ -522,13 +563,21 @@ __device__ void decide(const skk_state &in, double dx2, double dq, double dy2, d
 }
 
 // The decision's four sums, reduced by one workgroup in a fixed order.
+// kPer: partials per thread held in registers (counts above kPer * kBlock
+// fall back to one slot at a time).
+template <int kPer>
 __device__ void probe_sums(const double *part, int stride, int dx_blocks, int dq_blocks, int dy_blocks,
                            double &dx2, double &dq, double &dy2, double &dya)
 {
-    dx2 = combine(part + (size_t)R_DX * stride, dx_blocks, false);
-    dq = dq_blocks > 0 ? combine(part + (size_t)R_DQ * stride, dq_blocks, false) : 0.0;
-    dy2 = combine(part + (size_t)R_DY * stride, dy_blocks, false);
-    dya = combine(part + (size_t)R_INTER * stride, dy_blocks, false);
+    const double *const slots[4] = {part + (size_t)R_DX * stride, part + (size_t)R_DQ * stride,
+                                    part + (size_t)R_DY * stride, part + (size_t)R_INTER * stride};
+    const int counts[4] = {dx_blocks, dq_blocks, dy_blocks, dy_blocks};
+    double out[4];
+    combine_sums<4, kPer>(slots, counts, out);
+    dx2 = out[0];
+    dq = dq_blocks > 0 ? out[1] : 0.0;
+    dy2 = out[2];
+    dya = out[3];
 }
 
 // One workgroup after the A x_cand launch: the decision, committed to st.
@@ -536,7 +585,7 @@ __global__ __launch_bounds__(kBlock) void k_probe_finalize(skk_state *st, const 
                                                            int dx_blocks, int dq_blocks, int dy_blocks)
 {
     double dx2, dq, dy2, dya;
-    probe_sums(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya);
+    probe_sums<SKK_SPMV_GRID / kBlock>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya);
     if (threadIdx.x != 0) return;
     skk_state next;
     decide(*st, dx2, dq, dy2, dya, next);
@@ -554,7 +603,8 @@ __global__ __launch_bounds__(kBlock) void k_dual_decide(Iter<T> I, Data<T> D, co
 {
     __shared__ skk_state s_next;
     double dx2, dq, dy2, dya;
-    probe_sums(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya);
+    // fused only while the products have at most 1024 workgroups together
+    probe_sums<1024 / kBlock>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya);
     if (threadIdx.x == 0) {
         decide(*snap, dx2, dq, dy2, dya, s_next);
         if (blockIdx.x == 0) *st = s_next;
