@@ -144,6 +144,8 @@ typedef struct skk_pdhg {
     double *eval_out;                        /* [2][8] residual sums        */
     int part_capacity;                       /* blocks per partial slot     */
     int elem_grid;                           /* fixed grid for reductions   */
+    int probe_grid;                          /* workgroups of the step probe
+                                                (A, or Q and A in one launch) */
 } skk_pdhg;
 
 enum { SKK_F64 = 0, SKK_F32 = 1 };

@@ -252,6 +252,66 @@ __device__ __forceinline__ AT row_part(const int *__restrict__ ci, const VT *__r
 // given -- the candidate buffer of the rotating pair (x0, x1) named by the
 // device state (cur ^ 1).  Sums are in a fixed order, so the result does
 // not depend on the launch shape.
+// One row tile [r0, r1) of a CSR product, summed into the epilogue.  A
+// tile of one row longer than SKK_TILE_NNZ is reduced by the whole
+// workgroup straight from memory; otherwise the tile's entries are
+// multiplied by consecutive lanes into LDS (`prod`) and each row is summed
+// from there.  Every thread of the workgroup calls this for the same tile.
+template <typename VT, typename XT, typename AT, bool NT, class Epi>
+__device__ __forceinline__ void spmv_tile(const int *__restrict__ rp, const int *__restrict__ ci,
+                                          const VT *__restrict__ val, const XT *__restrict__ x, int r0,
+                                          int r1, const Epi &e, double *part, AT *prod)
+{
+    constexpr int kPer = SKK_TILE_NNZ / kBlock;
+    const int tid = threadIdx.x;
+    const int p0 = rp[r0], p1 = rp[r1];
+    const int nr = r1 - r0;
+    if (p1 - p0 > SKK_TILE_NNZ) {
+        // one long row: workgroup-strided partial sums, wave and LDS combine
+        AT sum = row_part<VT, XT, AT, NT>(ci, val, x, p0 + tid, p1, kBlock);
+        sum = wave_sum(sum);
+        if ((tid & 63) == 0) prod[tid >> 6] = sum;
+        __syncthreads();
+        if (tid == 0) {
+            AT tot = prod[0];
+            for (int w = 1; w < kWaves; ++w) tot += prod[w];
+            e.row(r0, tot, part);
+        }
+        __syncthreads();
+        return;
+    }
+    // products of the tile's entries, kPer independent loads per lane
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int p = p0 + k * kBlock + tid;
+        if (p < p1) prod[p - p0] = static_cast<AT>(mload<NT>(val + p)) * static_cast<AT>(x[mload<NT>(ci + p)]);
+    }
+    __syncthreads();
+    if (nr * 2 <= kBlock) {
+        // 2^lg lanes per row, lg = floor(log2(kBlock / nr)) capped at 6
+        int lg = 0;
+        while (lg < 6 && (nr << (lg + 1)) <= kBlock) ++lg;
+        const int L = 1 << lg, g = tid >> lg, lane = tid & (L - 1);
+        AT sum = 0;
+        int row = r0 + g;
+        if (g < nr) {
+            const int q1 = rp[row + 1] - p0;
+            for (int q = rp[row] - p0 + lane; q < q1; q += L) sum += prod[q];
+        }
+        for (int off = L >> 1; off > 0; off >>= 1) sum += shfl_xor_d(sum, off);
+        if (g < nr && lane == 0) e.row(row, sum, part);
+    } else {
+        for (int g = tid; g < nr; g += kBlock) {
+            const int row = r0 + g;
+            const int q1 = rp[row + 1] - p0;
+            AT sum = 0;
+            for (int q = rp[row] - p0; q < q1; ++q) sum += prod[q];
+            e.row(row, sum, part);
+        }
+    }
+    __syncthreads();   // prod is reused by the next tile
+}
+
 template <typename VT, typename XT, typename AT, class Epi, bool NT>
 __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
                                                  const int *__restrict__ ci,
@@ -259,63 +319,48 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
                                                  const XT *x1, const skk_state *sel,
                                                  const int *__restrict__ tile_row, int ntiles, Epi epi)
 {
-    constexpr int kPer = SKK_TILE_NNZ / kBlock;
     __shared__ AT prod[SKK_TILE_NNZ];
     Epi e = epi;
     e.begin(epi);
     const XT *__restrict__ x = (sel && (sel->cur ^ 1)) ? x1 : x0;
-    const int tid = threadIdx.x;
     double part[Epi::kSlots > 0 ? Epi::kSlots : 1] = {};
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int r0 = tile_row[t], r1 = tile_row[t + 1];
-        const int p0 = rp[r0], p1 = rp[r1];
-        const int nr = r1 - r0;
-        if (p1 - p0 > SKK_TILE_NNZ) {
-            // one long row: workgroup-strided partial sums, wave and LDS combine
-            AT sum = row_part<VT, XT, AT, NT>(ci, val, x, p0 + tid, p1, kBlock);
-            sum = wave_sum(sum);
-            if ((tid & 63) == 0) prod[tid >> 6] = sum;
-            __syncthreads();
-            if (tid == 0) {
-                AT tot = prod[0];
-                for (int w = 1; w < kWaves; ++w) tot += prod[w];
-                e.row(r0, tot, part);
-            }
-            __syncthreads();
-            continue;
-        }
-        // products of the tile's entries, kPer independent loads per lane
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int p = p0 + k * kBlock + tid;
-            if (p < p1) prod[p - p0] = static_cast<AT>(mload<NT>(val + p)) * static_cast<AT>(x[mload<NT>(ci + p)]);
-        }
-        __syncthreads();
-        if (nr * 2 <= kBlock) {
-            // 2^lg lanes per row, lg = floor(log2(kBlock / nr)) capped at 6
-            int lg = 0;
-            while (lg < 6 && (nr << (lg + 1)) <= kBlock) ++lg;
-            const int L = 1 << lg, g = tid >> lg, lane = tid & (L - 1);
-            AT sum = 0;
-            int row = r0 + g;
-            if (g < nr) {
-                const int q1 = rp[row + 1] - p0;
-                for (int q = rp[row] - p0 + lane; q < q1; q += L) sum += prod[q];
-            }
-            for (int off = L >> 1; off > 0; off >>= 1) sum += shfl_xor_d(sum, off);
-            if (g < nr && lane == 0) e.row(row, sum, part);
-        } else {
-            for (int g = tid; g < nr; g += kBlock) {
-                const int row = r0 + g;
-                const int q1 = rp[row + 1] - p0;
-                AT sum = 0;
-                for (int q = rp[row] - p0; q < q1; ++q) sum += prod[q];
-                e.row(row, sum, part);
-            }
-        }
-        __syncthreads();   // prod is reused by the next tile
-    }
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x)
+        spmv_tile<VT, XT, AT, NT>(rp, ci, val, x, tile_row[t], tile_row[t + 1], e, part, prod);
     e.end(part);
+}
+
+// Two products of the same vector in one launch (the Hessian and A of a QP
+// step probe): the tiles of the first matrix, then those of the second,
+// dealt round robin over the workgroups; each epilogue keeps its own
+// partials.
+struct TileSet {
+    const int *rp, *ci, *tile_row;
+    int ntiles;
+};
+
+template <typename VT, typename XT, typename AT, class Epi1, class Epi2, bool NT1, bool NT2>
+__global__ __launch_bounds__(kBlock) void k_spmv_pair(TileSet m1, const VT *__restrict__ v1, TileSet m2,
+                                                      const VT *__restrict__ v2, const XT *x0, const XT *x1,
+                                                      const skk_state *sel, Epi1 epi1, Epi2 epi2)
+{
+    __shared__ AT prod[SKK_TILE_NNZ];
+    Epi1 e1 = epi1;
+    e1.begin(epi1);
+    Epi2 e2 = epi2;
+    e2.begin(epi2);
+    const XT *__restrict__ x = (sel && (sel->cur ^ 1)) ? x1 : x0;
+    double part1[Epi1::kSlots > 0 ? Epi1::kSlots : 1] = {};
+    double part2[Epi2::kSlots > 0 ? Epi2::kSlots : 1] = {};
+    const int total = m1.ntiles + m2.ntiles;
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        if (t < m1.ntiles)
+            spmv_tile<VT, XT, AT, NT1>(m1.rp, m1.ci, v1, x, m1.tile_row[t], m1.tile_row[t + 1], e1, part1, prod);
+        else
+            spmv_tile<VT, XT, AT, NT2>(m2.rp, m2.ci, v2, x, m2.tile_row[t - m1.ntiles],
+                                       m2.tile_row[t - m1.ntiles + 1], e2, part2, prod);
+    }
+    e1.end(part1);
+    e2.end(part2);
 }
 
 template <typename YT, typename AT> struct EpiStore {
@@ -1131,13 +1176,6 @@ int grid_for(int n, int cap)
 
 template <typename T> constexpr int prec_index() { return std::is_same<T, float>::value ? SKK_F32 : SKK_F64; }
 
-template <typename T> int q_probe(const skk_pdhg &P, const Iter<T> &I)
-{
-    EpiQProbe<T, T> eq{};
-    eq.I = I; eq.st = P.st; eq.part = P.part + (size_t)R_DQ * P.part_capacity;
-    return launch_spmv<T, T, T>(P.Q, vals_of<T>(P.Q), I.X[0], I.X[1], P.st, eq);
-}
-
 // A x_cand with the probe epilogue (x_cand is X[cur ^ 1]).
 template <typename T> int probe_spmv(const skk_pdhg &P, const Iter<T> &I, bool snapshot = false)
 {
@@ -1147,17 +1185,48 @@ template <typename T> int probe_spmv(const skk_pdhg &P, const Iter<T> &I, bool s
     return launch_spmv<T, T, T>(P.A, vals_of<T>(P.A), I.X[0], I.X[1], P.st, pr);
 }
 
+// Q x_cand and A x_cand in one launch of P.probe_grid workgroups (QP).
+template <typename T> int qa_probe(const skk_pdhg &P, const Iter<T> &I, bool snapshot)
+{
+    EpiQProbe<T, T> eq{};
+    eq.I = I; eq.st = P.st; eq.part = P.part + (size_t)R_DQ * P.part_capacity;
+    EpiProbe<T, T> pr{};
+    pr.I = I; pr.st = P.st; pr.part = P.part; pr.stride = P.part_capacity;
+    pr.snap = snapshot ? P.snap : nullptr;
+    const TileSet q{P.Q.rp, P.Q.ci, P.Q.tile_row, P.Q.ntiles}, a{P.A.rp, P.A.ci, P.A.tile_row, P.A.ntiles};
+    const T *vq = vals_of<T>(P.Q), *va = vals_of<T>(P.A);
+    using EQ = EpiQProbe<T, T>;
+    using EA = EpiProbe<T, T>;
+    const dim3 g(P.probe_grid), b(kBlock);
+    if (P.Q.stream && P.A.stream)
+        hipLaunchKernelGGL((k_spmv_pair<T, T, T, EQ, EA, true, true>), g, b, 0, stream(), q, vq, a, va, I.X[0], I.X[1], P.st, eq, pr);
+    else if (P.Q.stream)
+        hipLaunchKernelGGL((k_spmv_pair<T, T, T, EQ, EA, true, false>), g, b, 0, stream(), q, vq, a, va, I.X[0], I.X[1], P.st, eq, pr);
+    else if (P.A.stream)
+        hipLaunchKernelGGL((k_spmv_pair<T, T, T, EQ, EA, false, true>), g, b, 0, stream(), q, vq, a, va, I.X[0], I.X[1], P.st, eq, pr);
+    else
+        hipLaunchKernelGGL((k_spmv_pair<T, T, T, EQ, EA, false, false>), g, b, 0, stream(), q, vq, a, va, I.X[0], I.X[1], P.st, eq, pr);
+    return launch_status();
+}
+
+// The step probe's products: A x_cand, with Q x_cand in the same launch for
+// a QP.
+template <typename T> int step_probe(const skk_pdhg &P, const Iter<T> &I, bool snapshot = false)
+{
+    return P.has_q ? qa_probe<T>(P, I, snapshot) : probe_spmv<T>(P, I, snapshot);
+}
+
 // The single-workgroup accept/reject decision.
 int probe_decide(const skk_pdhg &P)
 {
     hipLaunchKernelGGL(k_probe_finalize, dim3(1), dim3(kBlock), 0, stream(), P.st, P.part, P.part_capacity,
-                       P.AT.grid, P.has_q ? P.Q.grid : 0, P.A.grid);
+                       P.AT.grid, P.has_q ? P.probe_grid : 0, P.probe_grid);
     return launch_status();
 }
 
 template <typename T> int probe(const skk_pdhg &P, const Iter<T> &I)
 {
-    const int rc = probe_spmv<T>(P, I);
+    const int rc = step_probe<T>(P, I);
     return rc ? rc : probe_decide(P);
 }
 
@@ -1175,7 +1244,6 @@ template <typename T> int adaptive_iter(const skk_pdhg &P)
     ep.stride = P.part_capacity;
     // A^T y_cand: y_cand is Y[cur ^ 1], picked on the device
     if ((rc = launch_spmv<T, T, T>(P.AT, vals_of<T>(P.AT), I.Y[0], I.Y[1], P.st, ep))) return rc;
-    if (P.has_q && (rc = q_probe<T>(P, I))) return rc;
     return probe<T>(P, I);
 }
 
@@ -1198,7 +1266,7 @@ template <typename T> int adaptive_run(const skk_pdhg &P, int count)
             hipLaunchKernelGGL(k_dual_adaptive<T>, dim3(P.elem_grid), dim3(kBlock), 0, stream(), I, D, P.st, P.m);
         } else {
             hipLaunchKernelGGL(k_dual_decide<T>, dim3(P.elem_grid), dim3(kBlock), 0, stream(), I, D, P.snap, P.st,
-                               P.part, P.part_capacity, P.AT.grid, P.has_q ? P.Q.grid : 0, P.A.grid, P.m);
+                               P.part, P.part_capacity, P.AT.grid, P.has_q ? P.probe_grid : 0, P.probe_grid, P.m);
         }
         if ((rc = launch_status())) return rc;
         EpiPrimalAdaptive<T, T> ep{};
@@ -1206,8 +1274,7 @@ template <typename T> int adaptive_run(const skk_pdhg &P, int count)
         ep.part = P.part + (size_t)R_DX * P.part_capacity;
         ep.stride = P.part_capacity;
         if ((rc = launch_spmv<T, T, T>(P.AT, vals_of<T>(P.AT), I.Y[0], I.Y[1], P.st, ep))) return rc;
-        if (P.has_q && (rc = q_probe<T>(P, I))) return rc;
-        if ((rc = probe_spmv<T>(P, I, true))) return rc;
+        if ((rc = step_probe<T>(P, I, true))) return rc;
     }
     return count > 0 ? probe_decide(P) : 0;
 }
@@ -1229,9 +1296,7 @@ template <typename T> int adaptive_part(const skk_pdhg &P, int part)
         return launch_spmv<T, T, T>(P.AT, vals_of<T>(P.AT), I.Y[0], I.Y[1], P.st, ep);
     }
     if (part == 3) return probe_decide(P);
-    int rc;
-    if (P.has_q && (rc = q_probe<T>(P, I))) return rc;
-    return probe_spmv<T>(P, I);
+    return step_probe<T>(P, I);
 }
 
 template <typename T> int fixed_iter(const skk_pdhg &P)

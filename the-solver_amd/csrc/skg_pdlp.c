@@ -368,15 +368,21 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
         if (eg < 1) eg = 1;
         if (eg > 2048) eg = 2048;
         Q->elem_grid = eg;
+        /* a QP's step probe runs Q x and A x in one launch over both tile sets */
+        Q->probe_grid = E->A.k.grid;
+        if (E->has_q) {
+            const long long both = (long long)E->A.k.ntiles + E->Q.k.ntiles;
+            Q->probe_grid = both < SKK_SPMV_GRID ? (int)both : SKK_SPMV_GRID;
+        }
         cap = eg;
         if (E->A.k.grid > cap) cap = E->A.k.grid;
         if (E->AT.k.grid > cap) cap = E->AT.k.grid;
         if (E->has_q && E->Q.k.grid > cap) cap = E->Q.k.grid;
+        if (Q->probe_grid > cap) cap = Q->probe_grid;
         Q->part_capacity = cap;
         /* with few block partials every dual-prox workgroup can afford to
            reduce them itself: the step decision then rides in that launch */
-        Q->fuse_decision = !P->unfused &&
-                           E->A.k.grid + E->AT.k.grid + (E->has_q ? E->Q.k.grid : 0) <= FUSE_MAX_PARTIALS;
+        Q->fuse_decision = !P->unfused && E->AT.k.grid + Q->probe_grid <= FUSE_MAX_PARTIALS;
         if ((rc = dalloc(E, (void **)&Q->ax_d, 8 * mm)) || (rc = dalloc(E, (void **)&Q->aty_d, 8 * nn)) ||
             (rc = dalloc(E, (void **)&Q->st, sizeof(skk_state))) ||
             (rc = dalloc(E, (void **)&Q->snap, sizeof(skk_state))) ||
