@@ -196,3 +196,26 @@ def test_qp_reference_first_order_fixture():
     assert max(kkt[:3]) <= 1e-6, kkt
     assert pinf <= 1e-5
     lib.sk_gpu_solution_free(C.byref(s))
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_qp_fused_decision_matches_separate_launch(prec):
+    """A QP's step probe runs Q x and A x in one launch (two tile sets);
+    with the decision fused into the next dual prox the iterates are the
+    same, bit for bit, as with the one-workgroup decision launch, and the
+    solution satisfies the relative KKT tolerance against the oracle's
+    evaluation."""
+    model = sg.Model.generated("skw_sparse_qp", 300, 600, 3, 200, 3, 777)
+    p, i, v = model.csc()
+    rp, cj, vv = sg.csc_to_csr(model.nrow, model.ncol, p, i, v)
+    vec = {k: model.vec(k) for k in ("c", "rlow", "rupp", "clow", "cupp")}
+    arr = (model.nrow, model.ncol, rp, cj, vv, vec)
+    q = model.q_csc()
+    a = _gpu_qp(arr, q, eps_optimal=1e-7, precision=prec, unfused=1)
+    b = _gpu_qp(arr, q, eps_optimal=1e-7, precision=prec, unfused=0)
+    assert a[0] == 0 and b[0] == 0 and a[3].converged and b[3].converged, sg.last_error()
+    assert a[3].iterations == b[3].iterations and a[3].restarts == b[3].restarts
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    assert b[3].kernel_launches < a[3].kernel_launches
+    kkt = so.relative_kkt(model, b[1], b[2])
+    assert max(kkt[:3]) <= 1e-6, kkt
