@@ -239,12 +239,6 @@ static int solve_qp(const sk_gpu_csr *matrix, const sk_gpu_csr *hessian, const d
     if (qd) DUP(L.qd, qd, cols);
     DALLOC(L.part, 8 * L.grid);
     DALLOC(L.out, 8);
-    if ((rc = skk_malloc((void **)&L.counters, sizeof(unsigned) * 4))) goto fail;
-    bufs[nb++] = L.counters;
-    {
-        unsigned z[4] = {0, 0, 0, 0};
-        if ((rc = skk_h2d(L.counters, z, sizeof(z)))) goto fail;
-    }
     if ((rc = skk_host_alloc((void **)&h_out, sizeof(double) * 8))) goto fail;
     if ((rc = skk_legacy_init(L.x, L.xbar, L.l, L.u, cols, L.y, rows))) goto fail;
     final_iteration = st.max_iterations;
@@ -356,7 +350,6 @@ int sk_gpu_qp_pdhg_preconditioned_with_dual(const sk_gpu_csr *matrix, const sk_g
 typedef struct vws {         /* device workspace of one certificate call */
     dcsr A, AT, Q;
     double *c, *rl, *ru, *l, *u, *x, *y, *act, *rc, *qx, *part, *out;
-    unsigned *counters;
     skk_verify_in V;
 } vws;
 
@@ -365,7 +358,7 @@ static void vws_free(vws *w)
     dcsr_free(&w->A); dcsr_free(&w->AT); dcsr_free(&w->Q);
     skk_free(w->c); skk_free(w->rl); skk_free(w->ru); skk_free(w->l); skk_free(w->u);
     skk_free(w->x); skk_free(w->y); skk_free(w->act); skk_free(w->rc); skk_free(w->qx);
-    skk_free(w->part); skk_free(w->out); skk_free(w->counters);
+    skk_free(w->part); skk_free(w->out);
     memset(w, 0, sizeof(*w));
 }
 
@@ -399,21 +392,16 @@ static int vws_build(vws *w, const hcsr *A, const hcsr *Qh, const double *c, con
         (rc = dup_d(&w->l, l, n)) || (rc = dup_d(&w->u, u, n)) || (rc = dup_d(&w->x, x, n)) ||
         (y && (rc = dup_d(&w->y, y, m))) || (rc = dup_d(&w->act, NULL, m)) ||
         (rc = dup_d(&w->rc, NULL, n)) || (rc = dup_d(&w->qx, NULL, n)) ||
-        (rc = dup_d(&w->part, NULL, 8 * cap)) || (rc = dup_d(&w->out, NULL, 8)) ||
-        (rc = skk_malloc((void **)&w->counters, sizeof(unsigned) * 8))) {
+        (rc = dup_d(&w->part, NULL, 8 * cap)) || (rc = dup_d(&w->out, NULL, 8))) {
         vws_free(w);
         return skg_hip_fail("certificate upload", rc);
-    }
-    {
-        unsigned z[8] = {0};
-        if ((rc = skk_h2d(w->counters, z, sizeof(z)))) { vws_free(w); return skg_hip_fail("upload", rc); }
     }
     w->V.m = m; w->V.n = n;
     w->V.A = w->A.k; w->V.AT = w->AT.k; w->V.Q = Qh ? &w->Q.k : NULL;
     w->V.c = w->c; w->V.rl = w->rl; w->V.ru = w->ru; w->V.l = w->l; w->V.u = w->u;
     w->V.x = w->x; w->V.y = y ? w->y : NULL;
     w->V.act = w->act; w->V.rc = w->rc; w->V.qx = w->qx;
-    w->V.part = w->part; w->V.counters = w->counters; w->V.out = w->out; w->V.grid = g;
+    w->V.part = w->part; w->V.out = w->out; w->V.grid = g;
     return 0;
 }
 

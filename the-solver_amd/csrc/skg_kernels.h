@@ -140,7 +140,6 @@ typedef struct skk_pdhg {
     int fuse_decision;                       /* decide inside the next dual
                                                 prox (few block partials)    */
     double *part;                            /* block partials              */
-    unsigned *counters;                      /* self-resetting tickets      */
     double *eval_out;                        /* [2][8] residual sums        */
     int part_capacity;                       /* blocks per partial slot     */
     int elem_grid;                           /* fixed grid for reductions   */
@@ -198,7 +197,7 @@ typedef struct skk_verify_in {
     const skk_csr *Q;       /* may be NULL (full symmetric CSR) */
     const double *c, *rl, *ru, *l, *u, *x, *y;
     double *act, *rc, *qx;
-    double *part; unsigned *counters; double *out; int grid;
+    double *part; double *out; int grid;
 } skk_verify_in;
 int skk_verify(const skk_verify_in *V);
 
@@ -215,7 +214,7 @@ typedef struct skk_legacy {
     double *x, *xbar, *y, *act, *grad, *qg, *prev;
     const double *c, *l, *u, *rl, *ru, *tau_vec, *sigma_vec, *qd;
     double tau, sigma, theta;
-    double *part; unsigned *counters; double *out; int grid;
+    double *part; double *out; int grid;
 } skk_legacy;
 int skk_legacy_init(double *x, double *xbar, const double *l, const double *u, int n, double *y, int m);
 int skk_legacy_iter(const skk_legacy *L);

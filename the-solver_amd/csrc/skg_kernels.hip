@@ -1103,7 +1103,7 @@ struct LegacyCheck {
     int m, n, have_prev;
     const double *x, *y, *act, *grad, *qg, *qd, *c, *l, *u, *rl, *ru;
     double *prev;
-    double *part; int stride; unsigned *counters; double *out;
+    double *part; int stride; double *out;
 };
 
 __device__ __forceinline__ bool has_lo(double v) { return isfinite(v) && v > -5.0e29; }
@@ -1791,7 +1791,7 @@ int skk_legacy_check(const skk_legacy *L, int have_prev)
         if ((rc = launch_spmv<double, double, double>(*L->Q, L->Q->vd, L->x, L->x, nullptr, eq))) return rc;
     }
     LegacyCheck K{L->m, L->n, have_prev, L->x, L->y, L->act, L->grad, L->Q ? L->qg : nullptr, L->qd,
-                  L->c, L->l, L->u, L->rl, L->ru, L->prev, L->part, L->grid, L->counters, L->out};
+                  L->c, L->l, L->u, L->rl, L->ru, L->prev, L->part, L->grid, L->out};
     hipLaunchKernelGGL(k_legacy_check, dim3(L->grid), dim3(kBlock), 0, stream(), K);
     if ((rc = launch_status())) return rc;
     return finalize(L->part, L->grid, L->grid, 2, 6, L->out);

@@ -387,15 +387,10 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
             (rc = dalloc(E, (void **)&Q->st, sizeof(skk_state))) ||
             (rc = dalloc(E, (void **)&Q->snap, sizeof(skk_state))) ||
             (rc = dalloc(E, (void **)&Q->part, sizeof(double) * 16 * (size_t)cap)) ||
-            (rc = dalloc(E, (void **)&Q->counters, sizeof(unsigned) * 16)) ||
             (rc = dalloc(E, (void **)&Q->eval_out, sizeof(double) * 16)) ||
             (rc = dalloc(E, (void **)&E->pv, 8 * nn)) || (rc = dalloc(E, (void **)&E->pw, 8 * mm)) ||
             (rc = dalloc(E, (void **)&E->pv2, 8 * nn)) || (rc = dalloc(E, (void **)&E->pnorm, 8)))
             goto done;
-        {
-            unsigned zeros[16] = {0};
-            if ((rc = skk_h2d(Q->counters, zeros, sizeof(zeros)))) { rc = skg_hip_fail("upload", rc); goto done; }
-        }
         if ((rc = skk_host_alloc((void **)&E->h_eval, sizeof(double) * 16 + sizeof(skk_state)))) {
             rc = skg_hip_fail("hipHostMalloc", rc); goto done;
         }
