@@ -122,6 +122,7 @@ typedef struct skk_iter {
     void *X[2], *Y[2], *AX[3];
     void *xsum, *ysum, *xr, *yr, *xavg, *yavg, *xbar;
     void *QX[2];             /* QX[cur] = Q X[cur] (convex QP only)          */
+    int sum64;               /* xsum, ysum hold doubles (else the set's type) */
 } skk_iter;
 
 typedef struct skk_pdhg {

@@ -387,14 +387,37 @@ int launch_spmv(const skk_csr &A, const VT *vals, const XT *x0, const XT *x1,
 
 // ------------------------------------------------------------ PDHG state
 
+// The running sums of accepted iterates (the averages' numerators) are held
+// in the set's type or, with sum64, in FP64 (chosen per engine at set-up; a
+// uniform branch at each use).
+template <typename T> struct Sums {
+    void *p;
+    int wide;
+    __device__ __forceinline__ void add(int k, T v) const
+    {
+        if (wide) static_cast<double *>(p)[k] += static_cast<double>(v);
+        else static_cast<T *>(p)[k] += v;
+    }
+    __device__ __forceinline__ void zero(int k) const
+    {
+        if (wide) static_cast<double *>(p)[k] = 0.0;
+        else static_cast<T *>(p)[k] = T(0);
+    }
+    __device__ __forceinline__ double get(int k) const
+    {
+        return wide ? static_cast<double *>(p)[k] : static_cast<double>(static_cast<T *>(p)[k]);
+    }
+};
+
 template <typename T> struct Iter {
-    T *X[2], *Y[2], *AX[3], *xsum, *ysum, *xr, *yr, *xavg, *yavg, *xbar, *QX[2];
+    T *X[2], *Y[2], *AX[3], *xr, *yr, *xavg, *yavg, *xbar, *QX[2];
+    Sums<T> xsum, ysum;
     __host__ __device__ Iter() {}
     explicit Iter(const skk_iter &it)
     {
         for (int k = 0; k < 2; ++k) X[k] = static_cast<T *>(it.X[k]), Y[k] = static_cast<T *>(it.Y[k]);
         for (int k = 0; k < 3; ++k) AX[k] = static_cast<T *>(it.AX[k]);
-        xsum = static_cast<T *>(it.xsum); ysum = static_cast<T *>(it.ysum);
+        xsum = Sums<T>{it.xsum, it.sum64}; ysum = Sums<T>{it.ysum, it.sum64};
         xr = static_cast<T *>(it.xr); yr = static_cast<T *>(it.yr);
         xavg = static_cast<T *>(it.xavg); yavg = static_cast<T *>(it.yavg);
         xbar = static_cast<T *>(it.xbar);
@@ -462,12 +485,12 @@ __global__ void k_init_vectors(Iter<T> I, int n, int m, int warm_x, int warm_y)
     const int stride = gridDim.x * blockDim.x;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
         const T x = warm_x ? I.X[0][j] : T(0);
-        I.X[0][j] = x; I.X[1][j] = x; I.xsum[j] = T(0); I.xr[j] = x; I.xbar[j] = x; I.xavg[j] = x;
+        I.X[0][j] = x; I.X[1][j] = x; I.xsum.zero(j); I.xr[j] = x; I.xbar[j] = x; I.xavg[j] = x;
         if (I.QX[0]) { I.QX[0][j] = T(0); I.QX[1][j] = T(0); }
     }
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
         const T y = warm_y ? I.Y[0][i] : T(0);
-        I.Y[0][i] = y; I.Y[1][i] = y; I.ysum[i] = T(0); I.yr[i] = y; I.yavg[i] = y;
+        I.Y[0][i] = y; I.Y[1][i] = y; I.ysum.zero(i); I.yr[i] = y; I.yavg[i] = y;
         I.AX[0][i] = T(0); I.AX[1][i] = T(0); I.AX[2][i] = T(0);
     }
 }
@@ -497,7 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_dual_adaptive(Iter<T> I, Data<T> D,
         const T yi = y[i];
         const T v = yi + sigma * (T(2) * ax[i] - axp[i]);
         yc[i] = v - sigma * clampv(v / sigma, D.rl[i], D.ru[i]);
-        if (pending) I.ysum[i] += yi;
+        if (pending) I.ysum.add(i, yi);
     }
 }
 
@@ -529,7 +552,7 @@ template <typename T, typename AT> struct EpiPrimalAdaptive {
         const T g = D.c[j] + static_cast<T>(s) + (qx ? qx[j] : T(0));
         const T xn = clampv(xj - tau * g, D.l[j], D.u[j]);
         xc[j] = xn;
-        if (pending) I.xsum[j] += xj;
+        if (pending) I.xsum.add(j, xj);
         const double d = static_cast<double>(xn) - static_cast<double>(xj);
         p[0] += d * d;
     }
@@ -666,7 +689,7 @@ __global__ __launch_bounds__(kBlock) void k_dual_decide(Iter<T> I, Data<T> D, co
         const T yi = y[i];
         const T v = yi + sigma * (T(2) * ax[i] - axp[i]);
         yc[i] = v - sigma * clampv(v / sigma, D.rl[i], D.ru[i]);
-        if (pending) I.ysum[i] += yi;
+        if (pending) I.ysum.add(i, yi);
     }
 }
 
@@ -717,7 +740,7 @@ template <typename T, typename AT> struct EpiDualFixed {
         const T v = y[i] + sigma * static_cast<T>(s);
         const T yi = v - sigma * clampv(v / sigma, D.rl[i], D.ru[i]);
         y[i] = yi;
-        I.ysum[i] += yi;
+        I.ysum.add(i, yi);
     }
     __device__ void end(double *) const {}
 };
@@ -742,7 +765,7 @@ template <typename T, typename AT> struct EpiPrimalFixed {
         const T xn = clampv(xj - tau * (D.c[j] + static_cast<T>(s)), D.l[j], D.u[j]);
         I.xbar[j] = T(2) * xn - xj;
         x[j] = xn;
-        I.xsum[j] += xn;
+        I.xsum.add(j, xn);
     }
     __device__ void end(double *) const {}
 };
@@ -755,8 +778,8 @@ __global__ void k_flush(Iter<T> I, const skk_state *__restrict__ st, int n, int 
     if (!st->pending) return;
     const int cur = st->cur;
     const int stride = gridDim.x * blockDim.x;
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) I.xsum[j] += I.x(cur)[j];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) I.ysum[i] += I.y(cur)[i];
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) I.xsum.add(j, I.x(cur)[j]);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) I.ysum.add(i, I.y(cur)[i]);
 }
 
 template <typename T>
@@ -770,8 +793,8 @@ __global__ void k_average(Iter<T> I, const skk_state *__restrict__ st, int n, in
     }
     const T scale = static_cast<T>(1.0 / terms);
     const int stride = gridDim.x * blockDim.x;
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) I.xavg[j] = scale * I.xsum[j];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) I.yavg[i] = scale * I.ysum[i];
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) I.xavg[j] = static_cast<T>(scale * I.xsum.get(j));
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) I.yavg[i] = static_cast<T>(scale * I.ysum.get(i));
 }
 
 // The eight sums of PdlpKernels.cu:101-209 in FP64: ||Ax - proj Ax||^2,
@@ -853,13 +876,13 @@ __global__ void k_restart(Iter<T> I, skk_state *st, const double *__restrict__ a
         T x = I.x(cur)[j];
         if (use_avg) { x = I.xavg[j]; I.x(cur)[j] = x; if (qx_avg) I.qx(cur)[j] = static_cast<T>(qx_avg[j]); }
         I.xbar[j] = x;
-        I.xsum[j] = T(0);
+        I.xsum.zero(j);
         I.xr[j] = x;
     }
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
         T y = I.y(cur)[i];
         if (use_avg) { y = I.yavg[i]; I.y(cur)[i] = y; }
-        I.ysum[i] = T(0);
+        I.ysum.zero(i);
         I.yr[i] = y;
         if (adaptive) {
             const T a = use_avg ? static_cast<T>(ax_avg[i]) : I.ax(a0)[i];
@@ -876,13 +899,13 @@ __global__ void k_promote(Iter<double> D, Iter<float> F, int n, int m)
     const int stride = gridDim.x * blockDim.x;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
         D.X[0][j] = F.X[0][j]; D.X[1][j] = F.X[1][j];
-        D.xsum[j] = F.xsum[j]; D.xr[j] = F.xr[j]; D.xbar[j] = F.xbar[j]; D.xavg[j] = F.xavg[j];
+        static_cast<double *>(D.xsum.p)[j] = F.xsum.get(j); D.xr[j] = F.xr[j]; D.xbar[j] = F.xbar[j]; D.xavg[j] = F.xavg[j];
         if (D.QX[0]) { D.QX[0][j] = F.QX[0][j]; D.QX[1][j] = F.QX[1][j]; }
     }
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
         D.Y[0][i] = F.Y[0][i]; D.Y[1][i] = F.Y[1][i];
         D.AX[0][i] = F.AX[0][i]; D.AX[1][i] = F.AX[1][i]; D.AX[2][i] = F.AX[2][i];
-        D.ysum[i] = F.ysum[i]; D.yr[i] = F.yr[i]; D.yavg[i] = F.yavg[i];
+        static_cast<double *>(D.ysum.p)[i] = F.ysum.get(i); D.yr[i] = F.yr[i]; D.yavg[i] = F.yavg[i];
     }
 }
 
