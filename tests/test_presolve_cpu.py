@@ -277,3 +277,21 @@ def test_solve_lp_rejects_invalid_input():
     p.row_ptr[1] = 5
     with pytest.raises(RuntimeError):
         sg.solve_lp(p)
+
+
+@ref_missing
+def test_matches_live_reference_on_a_power_law_lp():
+    """The config-1 generator at a fifth of its size (200,000 x 400,000,
+    about 3.2M nonzeros): every pass runs and thousands of rows and columns
+    go, bit for bit as the reference's presolve."""
+    import ctypes as C
+    w = sg.load_workloads()
+    model = sg.Model()
+    q = C.c_double()
+    assert w.skw_zipf_lp(C.byref(model.m), 200000, 400000, 1.2, 8.0, 2000, 0.3, 20260826, C.byref(q)) == 0
+    model._generated = True
+    p = sg.LpProblem.from_model(model)
+    got = flat(p)
+    assert int(got["removed_rows"]) > 1000 and int(got["removed_cols"]) > 1000
+    bad = pc.mismatches(got, pc.result_arrays(so.ref_presolve(p)))
+    assert not bad, bad
