@@ -42,7 +42,8 @@ typedef struct skb_batch {
     int count, max_m, max_n, pad;
     const skb_inst *inst;
     const int *order;              /* queue position -> instance          */
-    const int *a_rp, *a_ci, *t_rp, *t_ci;
+    const int *a_rp, *t_rp;
+    const unsigned short *a_ci, *t_ci;  /* local column indices (< 65536) */
     const float *a_v, *t_v;
     const double *a_vd, *t_vd;
     const float *c, *l, *u, *rl, *ru;

@@ -15,8 +15,18 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 
 #include "skg_batch.h"
+
+// Inner loops over a row's entries are unrolled by this factor: with rows
+// of 5 to 10 entries the unrolled body keeps several matrix loads in flight
+// per lane (one accumulator, so the summation order is unchanged).
+// Measured on 4096 LPs with m in 20..400: 1.39x over no unrolling; 2 and 8
+// are slower than 4.
+#ifndef SKB_UNROLL
+#define SKB_UNROLL 4
+#endif
 
 namespace {
 
@@ -56,37 +66,46 @@ __device__ __forceinline__ void wg_sum(double (&v)[S], double *sh /* S * BT/64 *
 
 template <typename T> __device__ __forceinline__ T clampv(T v, T lo, T hi) { return fmin(fmax(v, lo), hi); }
 
+// pow, exp and log out of line: inlined, their polynomial constants are
+// hoisted into registers held for the whole kernel (about 40 VGPRs and
+// many SGPRs), which limits how many workgroups share a CU.  They run
+// once per 64 iterations or per restart.
+__device__ __noinline__ double pow_ool(double x, double y) { return pow(x, y); }
+__device__ __noinline__ double exp_ool(double x) { return exp(x); }
+__device__ __noinline__ double log_ool(double x) { return log(x); }
+
+// The value a lane of the wave holds (lane uniform; v_readlane).
+__device__ __forceinline__ double lane_value(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// One instance's operators and data.  The operators' column indices are
+// 16-bit (an instance's vectors must fit in LDS, so it has far fewer than
+// 65536 rows or columns): 12 rather than 16 bytes per entry for each
+// iteration to read, so the operators of the instances running on an XCD
+// compete less for its L2.
 struct Inst {
     int m, n;
-    const int *arp, *aci, *trp, *tci;      // CSR(A) and CSR(A^T), local indices
+    const int *arp, *trp;                  // CSR(A) and CSR(A^T) row pointers
+    const uint16_t *aci, *tci;             // their local column indices
     const float *av, *tv;
     const double *avd, *tvd;
     const float *c, *l, *u, *rl, *ru;
     const double *cd, *ld, *ud, *rld, *rud;
     double cost_norm, bound_norm;
-    // operator values and data in the iterate precision
-    template <typename T> __device__ const T *a() const;
-    template <typename T> __device__ const T *t() const;
-    template <typename T> __device__ const T *cv() const;
-    template <typename T> __device__ const T *lv() const;
-    template <typename T> __device__ const T *uv() const;
-    template <typename T> __device__ const T *rlv() const;
-    template <typename T> __device__ const T *ruv() const;
 };
-template <> __device__ const float *Inst::a<float>() const { return av; }
-template <> __device__ const double *Inst::a<double>() const { return avd; }
-template <> __device__ const float *Inst::t<float>() const { return tv; }
-template <> __device__ const double *Inst::t<double>() const { return tvd; }
-template <> __device__ const float *Inst::cv<float>() const { return c; }
-template <> __device__ const double *Inst::cv<double>() const { return cd; }
-template <> __device__ const float *Inst::lv<float>() const { return l; }
-template <> __device__ const double *Inst::lv<double>() const { return ld; }
-template <> __device__ const float *Inst::uv<float>() const { return u; }
-template <> __device__ const double *Inst::uv<double>() const { return ud; }
-template <> __device__ const float *Inst::rlv<float>() const { return rl; }
-template <> __device__ const double *Inst::rlv<double>() const { return rld; }
-template <> __device__ const float *Inst::ruv<float>() const { return ru; }
-template <> __device__ const double *Inst::ruv<double>() const { return rud; }
+
+// The FP32 or the FP64 copy of an array, by the iterate precision T.
+template <typename T>
+__device__ __forceinline__ const T *pick(const float *f, const double *d)
+{
+    if constexpr (sizeof(T) == 4) return f;
+    else return d;
+}
 
 __device__ Inst load_inst(const skb_batch &B, int k)
 {
@@ -126,7 +145,9 @@ __device__ void residual_sums(const Inst &I, const T *x, const T *y, const T *xr
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int j = threadIdx.x; j < I.n; j += BT) {
         double aty = 0.0;
-        for (int p = I.trp[j]; p < I.trp[j + 1]; ++p) aty += I.tvd[p] * (ys * (double)y[I.tci[p]]);
+        const int p1 = I.trp[j + 1];
+#pragma unroll SKB_UNROLL
+        for (int p = I.trp[j]; p < p1; ++p) aty += I.tvd[p] * (ys * (double)y[I.tci[p]]);
         const double xj = xs * (double)x[j], lo = I.ld[j], hi = I.ud[j], r = I.cd[j] + aty;
         const double dx = xj - (double)xr[j];
         acc[2] += I.cd[j] * xj;
@@ -143,7 +164,9 @@ __device__ void residual_sums(const Inst &I, const T *x, const T *y, const T *xr
     }
     for (int i = threadIdx.x; i < I.m; i += BT) {
         double ax = 0.0;
-        for (int p = I.arp[i]; p < I.arp[i + 1]; ++p) ax += I.avd[p] * (xs * (double)x[I.aci[p]]);
+        const int p1 = I.arp[i + 1];
+#pragma unroll SKB_UNROLL
+        for (int p = I.arp[i]; p < p1; ++p) ax += I.avd[p] * (xs * (double)x[I.aci[p]]);
         const double rl = I.rld[i], ru = I.rud[i];
         const double pr = ax - fmin(fmax(ax, rl), ru);
         const double yi = ys * (double)y[i], dy = yi - (double)yr[i];
@@ -161,12 +184,14 @@ __device__ void residual_sums(const Inst &I, const T *x, const T *y, const T *xr
 
 // y = A x (LDS vectors), one thread per row.
 template <int BT, typename T>
-__device__ __forceinline__ void spmv_rows(int rows, const int *rp, const int *ci, const T *v, const T *x,
+__device__ __forceinline__ void spmv_rows(int rows, const int *rp, const uint16_t *ci, const T *v, const T *x,
                                           T *y)
 {
     for (int i = threadIdx.x; i < rows; i += BT) {
         T s = 0;
-        for (int p = rp[i]; p < rp[i + 1]; ++p) s += v[p] * x[ci[p]];
+        const int p1 = rp[i + 1];
+#pragma unroll SKB_UNROLL
+        for (int p = rp[i]; p < p1; ++p) s += v[p] * x[ci[p]];
         y[i] = s;
     }
 }
@@ -184,8 +209,9 @@ __device__ void solve_instance(const skb_batch &B, int k, T *lds, double *sh)
     if (resume && !o.handoff) return;    // finished in the FP32 phase (uniform)
     const Inst I = load_inst(B, k);
     const int m = I.m, n = I.n;
-    const T *av = I.a<T>(), *tv = I.t<T>();
-    const T *cg = I.cv<T>(), *lg = I.lv<T>(), *ug = I.uv<T>(), *rlg = I.rlv<T>(), *rug = I.ruv<T>();
+    const T *av = pick<T>(I.av, I.avd), *tv = pick<T>(I.tv, I.tvd);
+    const T *cg = pick<T>(I.c, I.cd), *lg = pick<T>(I.l, I.ld), *ug = pick<T>(I.u, I.ud);
+    const T *rlg = pick<T>(I.rl, I.rld), *rug = pick<T>(I.ru, I.rud);
     // LDS layout: 7 column vectors, 9 row vectors
     T *X0 = lds, *X1 = X0 + n, *xsum = X1 + n, *xr = xsum + n, *cl = xr + n, *ll = cl + n, *ul = ll + n;
     T *Y0 = ul + n, *Y1 = Y0 + m, *AX0 = Y1 + m, *AX1 = AX0 + m, *AX2 = AX1 + m, *ysum = AX2 + m,
@@ -211,7 +237,9 @@ __device__ void solve_instance(const skb_batch &B, int k, T *lds, double *sh)
             double acc[1] = {0.0};
             for (int j = threadIdx.x; j < n; j += BT) {
                 T s = 0;
-                for (int p = I.trp[j]; p < I.trp[j + 1]; ++p) s += tv[p] * Y0[I.tci[p]];
+                const int p1 = I.trp[j + 1];
+#pragma unroll SKB_UNROLL
+                for (int p = I.trp[j]; p < p1; ++p) s += tv[p] * Y0[I.tci[p]];
                 X1[j] = s;
                 acc[0] += (double)s * (double)s;
             }
@@ -259,26 +287,49 @@ __device__ void solve_instance(const skb_batch &B, int k, T *lds, double *sh)
     Kkt best{INFINITY, INFINITY, INFINITY, 0, 0, INFINITY};
     while (iteration < P.max_iterations) {
         const int period = min(P.restart_period, P.max_iterations - iteration);
+        double g3 = 0.0, g6 = 0.0;
+        // An accepted iterate joins the running sums inside the next
+        // iteration's dual and primal loops, where each thread reads its
+        // own entries of it anyway; the sums are only read across threads
+        // at the window's end, after the flush and barrier below.
+        bool pending = false;
         for (int t = 0; t < period; ++t) {
+            if ((t & 63) == 0) {
+                // the step rule's factors (k + 1)^-0.3 and (k + 1)^-0.6 for
+                // the next 64 iterations, k = updates + 1 + lane: the same
+                // pow calls as per iteration, 64 at a time
+                const double kl = (updates + (double)(threadIdx.x & 63)) + 1.0;
+                g3 = pow_ool(kl + 1.0, -0.3);
+                g6 = pow_ool(kl + 1.0, -0.6);
+            }
             const T sigma = (T)(eta * omega), tau = (T)(eta / omega);
             for (int i = threadIdx.x; i < m; i += BT) {
-                const T v = y[i] + sigma * ((T)2 * ax[i] - axp[i]);
+                const T yi = y[i];
+                if (pending) ysum[i] += yi;
+                const T v = yi + sigma * ((T)2 * ax[i] - axp[i]);
                 yc[i] = v - sigma * clampv<T>(v / sigma, rll[i], rul[i]);
             }
             __syncthreads();
             double acc[3] = {0.0, 0.0, 0.0};
             for (int j = threadIdx.x; j < n; j += BT) {
                 T s = 0;
-                for (int p = I.trp[j]; p < I.trp[j + 1]; ++p) s += tv[p] * yc[I.tci[p]];
-                const T xn = clampv<T>(x[j] - tau * (cl[j] + s), ll[j], ul[j]);
+                const int p1 = I.trp[j + 1];
+#pragma unroll SKB_UNROLL
+                for (int p = I.trp[j]; p < p1; ++p) s += tv[p] * yc[I.tci[p]];
+                const T xj = x[j];
+                if (pending) xsum[j] += xj;
+                const T xn = clampv<T>(xj - tau * (cl[j] + s), ll[j], ul[j]);
                 xc[j] = xn;
-                const double dd = (double)xn - (double)x[j];
+                const double dd = (double)xn - (double)xj;
                 acc[0] += dd * dd;
             }
+            pending = false;
             __syncthreads();
             for (int i = threadIdx.x; i < m; i += BT) {
                 T s = 0;
-                for (int p = I.arp[i]; p < I.arp[i + 1]; ++p) s += av[p] * xc[I.aci[p]];
+                const int p1 = I.arp[i + 1];
+#pragma unroll SKB_UNROLL
+                for (int p = I.arp[i]; p < p1; ++p) s += av[p] * xc[I.aci[p]];
                 axc[i] = s;
                 const double dy = (double)yc[i] - (double)y[i];
                 acc[1] += dy * dy;
@@ -290,7 +341,7 @@ __device__ void solve_instance(const skb_batch &B, int k, T *lds, double *sh)
             const double eta_bar = (inter > 0.0 && movement > 0.0) ? movement / inter : eta_max;
             const bool accept = eta <= eta_bar;
             const double kk = updates + 1.0;
-            double next = fmin((1.0 - pow(kk + 1.0, -0.3)) * eta_bar, (1.0 + pow(kk + 1.0, -0.6)) * eta);
+            double next = fmin((1.0 - lane_value(g3, t & 63)) * eta_bar, (1.0 + lane_value(g6, t & 63)) * eta);
             if (!(next > 0.0) || !isfinite(next)) next = eta;
             eta = fmin(fmax(next, eta_min), eta_max);
             updates = kk;
@@ -299,11 +350,14 @@ __device__ void solve_instance(const skb_batch &B, int k, T *lds, double *sh)
                 T *t0 = x; x = xc; xc = t0;
                 t0 = y; y = yc; yc = t0;
                 t0 = axp; axp = ax; ax = axc; axc = t0;
-                for (int j = threadIdx.x; j < n; j += BT) xsum[j] += x[j];
-                for (int i = threadIdx.x; i < m; i += BT) ysum[i] += y[i];
-                __syncthreads();
+                pending = true;
             }
         }
+        if (pending) {
+            for (int j = threadIdx.x; j < n; j += BT) xsum[j] += x[j];
+            for (int i = threadIdx.x; i < m; i += BT) ysum[i] += y[i];
+        }
+        __syncthreads();
         iteration += period;
         since_restart += period;
         if (kFp32) fp32_iterations += period;
@@ -356,7 +410,7 @@ __device__ void solve_instance(const skb_batch &B, int k, T *lds, double *sh)
         ++restarts;
         final_avg = 0;
         if (best.pmove > 1e-14 && best.dmove > 1e-14) {
-            const double upd = exp(0.5 * log(best.dmove / best.pmove) + 0.5 * log(omega));
+            const double upd = exp_ool(0.5 * log_ool(best.dmove / best.pmove) + 0.5 * log_ool(omega));
             if (isfinite(upd)) omega = fmin(1e6, fmax(1e-6, upd));
         }
     }
@@ -382,7 +436,15 @@ __device__ void solve_instance(const skb_batch &B, int k, T *lds, double *sh)
     __syncthreads();
 }
 
-template <int BT, typename T> __global__ __launch_bounds__(BT) void k_batch_pdlp(skb_batch B)
+// Register budget: at most 512 / SKB_WAVES_PER_EU VGPRs per lane, so that
+// SKB_WAVES_PER_EU waves share a SIMD.  Unconstrained the kernel takes
+// about 170 (two waves per SIMD); at 4 the compiler spills a few values of
+// the once-per-window code, and 4096 LPs with m in 20..400 run 8% faster.
+#ifndef SKB_WAVES_PER_EU
+#define SKB_WAVES_PER_EU 4
+#endif
+template <int BT, typename T>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(SKB_WAVES_PER_EU))) void k_batch_pdlp(skb_batch B)
 {
     extern __shared__ double lds_raw[];
     T *lds = reinterpret_cast<T *>(lds_raw);
@@ -392,7 +454,9 @@ template <int BT, typename T> __global__ __launch_bounds__(BT) void k_batch_pdlp
         if (threadIdx.x == 0)
             next = (int)__hip_atomic_fetch_add(B.queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
-        const int q = next;
+        // uniform by construction: readfirstlane lets the compiler keep the
+        // instance's pointers and sizes in scalar registers
+        const int q = __builtin_amdgcn_readfirstlane(next);
         __syncthreads();
         if (q >= B.count) break;       // every wave of the workgroup leaves together
         solve_instance<BT, T>(B, B.order[q], lds, sh);

@@ -41,7 +41,8 @@ typedef struct hbatch {
     int count, rows, cols, nnz, arp, trp;
     skb_inst *inst;
     int *order;
-    int *a_rp, *a_ci, *t_rp, *t_ci;
+    int *a_rp, *t_rp;
+    unsigned short *a_ci, *t_ci;       /* local column indices */
     float *a_v, *t_v;
     double *a_vd, *t_vd;
     double *cd, *ld, *ud, *rld, *rud;
@@ -89,7 +90,7 @@ static int pack(hbatch *h, int count, const sk_model *models, const sk_gpu_pdlp_
         return skg_fail("batch too large for 32-bit offsets");
     h->rows = (int)rows; h->cols = (int)cols; h->nnz = (int)nnz; h->arp = (int)arp; h->trp = (int)trp;
 #define AL(p, t, c) if (!((p) = (t *)malloc(sizeof(t) * ((size_t)(c) + 1)))) return skg_fail("out of host memory")
-    AL(h->a_rp, int, arp); AL(h->t_rp, int, trp); AL(h->a_ci, int, nnz); AL(h->t_ci, int, nnz);
+    AL(h->a_rp, int, arp); AL(h->t_rp, int, trp); AL(h->a_ci, unsigned short, nnz); AL(h->t_ci, unsigned short, nnz);
     AL(h->a_v, float, nnz); AL(h->t_v, float, nnz); AL(h->a_vd, double, nnz); AL(h->t_vd, double, nnz);
     AL(h->cd, double, cols); AL(h->ld, double, cols); AL(h->ud, double, cols);
     AL(h->rld, double, rows); AL(h->rud, double, rows); AL(h->R, double, rows); AL(h->C, double, cols);
@@ -114,12 +115,12 @@ static int pack(hbatch *h, int count, const sk_model *models, const sk_gpu_pdlp_
         for (i = 0; i < d->m; ++i)
             for (p = A.p[i]; p < A.p[i + 1]; ++p) {
                 const double v = A.v[p] * R[i] * Cs[A.j[p]];
-                h->a_ci[d->nz + p] = A.j[p]; h->a_vd[d->nz + p] = v; h->a_v[d->nz + p] = (float)v;
+                h->a_ci[d->nz + p] = (unsigned short)A.j[p]; h->a_vd[d->nz + p] = v; h->a_v[d->nz + p] = (float)v;
             }
         for (j = 0; j < d->n; ++j)
             for (p = AT.p[j]; p < AT.p[j + 1]; ++p) {
                 const double v = AT.v[p] * R[AT.j[p]] * Cs[j];
-                h->t_ci[d->nz + p] = AT.j[p]; h->t_vd[d->nz + p] = v; h->t_v[d->nz + p] = (float)v;
+                h->t_ci[d->nz + p] = (unsigned short)AT.j[p]; h->t_vd[d->nz + p] = v; h->t_v[d->nz + p] = (float)v;
             }
         for (j = 0; j < d->n; ++j) {
             h->cd[d->col + j] = md->c[j] * Cs[j];
@@ -147,7 +148,9 @@ typedef struct dbatch {
 static int dput(dbatch *d, const void **dst, const void *src, size_t bytes)
 {
     void *q = NULL;
-    int rc = skk_malloc(&q, bytes ? bytes : 8);
+    int rc;
+    if (d->np >= (int)(sizeof(d->p) / sizeof(d->p[0]))) return skg_fail("too many batch buffers");
+    rc = skk_malloc(&q, bytes ? bytes : 8);
     if (rc) return rc;
     d->p[d->np++] = q;
     if (src && bytes && (rc = skk_h2d(q, src, bytes))) return rc;
@@ -213,13 +216,17 @@ static double timed_launch(const skb_batch *B, const phases *ph, void **graph, i
     return skk_timeline_ms(0, 1);
 }
 
+#ifndef SKB_MAX_PER_CU
+#define SKB_MAX_PER_CU 8
+#endif
+
 static int grid_for(int count, int bt, size_t lds_bytes)
 {
     const int cus = skb_cu_count();
     const int by_lds = lds_bytes > 0 ? (int)(163840 / lds_bytes) : 8;
     int per_cu = 2048 / bt;
     if (per_cu > by_lds) per_cu = by_lds;
-    if (per_cu > 8) per_cu = 8;
+    if (per_cu > SKB_MAX_PER_CU) per_cu = SKB_MAX_PER_CU;
     if (per_cu < 1) per_cu = 1;
     return count < cus * per_cu ? (count > 0 ? count : 1) : cus * per_cu;
 }
@@ -256,6 +263,8 @@ int sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_opt
         if (models[k].ncol > max_n) max_n = models[k].ncol;
     }
     if (count == 0) return 0;
+    if (max_m > 65536 || max_n > 65536)     /* 16-bit local indices (LDS bounds them far lower) */
+        return skg_fail("batch instances are limited to 65536 rows and columns");
     if (skg_have_device()) return -1;
     mixed = o.pdlp.precision == SK_GPU_PREC_MIXED;
     if (!mixed && o.pdlp.precision != SK_GPU_PREC_FP64) return skg_fail("unknown precision %d", o.pdlp.precision);
@@ -273,7 +282,8 @@ int sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_opt
     PUT(B.inst, h.inst, sizeof(skb_inst) * (size_t)count);
     PUT(B.order, h.order, sizeof(int) * (size_t)count);
     PUT(B.a_rp, h.a_rp, sizeof(int) * (size_t)h.arp); PUT(B.t_rp, h.t_rp, sizeof(int) * (size_t)h.trp);
-    PUT(B.a_ci, h.a_ci, sizeof(int) * (size_t)h.nnz); PUT(B.t_ci, h.t_ci, sizeof(int) * (size_t)h.nnz);
+    PUT(B.a_ci, h.a_ci, sizeof(unsigned short) * (size_t)h.nnz);
+    PUT(B.t_ci, h.t_ci, sizeof(unsigned short) * (size_t)h.nnz);
     PUT(B.a_v, h.a_v, sizeof(float) * (size_t)h.nnz); PUT(B.t_v, h.t_v, sizeof(float) * (size_t)h.nnz);
     PUT(B.a_vd, h.a_vd, sizeof(double) * (size_t)h.nnz); PUT(B.t_vd, h.t_vd, sizeof(double) * (size_t)h.nnz);
     PUT(B.cd, h.cd, sizeof(double) * (size_t)h.cols); PUT(B.ld, h.ld, sizeof(double) * (size_t)h.cols);
