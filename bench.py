@@ -421,16 +421,17 @@ def workload_batch3(args):
                                       "p90": float(np.percentile(its, 90)), "max": int(its.max()),
                                       "total": int(its.sum())}}}
     # plain FP64 GPU PDHG, one instance at a time, on a sample
-    lib = sg.load_gpu()
-    sample = list(range(0, count, max(1, count // 32)))
-    t = time.perf_counter()
-    for i in sample:
-        S = _session_qp(lib, models[i], _arrays(models[i]), None, precision=sg.PREC_FP64, use_graph=1)
-        S.solve()
-        S.close()
-    seq = (time.perf_counter() - t) / len(sample)
-    extra["sequential_fp64_s_per_instance"] = seq
-    extra["batch_speedup_vs_sequential_fp64"] = round(seq * count / st.device_seconds, 1)
+    if not args.skip_solve:
+        lib = sg.load_gpu()
+        sample = list(range(0, count, max(1, count // 32)))
+        t = time.perf_counter()
+        for i in sample:
+            S = _session_qp(lib, models[i], _arrays(models[i]), None, precision=sg.PREC_FP64, use_graph=1)
+            S.solve()
+            S.close()
+        seq = (time.perf_counter() - t) / len(sample)
+        extra["sequential_fp64_s_per_instance"] = seq
+        extra["batch_speedup_vs_sequential_fp64"] = round(seq * count / st.device_seconds, 1)
     if not args.skip_cpu:
         sys.path.insert(0, str(ROOT / "oracle"))
         import sk_oracle as so
