@@ -63,6 +63,8 @@ size_t skb_lds_bytes(int max_m, int max_n, int word_bytes);
 int    skb_launch(const skb_batch *B, int block_threads, int grid, int fp64);
 int    skb_max_lds(void);
 int    skb_cu_count(void);
+/* Workgroups of one phase's kernel a CU holds at once (-1 if unknown). */
+int    skb_blocks_per_cu(int block_threads, int fp64, size_t lds_bytes);
 
 #ifdef __cplusplus
 }

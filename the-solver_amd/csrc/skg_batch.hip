@@ -479,6 +479,22 @@ template <typename T> void launch_phase(const skb_batch *B, int block_threads, i
     }
 }
 
+// Workgroups of k_batch_pdlp<block_threads, T> one CU holds at once with
+// `lds` bytes of dynamic LDS (registers, LDS and wave slots), or -1.
+template <typename T> int resident(int block_threads, size_t lds)
+{
+    int v = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (block_threads) {
+    case 64:  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_batch_pdlp<64, T>, 64, lds); break;
+    case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_batch_pdlp<128, T>, 128, lds); break;
+    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_batch_pdlp<256, T>, 256, lds); break;
+    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_batch_pdlp<512, T>, 512, lds); break;
+    default: break;
+    }
+    return e == hipSuccess ? v : -1;
+}
+
 }  // namespace
 
 extern "C" {
@@ -506,6 +522,11 @@ int skb_max_lds(void)
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 65536;
     return v;
+}
+
+int skb_blocks_per_cu(int block_threads, int fp64, size_t lds)
+{
+    return fp64 ? resident<double>(block_threads, lds) : resident<float>(block_threads, lds);
 }
 
 int skb_cu_count(void)

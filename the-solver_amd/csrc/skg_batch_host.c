@@ -220,12 +220,17 @@ static double timed_launch(const skb_batch *B, const phases *ph, void **graph, i
 #define SKB_MAX_PER_CU 8
 #endif
 
-static int grid_for(int count, int bt, size_t lds_bytes)
+/* Persistent grid of one phase: as many workgroups as the device holds at
+ * once (the runtime's occupancy for this kernel, block size and LDS). */
+static int grid_for(int count, int bt, size_t lds_bytes, int fp64)
 {
     const int cus = skb_cu_count();
-    const int by_lds = lds_bytes > 0 ? (int)(163840 / lds_bytes) : 8;
-    int per_cu = 2048 / bt;
-    if (per_cu > by_lds) per_cu = by_lds;
+    int per_cu = skb_blocks_per_cu(bt, fp64, lds_bytes);
+    if (per_cu <= 0) {
+        const int by_lds = lds_bytes > 0 ? (int)(163840 / lds_bytes) : 8;
+        per_cu = 2048 / bt;
+        if (per_cu > by_lds) per_cu = by_lds;
+    }
     if (per_cu > SKB_MAX_PER_CU) per_cu = SKB_MAX_PER_CU;
     if (per_cu < 1) per_cu = 1;
     return count < cus * per_cu ? (count > 0 ? count : 1) : cus * per_cu;
@@ -339,8 +344,8 @@ int sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_opt
             phases tp = ph;
             double ms;
             tp.bt = cand[k];
-            tp.grid32 = grid_for(sample, cand[k], lds32);
-            tp.grid64 = grid_for(sample, cand[k], lds64);
+            tp.grid32 = grid_for(sample, cand[k], lds32, 0);
+            tp.grid64 = grid_for(sample, cand[k], lds64, 1);
             ms = timed_launch(&T, &tp, &none, 0);
             if (ms < 0.0) { rc = skk_last_launch_error(); goto fail; }
             stats->tune_ms[k] = ms;
@@ -349,8 +354,8 @@ int sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_opt
     }
     if (bt != 64 && bt != 128 && bt != 256 && bt != 512) { skg_fail("block_threads must be 64..512"); rc = -1; goto fail; }
     ph.bt = bt;
-    ph.grid32 = grid_for(count, bt, lds32);
-    ph.grid64 = grid_for(count, bt, lds64);
+    ph.grid32 = grid_for(count, bt, lds32, 0);
+    ph.grid64 = grid_for(count, bt, lds64, 1);
     {
         const double ms = timed_launch(&B, &ph, &graph, o.use_graph);
         if (ms < 0.0) { rc = skk_last_launch_error(); skg_hip_fail("batch launch", rc ? rc : -1); goto fail; }
