@@ -54,7 +54,8 @@ typedef struct skb_batch {
     skb_params params;
 } skb_batch;
 
-/* LDS of one workgroup: 7 column and 9 row vectors of 4- or 8-byte words. */
+/* LDS of one workgroup: 7 column and 9 row vectors of 4- or 8-byte words
+ * and the row pointers of A and A^T (max_m + max_n + 2 ints). */
 size_t skb_lds_bytes(int max_m, int max_n, int word_bytes);
 /* One phase over the whole batch: fp64 = 0 runs the FP32 phase from a cold
  * start; fp64 = 1 runs FP64 iterates -- from a cold start when

@@ -207,17 +207,28 @@ __device__ void solve_instance(const skb_batch &B, int k, T *lds, double *sh)
     const bool resume = !kFp32 && P.mixed;
     skb_out &o = B.out[k];
     if (resume && !o.handoff) return;    // finished in the FP32 phase (uniform)
-    const Inst I = load_inst(B, k);
+    Inst I = load_inst(B, k);
     const int m = I.m, n = I.n;
     const T *av = pick<T>(I.av, I.avd), *tv = pick<T>(I.tv, I.tvd);
     const T *cg = pick<T>(I.c, I.cd), *lg = pick<T>(I.l, I.ld), *ug = pick<T>(I.u, I.ud);
     const T *rlg = pick<T>(I.rl, I.rld), *rug = pick<T>(I.ru, I.rud);
-    // LDS layout: 7 column vectors, 9 row vectors
+    // LDS layout: 7 column vectors, 9 row vectors, then the row pointers
     T *X0 = lds, *X1 = X0 + n, *xsum = X1 + n, *xr = xsum + n, *cl = xr + n, *ll = cl + n, *ul = ll + n;
     T *Y0 = ul + n, *Y1 = Y0 + m, *AX0 = Y1 + m, *AX1 = AX0 + m, *AX2 = AX1 + m, *ysum = AX2 + m,
       *yr = ysum + m, *rll = yr + m, *rul = rll + m;
     for (int j = threadIdx.x; j < n; j += BT) { cl[j] = cg[j]; ll[j] = lg[j]; ul[j] = ug[j]; }
     for (int i = threadIdx.x; i < m; i += BT) { rll[i] = rlg[i]; rul[i] = rug[i]; }
+    // the row pointers of A and A^T (m + n + 2 ints) follow in LDS: every
+    // row of every product starts with them, and an LDS read is a few
+    // hundred cycles shorter than an L2 hit
+    {
+        int *arp_l = reinterpret_cast<int *>(rul + m), *trp_l = arp_l + (m + 1);
+        for (int i = threadIdx.x; i <= m; i += BT) arp_l[i] = I.arp[i];
+        for (int j = threadIdx.x; j <= n; j += BT) trp_l[j] = I.trp[j];
+        I.arp = arp_l;
+        I.trp = trp_l;
+    }
+    __syncthreads();
 
     double anorm, eta, omega = 1.0, updates = 0.0;
     int iteration = 0, restarts = 0, fp32_iterations = 0;
@@ -501,7 +512,8 @@ extern "C" {
 
 size_t skb_lds_bytes(int max_m, int max_n, int word_bytes)
 {
-    return (size_t)word_bytes * (7 * (size_t)max_n + 9 * (size_t)max_m);
+    return (size_t)word_bytes * (7 * (size_t)max_n + 9 * (size_t)max_m) +
+           sizeof(int) * ((size_t)max_m + (size_t)max_n + 2);
 }
 
 int skb_launch(const skb_batch *B, int block_threads, int grid, int fp64)

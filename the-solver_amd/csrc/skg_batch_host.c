@@ -324,7 +324,8 @@ int sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_opt
     /* autotune the workgroup size: time four restart windows of a sample of
        instances spread over the size order, as many as one wave of the
        grid holds -- long enough that the per-iteration rate, not the
-       start-up (power iteration, first windows), decides */
+       start-up (power iteration, first windows), decides -- with FP32 and
+       (when the FP64 tail runs) with FP64 iterates */
     bt = o.block_threads;
     if (bt == 0) {
         const int cand[3] = {128, 256, 512};
@@ -348,6 +349,20 @@ int sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_opt
             tp.grid64 = grid_for(sample, cand[k], lds64, 1);
             ms = timed_launch(&T, &tp, &none, 0);
             if (ms < 0.0) { rc = skk_last_launch_error(); goto fail; }
+            if (mixed && ph.run64) {
+                /* four windows hand nothing to the FP64 tail, which is most
+                   of a batch's time and whose LDS and occupancy differ:
+                   time four windows of FP64 iterates on the sample too */
+                skb_batch T64 = T;
+                phases t64 = tp;
+                double ms64;
+                T64.params.mixed = 0;
+                t64.run32 = 0;
+                t64.run64 = 1;
+                ms64 = timed_launch(&T64, &t64, &none, 0);
+                if (ms64 < 0.0) { rc = skk_last_launch_error(); goto fail; }
+                ms += ms64;
+            }
             stats->tune_ms[k] = ms;
             if (ms < best) { best = ms; bt = cand[k]; }
         }
