@@ -76,6 +76,16 @@ typedef struct skk_bins {
 #ifndef SKK_SPMV_GRID
 #define SKK_SPMV_GRID 4096
 #endif
+/* The step decision is taken inside the next dual prox (every workgroup
+ * reducing the probe partials itself) while the A^T product and the step
+ * probe have at most SKK_FUSE_PARTIALS workgroups together; SKK_FUSE_PER
+ * partials per thread and slot are held in registers for it. */
+#ifndef SKK_FUSE_PARTIALS
+#define SKK_FUSE_PARTIALS 1024
+#endif
+#ifndef SKK_FUSE_PER
+#define SKK_FUSE_PER 4
+#endif
 /* A matrix whose FP64 entries (12 bytes each) exceed this many bytes is read
  * with non-temporal loads: it cannot stay in the caches between products,
  * so it should not evict the gathered vector.  Smaller matrices are read

@@ -671,8 +671,9 @@ __global__ __launch_bounds__(kBlock) void k_dual_decide(Iter<T> I, Data<T> D, co
 {
     __shared__ skk_state s_next;
     double dx2, dq, dy2, dya;
-    // fused only while the products have at most 1024 workgroups together
-    probe_sums<1024 / kBlock>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya);
+    // fused only while the products have at most SKK_FUSE_PARTIALS
+    // workgroups together
+    probe_sums<SKK_FUSE_PER>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya);
     if (threadIdx.x == 0) {
         decide(*snap, dx2, dq, dy2, dya, s_next);
         if (blockIdx.x == 0) *st = s_next;

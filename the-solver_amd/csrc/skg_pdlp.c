@@ -20,7 +20,7 @@ skg_tuning g_skg_tuning = {4, 512};
 
 /* Block partials (A, A^T and Q products together) up to which the step
  * decision is fused into the next dual prox. */
-#define FUSE_MAX_PARTIALS 1024
+#define FUSE_MAX_PARTIALS SKK_FUSE_PARTIALS
 
 void sk_gpu_pdlp_params_default(sk_gpu_pdlp_params *p)
 {
