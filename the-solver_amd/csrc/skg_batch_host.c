@@ -364,8 +364,15 @@ int sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_opt
                 ms += ms64;
             }
             stats->tune_ms[k] = ms;
-            if (ms < best) { best = ms; bt = cand[k]; }
+            if (ms < best) best = ms;
         }
+        /* the largest size within 3% of the fastest: the probe is short and
+           misses the end of a batch, where few instances are left and more
+           lanes per instance pay (4,096 LPs: 256 and 512 threads probe
+           within 1% of each other, and the whole batch takes 0.40 s with
+           256 against 0.35 s with 512) */
+        for (k = 0; k < 3; ++k)
+            if (stats->tune_ms[k] <= 1.03 * best) bt = cand[k];
     }
     if (bt != 64 && bt != 128 && bt != 256 && bt != 512) { skg_fail("block_threads must be 64..512"); rc = -1; goto fail; }
     ph.bt = bt;
