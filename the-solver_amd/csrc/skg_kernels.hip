@@ -31,6 +31,7 @@
 namespace {
 
 constexpr int kBlock = SKK_BLOCK;
+static_assert(kBlock >= 128, "the decision kernels use a thread of the second wave");
 constexpr int kWaves = kBlock / 64;
 constexpr int kMaxDevices = 16;
 
@@ -602,7 +603,22 @@ template <typename T, typename AT> struct EpiProbe {
 // Accept/reject decision and next step size (PdlpKernels.cu:454-493) from
 // the reduced probe sums, committed by rotating buffer indices: `out` is
 // `in` after the decision.
-__device__ void decide(const skk_state &in, double dx2, double dq, double dy2, double dya, skk_state &out)
+// The step rule's factors after `updates` decisions: (1 - (k+1)^-0.3,
+// 1 + (k+1)^-0.6) with k = updates + 1.  The decision kernels have one
+// thread of a second wave compute them while the others reduce the probe
+// partials, so the two pow calls are off the decision's critical path.
+struct StepFactors {
+    double shrink, grow;
+};
+
+__device__ StepFactors step_factors(double updates)
+{
+    const double k = updates + 1.0;
+    return StepFactors{1.0 - pow(k + 1.0, -0.3), 1.0 + pow(k + 1.0, -0.6)};
+}
+
+__device__ void decide(const skk_state &in, double dx2, double dq, double dy2, double dya, const StepFactors &f,
+                       skk_state &out)
 {
     out = in;
     // with a Hessian the local bound also covers dx'Q dx (>= 0 for convex Q)
@@ -612,9 +628,7 @@ __device__ void decide(const skk_state &in, double dx2, double dq, double dy2, d
     const double eta_bar = (inter > 0.0 && movement > 0.0) ? movement / inter : in.eta_max;
     const bool accept = eta <= eta_bar;
     const double k = in.updates + 1.0;
-    const double shrink = 1.0 - pow(k + 1.0, -0.3);
-    const double grow = 1.0 + pow(k + 1.0, -0.6);
-    double next = fmin(shrink * eta_bar, grow * eta);
+    double next = fmin(f.shrink * eta_bar, f.grow * eta);
     if (!(next > 0.0) || !isfinite(next)) next = eta;
     next = fmin(fmax(next, in.eta_min), in.eta_max);
     out.eta = next;
@@ -652,11 +666,13 @@ __device__ void probe_sums(const double *part, int stride, int dx_blocks, int dq
 __global__ __launch_bounds__(kBlock) void k_probe_finalize(skk_state *st, const double *part, int stride,
                                                            int dx_blocks, int dq_blocks, int dy_blocks)
 {
+    __shared__ StepFactors s_f;
     double dx2, dq, dy2, dya;
+    if (threadIdx.x == 64) s_f = step_factors(st->updates);   // visible after the reduction's barriers
     probe_sums<SKK_SPMV_GRID / kBlock>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya);
     if (threadIdx.x != 0) return;
     skk_state next;
-    decide(*st, dx2, dq, dy2, dya, next);
+    decide(*st, dx2, dq, dy2, dya, s_f, next);
     *st = next;
 }
 
@@ -670,12 +686,14 @@ __global__ __launch_bounds__(kBlock) void k_dual_decide(Iter<T> I, Data<T> D, co
                                                        int dx_blocks, int dq_blocks, int dy_blocks, int m)
 {
     __shared__ skk_state s_next;
+    __shared__ StepFactors s_f;
     double dx2, dq, dy2, dya;
+    if (threadIdx.x == 64) s_f = step_factors(snap->updates);   // visible after the reduction's barriers
     // fused only while the products have at most SKK_FUSE_PARTIALS
     // workgroups together
     probe_sums<SKK_FUSE_PER>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya);
     if (threadIdx.x == 0) {
-        decide(*snap, dx2, dq, dy2, dya, s_next);
+        decide(*snap, dx2, dq, dy2, dya, s_f, s_next);
         if (blockIdx.x == 0) *st = s_next;
     }
     __syncthreads();
