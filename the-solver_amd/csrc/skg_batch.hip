@@ -35,22 +35,47 @@ hipStream_t bstream()
     return static_cast<hipStream_t>(skk_stream());
 }
 
-template <int BT> __device__ __forceinline__ double wsum(double v)
+// The value a lane of the wave holds (lane uniform; v_readlane).
+__device__ __forceinline__ double lane_value(double v, int lane)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Workgroup sum of S doubles; every thread receives the totals.
-template <int BT, int S>
+// A lane's neighbour under a DPP pattern (all 64 lanes active).
+template <int Ctrl> __device__ __forceinline__ double dpp(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), Ctrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), Ctrl, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Wave sum, the same in every lane: pairs, quads, eights and rows of 16
+// through DPP lane swaps (each step adds two equal-shaped partial sums, so
+// every lane of a row ends with the same bits), then the four row sums in a
+// fixed order from v_readlane.
+__device__ __forceinline__ double wsum(double v)
+{
+    v += dpp<0xB1>(v);    // quad_perm [1,0,3,2]: lane ^ 1
+    v += dpp<0x4E>(v);    // quad_perm [2,3,0,1]: lane ^ 2
+    v += dpp<0x141>(v);   // row_half_mirror: lane i <-> 7 - i of its eight
+    v += dpp<0x140>(v);   // row_mirror: lane i <-> 15 - i of its row
+    return ((lane_value(v, 0) + lane_value(v, 16)) + lane_value(v, 32)) + lane_value(v, 48);
+}
+
+// Workgroup sum of S doubles; every thread receives the totals.  Without
+// kTail the closing barrier is left to the caller, which must pass another
+// barrier before the next wg_sum writes sh.
+template <int BT, int S, bool kTail = true>
 __device__ __forceinline__ void wg_sum(double (&v)[S], double *sh /* S * BT/64 */)
 {
     constexpr int W = BT / 64;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        const double t = wsum<BT>(v[s]);
+        const double t = wsum(v[s]);
         if (lane == 0) sh[s * W + wave] = t;
     }
     __syncthreads();
@@ -61,7 +86,7 @@ __device__ __forceinline__ void wg_sum(double (&v)[S], double *sh /* S * BT/64 *
         for (int w = 1; w < W; ++w) t += sh[s * W + w];
         v[s] = t;
     }
-    __syncthreads();
+    if (kTail) __syncthreads();
 }
 
 template <typename T> __device__ __forceinline__ T clampv(T v, T lo, T hi) { return fmin(fmax(v, lo), hi); }
@@ -74,14 +99,6 @@ __device__ __noinline__ double pow_ool(double x, double y) { return pow(x, y); }
 __device__ __noinline__ double exp_ool(double x) { return exp(x); }
 __device__ __noinline__ double log_ool(double x) { return log(x); }
 
-// The value a lane of the wave holds (lane uniform; v_readlane).
-__device__ __forceinline__ double lane_value(double v, int lane)
-{
-    const long long b = __double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
 
 // One instance's operators and data.  The operators' column indices are
 // 16-bit (an instance's vectors must fit in LDS, so it has far fewer than
@@ -346,7 +363,7 @@ __device__ void solve_instance(const skb_batch &B, int k, T *lds, double *sh)
                 acc[1] += dy * dy;
                 acc[2] += dy * ((double)s - (double)ax[i]);
             }
-            wg_sum<BT, 3>(acc, sh);
+            wg_sum<BT, 3, false>(acc, sh);   // two barriers pass before the next one
             const double inter = fabs(acc[2]);
             const double movement = 0.5 * (omega * acc[0] + acc[1] / omega);
             const double eta_bar = (inter > 0.0 && movement > 0.0) ? movement / inter : eta_max;
