@@ -331,7 +331,13 @@ __device__ __forceinline__ void spmv_tile(const int *__restrict__ rp, const int 
             const int q1 = rp[row + 1] - p0;
             for (int q = rp[row] - p0 + lane; q < q1; q += L) sum += prod[q];
         }
-        for (int off = L >> 1; off > 0; off >>= 1) sum += shfl_xor_d(sum, off);
+        // the L lanes of a row combine through DPP swaps up to 16 lanes
+        // (L is uniform; all lanes take part), then LDS permutes
+        if (L >= 2) sum += dpp<0xB1>(sum);     // lane ^ 1
+        if (L >= 4) sum += dpp<0x4E>(sum);     // lane ^ 2
+        if (L >= 8) sum += dpp<0x141>(sum);    // the other four of the eight
+        if (L >= 16) sum += dpp<0x140>(sum);   // the other eight of the row
+        for (int off = 16; off < L; off <<= 1) sum += shfl_xor_d(sum, off);
         if (g < nr && lane == 0) e.row(row, sum, part);
     } else {
         for (int g = tid; g < nr; g += kBlock) {
