@@ -3,9 +3,10 @@
 // One workgroup owns one instance at a time and takes the next one from an
 // atomic work queue (instances ordered largest first, so the long ones start
 // early and the tail is made of short ones).  The iterates of an instance
-// live in LDS in FP32; the operators A and A^T (CSR, FP32 values for the
-// iteration, FP64 values for the checks) stay in global memory, where a
-// workgroup's few tens of KB remain L2-resident.  The whole PDLP control
+// live in LDS in FP32, with the row pointers of A and A^T; their entries
+// (16-bit column indices, FP32 values for the iteration, FP64 values for
+// the checks) stay in global memory, where a workgroup's few tens of KB
+// remain L2-resident.  The whole PDLP control
 // loop of src/cuda/GpuPdlp.cpp:252-534 -- ||A|| by power iteration, the
 // adaptive step of PdlpKernels.cu:405-518, FP64 evaluation of the current
 // and averaged iterate every `period` iterations, restarts (0.2 / 0.8 /
