@@ -433,7 +433,14 @@ typedef struct sk_gpu_batch_stats {
 
 void sk_gpu_batch_options_default(sk_gpu_batch_options *o);
 /* models[count]: continuous LPs without Q.  x_out holds the instances'
- * columns back to back (sum of ncol), y_out their rows (sum of nrow). */
+ * columns back to back (sum of ncol), y_out their rows (sum of nrow).
+ * An instance's vectors live in one workgroup's LDS: the largest instance
+ * must fit (4 * (7 ncol + 9 nrow) + 4 * (ncol + nrow + 2) bytes for the
+ * FP32 phase, within the device's per-workgroup LDS; about 160 KB on
+ * MI355X), else -1 with the size in sk_gpu_last_error(); the FP64 tail is
+ * skipped when its 8-byte vectors do not fit.  block_threads = 0 picks the
+ * size by timing a sample (tune_ms); the result of an instance depends on
+ * block_threads (the reductions' shape), not on the batch around it. */
 int  sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_options *opt,
                         double *x_out, double *y_out, sk_gpu_batch_result *results,
                         sk_gpu_batch_stats *stats);
