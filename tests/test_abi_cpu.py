@@ -87,3 +87,13 @@ def test_generators_are_deterministic_and_follow_their_laws():
     import scipy.sparse as sp
     Q = sp.csc_matrix((qx, qi, qp_), shape=(120, 120)).toarray()
     assert np.allclose(Q, Q.T) and np.linalg.eigvalsh(Q).min() > 0.0
+
+
+def test_batch_validates_before_touching_the_device():
+    """sk_gpu_batch_solve rejects what the persistent kernel cannot take
+    before any device call: instances indexed by 16-bit local indices must
+    stay at or below 65,536 rows and columns."""
+    big = sg.Model.generated("skw_bounded_lp", 70000, 20000, 5, 3)
+    small = sg.Model.generated("skw_bounded_lp", 30, 60, 5, 4)
+    rc, _, _, _, _ = sg.batch_solve([small, big])
+    assert rc == -1 and "65536" in sg.last_error()
