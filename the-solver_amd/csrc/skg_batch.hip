@@ -19,6 +19,7 @@
 #include <cstdint>
 
 #include "skg_batch.h"
+#include "skg_wave.h"
 
 // Inner loops over a row's entries are unrolled by this factor: with rows
 // of 5 to 10 entries the unrolled body keeps several matrix loads in flight
@@ -36,36 +37,6 @@ hipStream_t bstream()
     return static_cast<hipStream_t>(skk_stream());
 }
 
-// The value a lane of the wave holds (lane uniform; v_readlane).
-__device__ __forceinline__ double lane_value(double v, int lane)
-{
-    const long long b = __double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-// A lane's neighbour under a DPP pattern (all 64 lanes active).
-template <int Ctrl> __device__ __forceinline__ double dpp(double v)
-{
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), Ctrl, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), Ctrl, 0xf, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-
-// Wave sum, the same in every lane: pairs, quads, eights and rows of 16
-// through DPP lane swaps (each step adds two equal-shaped partial sums, so
-// every lane of a row ends with the same bits), then the four row sums in a
-// fixed order from v_readlane.
-__device__ __forceinline__ double wsum(double v)
-{
-    v += dpp<0xB1>(v);    // quad_perm [1,0,3,2]: lane ^ 1
-    v += dpp<0x4E>(v);    // quad_perm [2,3,0,1]: lane ^ 2
-    v += dpp<0x141>(v);   // row_half_mirror: lane i <-> 7 - i of its eight
-    v += dpp<0x140>(v);   // row_mirror: lane i <-> 15 - i of its row
-    return ((lane_value(v, 0) + lane_value(v, 16)) + lane_value(v, 32)) + lane_value(v, 48);
-}
-
 // Workgroup sum of S doubles; every thread receives the totals.  Without
 // kTail the closing barrier is left to the caller, which must pass another
 // barrier before the next wg_sum writes sh.
@@ -76,7 +47,7 @@ __device__ __forceinline__ void wg_sum(double (&v)[S], double *sh /* S * BT/64 *
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        const double t = wsum(v[s]);
+        const double t = wave_sum(v[s]);
         if (lane == 0) sh[s * W + wave] = t;
     }
     __syncthreads();
