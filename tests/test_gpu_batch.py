@@ -85,3 +85,24 @@ def test_fp64_tail_and_fp64_mode_agree():
         assert abs(c @ xm[k] - c @ xd[k]) <= 1e-6 * (1 + abs(c @ xd[k])), k
         assert 0 < rm[k].fp32_iterations <= rm[k].iterations
         assert max(so.relative_kkt(md, xm[k], ym[k])[:3]) <= 1e-6
+
+
+def test_full_size_batch_sample_meets_the_tolerance_in_the_oracle():
+    """The config-3 batch at its full size (4,096 LPs, m ~ U{20..400}, the
+    benchmark's seed, autotuned block size): every instance reports
+    convergence (eps 1e-6 on the scaled instance), and a spread sample of 64
+    returned points has the oracle's FP64 relative KKT of the unscaled
+    instance within 1e-5 (the bar the smaller test above uses), with the
+    reported objective c'x."""
+    count, seed = 4096, 20260828
+    models = _models(count, 20, 400, seed)
+    rc, xs, ys, res, st = sg.batch_solve(models, eps_optimal=1e-6, max_iterations=200000)
+    assert rc == 0, sg.last_error()
+    assert st.converged == count and all(r.status == 0 for r in res)
+    assert st.block_threads in (128, 256, 512)
+    for k in range(0, count, count // 64):
+        md = models[k]
+        kkt = so.relative_kkt(md, xs[k], ys[k])
+        assert max(kkt[:3]) <= 1e-5, (k, kkt)
+        c = md.vec("c")
+        assert abs(res[k].objective - c @ xs[k]) <= 1e-9 * (1 + abs(c @ xs[k])), k
