@@ -139,3 +139,39 @@ def test_gather_rate_probe():
     assert lib.sk_gpu_gather_rate(1 << 20, 1 << 22, 3, C.byref(rate)) == 0, sg.last_error()
     assert 1e9 < rate.value < 1e13
     assert lib.sk_gpu_gather_rate(0, 10, 1, C.byref(rate)) == -1
+
+
+def test_headline_size_products_match_scipy():
+    """Both products at the headline LP's full size (1M x 2M, 15.8M nonzeros,
+    Zipf(1.2) column degrees up to 2,000, so A^T has rows long enough for the
+    whole-workgroup path) against scipy in FP64, entry by entry within
+    1e-12 of each row's absolute mass."""
+    w = sg.load_workloads()
+    md = sg.Model()
+    q = C.c_double()
+    assert w.skw_zipf_lp(C.byref(md.m), 1_000_000, 2_000_000, 1.2, 8.0, 2000, 0.3, 20260826, C.byref(q)) == 0
+    md._generated = True
+    p, i, v = md.csc()
+    import scipy.sparse as sp
+    A = sp.csc_matrix((v, i, p), shape=(md.nrow, md.ncol)).tocsr()
+    A.sort_indices()
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(md.ncol)
+    y = rng.standard_normal(md.nrow)
+    lib = sg.load_gpu()
+    h = sg.GpuCsr()
+    rp, cj, vals = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data
+    assert lib.sk_gpu_csr_create(C.byref(h), md.nrow, md.ncol, int(rp[-1]), sg.iptr(rp), sg.iptr(cj),
+                                 sg.dptr(vals)) == 0, sg.last_error()
+    dx, dy = _dev(x), _dev(np.full(md.nrow, np.nan))
+    dyin, dxo = _dev(y), _dev(np.full(md.ncol, np.nan))
+    _sync()
+    assert lib.sk_gpu_spmv_device_f64(C.byref(h), dx.value, dy.value) == 0
+    assert lib.sk_gpu_spmv_transpose_device_f64(C.byref(h), dyin.value, dxo.value) == 0
+    _sync()
+    absA = abs(A)
+    scale = absA @ np.abs(x) + 1e-300
+    assert np.all(np.abs(dy.numpy() - A @ x) <= 1e-12 * scale)
+    scale_t = absA.T @ np.abs(y) + 1e-300
+    assert np.all(np.abs(dxo.numpy() - A.T @ y) <= 1e-12 * scale_t)
+    lib.sk_gpu_csr_destroy(C.byref(h))
