@@ -219,3 +219,24 @@ def test_qp_fused_decision_matches_separate_launch(prec):
     assert b[3].kernel_launches < a[3].kernel_launches
     kkt = so.relative_kkt(model, b[1], b[2])
     assert max(kkt[:3]) <= 1e-6, kkt
+
+
+def test_headline_size_first_iterations_match_the_oracle():
+    """The FP64 engine on the headline LP at its full size (1M x 2M, 15.8M
+    nonzeros) against the oracle's FP64 PDLP (Ruiz + Pock-Chambolle scaling,
+    30 power iterations, adaptive steps), both stopped after 8 iterations:
+    the same ||A|| and the same iterate, up to the summation order of the
+    products and reductions."""
+    import ctypes as C
+    w = sg.load_workloads()
+    model = sg.Model()
+    q = C.c_double()
+    assert w.skw_zipf_lp(C.byref(model.m), 1_000_000, 2_000_000, 1.2, 8.0, 2000, 0.3, 20260826, C.byref(q)) == 0
+    model._generated = True
+    P = dict(max_iterations=8, precision=sg.PREC_FP64, use_graph=0)
+    rc_o, xo, yo, sto = so.first_order_pdlp(model, sg.default_pdlp_params(**P))
+    rc, x, y, st = gpu_pdlp(model, **P)
+    assert rc == 0 and rc_o == 0 and st.iterations == 8 and sto.iterations == 8
+    assert abs(st.matrix_norm - sto.matrix_norm) <= 1e-10 * sto.matrix_norm
+    assert np.linalg.norm(x - xo) <= 1e-8 * (1.0 + np.linalg.norm(xo))
+    assert np.linalg.norm(y - yo) <= 1e-8 * (1.0 + np.linalg.norm(yo))
