@@ -182,8 +182,13 @@ int  sk_gpu_csr_create(sk_gpu_csr *matrix, int rows, int cols, int nnz,
 void sk_gpu_csr_destroy(sk_gpu_csr *matrix);
 
 /* cuda/include/sankhya_cuda.h:38-42.  Device pointers; enqueued on the
- * library stream, return after launch.  0, -1 (bad argument) or a positive
- * hipError_t.  The transpose product is deterministic (no atomics). */
+ * library stream, return after launch.  That stream is a blocking stream,
+ * so, as with the reference's default-stream contract (sankhya_cuda.h:
+ * 35-37), work the caller queued on the null stream before the call (e.g.
+ * a hipMemcpyAsync into x) runs first, and a later null-stream hipMemcpy of
+ * y waits for the product; no sk_gpu_synchronize is needed for that.
+ * 0, -1 (bad argument) or a positive hipError_t.  The transpose product is
+ * deterministic (no atomics). */
 int sk_gpu_spmv_device_f64(const sk_gpu_csr *matrix, const double *device_x,
                            double *device_y);
 int sk_gpu_spmv_transpose_device_f64(const sk_gpu_csr *matrix,
@@ -198,6 +203,14 @@ int sk_gpu_spmv_f64(int rows, int cols, int nnz, const int *row_offsets,
                     const int *column_indices, const double *values,
                     const double *x, double *y);
 int sk_gpu_axpy_f64(int n, double alpha, const double *x, double *y);
+/* FP32 counterpart of sankhya_cuda_spmv_f64 (sankhya_cuda.h:153-158; no FP32
+ * product there): the mixed engine's iteration product on host arrays --
+ * values rounded to FP32 once, FP32 x, FP32 products and sums.
+ * transpose = 1 computes y = A^T x through the stored CSR(A^T), as the
+ * engine's A^T y does (x has `rows` entries, y `cols`).  0 or -1. */
+int sk_gpu_spmv_f32(int rows, int cols, int nnz, const int *row_offsets,
+                    const int *column_indices, const double *values, int transpose,
+                    const float *x, float *y);
 
 /* cuda/include/sankhya_cuda.h:167: thread-local text of the last failure. */
 const char *sk_gpu_last_error(void);
@@ -292,9 +305,10 @@ typedef struct sk_gpu_pdlp_params {  /* src/cuda/GpuPdlp.hpp:19-67 PdlpParams */
     double eta_min_factor;   /* 1e-6                                        */
     double eta_max_factor;   /* 1e4                                         */
     int    precision;        /* SK_GPU_PREC_*                               */
-    double tail_switch;      /* MIXED: KKT at which FP64 takes over (1e-5)  */
-    int    tail_stall_windows; /* MIXED: windows without progress before the
-                                  FP64 tail starts early (32)               */
+    double tail_switch;      /* MIXED, refine 0: KKT at which FP64 takes
+                                over (1e-5)                                 */
+    int    tail_stall_windows; /* MIXED, refine 0: windows without progress
+                                  before the FP64 tail starts early (32)    */
     int    scaling;          /* 1: Ruiz then Pock-Chambolle, 0: none        */
     int    ruiz_iterations;  /* 100                                         */
     double ruiz_tol;         /* 1e-2                                        */
@@ -307,8 +321,27 @@ typedef struct sk_gpu_pdlp_params {  /* src/cuda/GpuPdlp.hpp:19-67 PdlpParams */
                                 dual prox (the result is identical either way;
                                 fusion saves a launch per iteration when the
                                 products have few workgroups)             */
+    int    refine;           /* MIXED: 1 (default) anchored refinement -- the
+                                FP32 iterates hold the correction to an FP64
+                                anchor that absorbs the restart point at
+                                every restart, so FP32 rounding scales with
+                                the distance still to go, and no FP64 tail
+                                runs.  0: FP32 iterates of the full point,
+                                FP64 tail from tail_switch / a stall.       */
+    const volatile int *cancel; /* GpuPdlp.hpp:60-66 cancel: polled at every
+                                window read-back; nonzero stops the solve
+                                with stats->cancelled = 1.  NULL: never.    */
+    int    restart_rule;     /* SK_GPU_RESTART_KKT (0, the reference's
+                                relative-KKT merit, GpuPdlp.cpp:437-451) or
+                                SK_GPU_RESTART_NDG (1, normalized duality
+                                gap evaluated in FP64)                      */
     int    pad;
 } sk_gpu_pdlp_params;
+
+enum {
+    SK_GPU_RESTART_KKT = 0,
+    SK_GPU_RESTART_NDG = 1
+};
 
 typedef struct sk_gpu_pdlp_stats {   /* src/cuda/GpuPdlp.hpp:69-92 PdlpStats */
     int    converged;
@@ -333,6 +366,11 @@ typedef struct sk_gpu_pdlp_stats {   /* src/cuda/GpuPdlp.hpp:69-92 PdlpStats */
     double loop_seconds;         /* device-resident loop only               */
     double setup_seconds;        /* scaling, binning, upload                */
     double primal_weight;
+    int    reanchors;            /* anchored refinement: anchor moves       */
+    int    iterations_to_merit[4]; /* iterations at the first window whose
+                                    relative KKT merit max(primal, dual, gap)
+                                    is <= 1e-3, 1e-4, 1e-5, 1e-6 (0: never) */
+    double seconds_to_merit[4];  /* loop seconds at those windows           */
 } sk_gpu_pdlp_stats;
 
 void sk_gpu_pdlp_params_default(sk_gpu_pdlp_params *p);
@@ -389,6 +427,10 @@ int  sk_gpu_pdlp_session_solve(sk_gpu_pdlp_session *session, const double *x0,
                                sk_gpu_pdlp_stats *stats);
 int  sk_gpu_pdlp_session_iterate(sk_gpu_pdlp_session *session, int iterations,
                                  double *kernel_ms);
+/* The stopping limits of later solves (GpuPdlp.hpp:19-37 max_iterations,
+ * time_limit_seconds, eps_optimal); the resident operator is kept. */
+int  sk_gpu_pdlp_session_set_limits(sk_gpu_pdlp_session *session, int max_iterations,
+                                    double time_limit_seconds, double eps_optimal);
 void sk_gpu_pdlp_session_destroy(sk_gpu_pdlp_session *session);
 
 /* ------------------------------------------------------------------------
@@ -477,19 +519,27 @@ int       sk_gpu_mlp_gradients(sk_gpu_mlp *mlp, const float *X, const float *Y, 
 /* ------------------------------------------------------------------------
  * sk_solve-style entry point.  Mirrors sk_solve (c/include/sankhya.h:173,
  * c/src/sk_solve.c:1009-1046) for continuous LP / convex QP, running the
- * GPU engine; OPTIMAL is reported only after the FP64 certificate computed
- * on the device passes.
+ * GPU engine.  OPTIMAL is reported only when, on the device in FP64, the
+ * relative KKT triple of the original model is within pdlp.eps_optimal AND
+ * the sk_verify certificate is within the reference's margins
+ * (c/src/sk_solve.c:987-996): primal infeasibility <= 100 primal_tol, and
+ * for a QP dual infeasibility and complementarity <= 100 dual_tol.  A
+ * converged loop that misses either continues from its point with a 10x
+ * tighter tolerance (at most three times) within the iteration and time
+ * limits left; otherwise ITERATION_LIMIT or TIME_LIMIT (sk_solve.c:997-999),
+ * NUMERIC_FAILURE for a non-finite point.  `g` is read only (NULL:
+ * defaults); statistics go to `stats` (may be NULL).
  * ---------------------------------------------------------------------- */
 typedef struct sk_gpu_options {
     sk_gpu_pdlp_params pdlp;
     const double *x0;        /* optional warm start, original space          */
     const double *y0;
-    sk_gpu_pdlp_stats stats; /* out: solve statistics                        */
 } sk_gpu_options;
 
 void      sk_gpu_solver_options_default(sk_gpu_options *g);
 sk_status sk_gpu_solve(const sk_model *m, const sk_options *o,
-                       sk_gpu_options *g, sk_solution *s);
+                       const sk_gpu_options *g, sk_solution *s,
+                       sk_gpu_pdlp_stats *stats);
 /* c/include/sankhya.h:177 sk_verify, evaluated on the device in FP64. */
 sk_status sk_gpu_verify(const sk_model *m, sk_solution *s);
 

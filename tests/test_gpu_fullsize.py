@@ -1,0 +1,189 @@
+"""Parity at the benchmark shapes, through the C ABI.
+
+* The mixed engine's FP32 products (sk_gpu_spmv_f32: FP32 values, FP32
+  gathers, FP32 sums -- the iteration kernel) against the FP64 product of the
+  same data, within the standard forward error bound of a rounded dot
+  product: |fl(a.x) - a.x| <= gamma_{k+2} sum |a_i x_i| with
+  gamma_k = k u / (1 - k u), u = 2^-24 and k the row length (k terms summed
+  in any order, plus one rounding of the value to FP32 and one of the
+  product; x is FP32 already).
+* The Hessian product of the config-2 QP (500k x 500k, Q = D + B'B) against
+  scipy, FP64 and FP32.
+* The config-2 QP solved by both engines, each result certified by the
+  oracle's FP64 relative KKT and agreeing in objective.
+* The mixed engine's first restart window on the headline LP against the
+  oracle's FP64 PDLP.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import sankhya_gpu as sg
+import sk_oracle as so
+from helpers import gpu_pdlp
+
+pytestmark = pytest.mark.gpu
+
+U32 = 2.0 ** -24
+
+
+def _gamma(k):
+    return k * U32 / (1.0 - k * U32)
+
+
+def _headline():
+    w = sg.load_workloads()
+    md = sg.Model()
+    q = C.c_double()
+    assert w.skw_zipf_lp(C.byref(md.m), 1_000_000, 2_000_000, 1.2, 8.0, 2000, 0.3, 20260826, C.byref(q)) == 0
+    md._generated = True
+    return md
+
+
+def _csr(model):
+    p, i, v = model.csc()
+    A = sp.csc_matrix((v, i, p), shape=(model.nrow, model.ncol)).tocsr()
+    A.sort_indices()
+    return A
+
+
+def _f32_product(A, x32, transpose):
+    lib = sg.load_gpu()
+    rp, cj = A.indptr.astype(np.int32), A.indices.astype(np.int32)
+    out = np.full(A.shape[1] if transpose else A.shape[0], np.nan, np.float32)
+    fp = C.POINTER(C.c_float)
+    rc = lib.sk_gpu_spmv_f32(A.shape[0], A.shape[1], int(rp[-1]), sg.iptr(rp), sg.iptr(cj), sg.dptr(A.data),
+                             int(transpose), x32.ctypes.data_as(fp), out.ctypes.data_as(fp))
+    assert rc == 0, sg.last_error()
+    return out
+
+
+def _check_f32(A, x32, transpose):
+    """FP32 product vs the FP64 product of the FP64 matrix with the same x,
+    row by row within gamma_{k+2} of the row's absolute mass."""
+    y32 = _f32_product(A, x32, transpose).astype(np.float64)
+    M = A.T.tocsr() if transpose else A
+    x = x32.astype(np.float64)
+    exact = M @ x
+    mass = abs(M) @ np.abs(x)
+    k = np.diff(M.indptr).astype(np.float64)
+    bound = _gamma(k + 2) * mass
+    err = np.abs(y32 - exact)
+    assert np.all(np.isfinite(y32))
+    assert np.all(err <= bound), (float(np.max(err / np.maximum(bound, 1e-300))))
+    # the bound is not vacuous: the typical error is far inside it, and a
+    # product with one entry dropped per row would not fit it
+    return float(np.median(err[mass > 0] / bound[mass > 0]))
+
+
+def test_fp32_products_known_answer_and_small_family():
+    off = np.array([0, 2, 4], np.int32)
+    A = sp.csr_matrix((np.array([1.0, 2.0, 3.0, 4.0]), np.array([0, 2, 1, 2]), off), shape=(2, 3))
+    y = _f32_product(A, np.array([2, 5, 7], np.float32), False)
+    assert np.array_equal(y, [16.0, 43.0])
+    yt = _f32_product(A, np.array([16, 43], np.float32), True)
+    assert np.array_equal(yt, [16.0, 129.0, 204.0])
+    rng = np.random.default_rng(11)
+    for seed, (m, n, dens) in enumerate([(300, 500, 0.02), (64, 40000, 0.01), (5000, 70, 0.3)]):
+        A = sp.random(m, n, density=dens, format="csr", random_state=seed, data_rvs=rng.standard_normal)
+        A.sort_indices()
+        _check_f32(A, rng.standard_normal(n).astype(np.float32), False)
+        _check_f32(A, rng.standard_normal(m).astype(np.float32), True)
+
+
+def test_headline_size_fp32_products_within_fp32_bound():
+    """Both FP32 products of the 1M x 2M headline matrix (rows of A^T up to
+    2,000 entries: the whole-workgroup path)."""
+    A = _csr(_headline())
+    rng = np.random.default_rng(5)
+    med = _check_f32(A, rng.standard_normal(A.shape[1]).astype(np.float32), False)
+    med_t = _check_f32(A, rng.standard_normal(A.shape[0]).astype(np.float32), True)
+    assert med < 0.25 and med_t < 0.25, (med, med_t)
+
+
+def _config2():
+    return sg.Model.generated("skw_sparse_qp", 250000, 500000, 6, 50000, 3, 20260827)
+
+
+def test_config2_hessian_product_matches_scipy():
+    """Q x of the config-2 QP (n = 500,000, Q = D + B'B with B 50,000 x n,
+    3 per row) on the device: FP64 within 1e-12 of each row's mass, FP32
+    within the FP32 bound."""
+    model = _config2()
+    qp, qi, qv = model.q_csc()
+    Q = sp.csc_matrix((qv, qi, qp), shape=(model.ncol, model.ncol)).tocsr()
+    Q.sort_indices()
+    assert abs(Q - Q.T).max() <= 1e-12 * abs(Q).max()
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(model.ncol)
+    y = np.full(model.ncol, np.nan)
+    rp, cj = Q.indptr.astype(np.int32), Q.indices.astype(np.int32)
+    assert sg.load_gpu().sk_gpu_spmv_f64(model.ncol, model.ncol, int(rp[-1]), sg.iptr(rp), sg.iptr(cj),
+                                         sg.dptr(Q.data), sg.dptr(x), sg.dptr(y)) == 0, sg.last_error()
+    assert np.all(np.abs(y - Q @ x) <= 1e-12 * (abs(Q) @ np.abs(x) + 1e-300))
+    _check_f32(Q, x.astype(np.float32), False)
+    A = _csr(model)
+    _check_f32(A, x.astype(np.float32), False)
+    _check_f32(A, rng.standard_normal(model.nrow).astype(np.float32), True)
+
+
+@pytest.mark.parametrize("prec", [sg.PREC_FP64, sg.PREC_MIXED])
+def test_config2_solve_certified_by_the_oracle(prec):
+    """The config-2 QP at its real shape (nnz(A) 3M, nnz(Q) ~0.8M), solved to
+    1e-6: the oracle's FP64 relative KKT of the returned point is <= 1e-6
+    (slack 1.05 for the summation order of the FP64 evaluation), and the two
+    engines' objectives agree within 1e-5 relative (each is within ~1e-6 of
+    the optimum)."""
+    model = _config2()
+    p, i, v = model.csc()
+    rp, cj, vv = sg.csc_to_csr(model.nrow, model.ncol, p, i, v)
+    vec = {k: model.vec(k) for k in ("c", "rlow", "rupp", "clow", "cupp")}
+    q = model.q_csc()
+    lib = sg.load_gpu()
+    P = sg.default_pdlp_params(eps_optimal=1e-6, precision=prec, use_graph=1, time_limit_seconds=60.0)
+    x = np.zeros(model.ncol)
+    y = np.zeros(model.nrow)
+    st = sg.PdlpStats()
+    rc = lib.sk_gpu_pdlp_solve(model.nrow, model.ncol, int(rp[-1]), sg.iptr(rp), sg.iptr(cj), sg.dptr(vv),
+                               sg.iptr(q[0]), sg.iptr(q[1]), sg.dptr(q[2]), int(q[0][-1]), sg.dptr(vec["c"]),
+                               sg.dptr(vec["rlow"]), sg.dptr(vec["rupp"]), sg.dptr(vec["clow"]),
+                               sg.dptr(vec["cupp"]), None, None, C.byref(P), sg.dptr(x), sg.dptr(y), C.byref(st))
+    assert rc == 0 and st.converged, (sg.last_error(), st.as_dict())
+    kkt = so.relative_kkt(model, x, y)
+    assert max(kkt[:3]) <= 1.05e-6, kkt
+    if prec == sg.PREC_MIXED:
+        # anchored refinement: no FP64 tail was needed
+        assert st.fp32_iterations >= 0.9 * st.iterations, st.as_dict()
+    test_config2_solve_certified_by_the_oracle.objs[prec] = float(kkt[3])
+    if len(test_config2_solve_certified_by_the_oracle.objs) == 2:
+        a, b = test_config2_solve_certified_by_the_oracle.objs.values()
+        assert abs(a - b) <= 1e-5 * (1 + abs(a)), (a, b)
+
+
+test_config2_solve_certified_by_the_oracle.objs = {}
+
+
+def test_headline_mixed_engine_first_window_matches_the_oracle():
+    """The mixed engine (FP32 iterates anchored at the FP64 start point) on
+    the 1M x 2M headline LP for one restart window (64 adaptive iterations,
+    then the first restart) against the oracle's FP64 PDLP stopped at the
+    same point: the same ||A|| (FP64 power iteration in both), and iterates
+    equal up to FP32 rounding, held to 1e-4 of the iterate's norm (64
+    iterations of a non-expansive map, each adding O(100 u) relative
+    rounding through its three products)."""
+    model = _headline()
+    P = dict(max_iterations=64, use_graph=0)
+    rc_o, xo, yo, sto = so.first_order_pdlp(model, sg.default_pdlp_params(precision=sg.PREC_FP64, **P))
+    rc, x, y, st = gpu_pdlp(model, precision=sg.PREC_MIXED, **P)
+    assert rc == 0 and rc_o == 0 and st.iterations == 64 and sto.iterations == 64
+    assert st.fp32_iterations == 64
+    assert abs(st.matrix_norm - sto.matrix_norm) <= 1e-10 * sto.matrix_norm
+    ex = np.linalg.norm(x - xo) / (1.0 + np.linalg.norm(xo))
+    ey = np.linalg.norm(y - yo) / (1.0 + np.linalg.norm(yo))
+    assert ex <= 1e-4 and ey <= 1e-4, (ex, ey)
+    # and the FP64 evaluations of the window agree with the oracle's
+    for a, b in ((st.relative_primal_residual, sto.relative_primal_residual),
+                 (st.relative_gap, sto.relative_gap)):
+        assert abs(a - b) <= 1e-3 * abs(b) + 1e-12, (a, b)

@@ -23,7 +23,7 @@ def _solve(model, prec):
     lib.sk_gpu_solver_options_default(C.byref(g))
     g.pdlp.precision = prec
     s = sg.SkSolution()
-    rc = lib.sk_gpu_solve(model.ptr(), C.byref(o), C.byref(g), C.byref(s))
+    rc = lib.sk_gpu_solve(model.ptr(), C.byref(o), C.byref(g), C.byref(s), None)
     x = np.ctypeslib.as_array(s.x, shape=(model.ncol,)).copy() if s.x else None
     out = (rc, s.result, s.objective, x)
     lib.sk_gpu_solution_free(C.byref(s))

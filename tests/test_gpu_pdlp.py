@@ -183,7 +183,7 @@ def test_qp_reference_first_order_fixture():
     go = sg.GpuOptions(); lib.sk_gpu_solver_options_default(C.byref(go))
     go.pdlp.eps_optimal = 1e-7
     s = sg.SkSolution()
-    assert lib.sk_gpu_solve(model.ptr(), C.byref(o), C.byref(go), C.byref(s)) == 0
+    assert lib.sk_gpu_solve(model.ptr(), C.byref(o), C.byref(go), C.byref(s), None) == 0
     assert s.result == sg.SK_RESULT_OPTIMAL, sg.last_error()
     x = np.ctypeslib.as_array(s.x, shape=(model.ncol,)).copy()
     y = np.ctypeslib.as_array(s.y, shape=(model.nrow,)).copy()

@@ -20,8 +20,9 @@ def _solve(model, **params):
     for k, v in params.items():
         setattr(g.pdlp, k, v)
     s = sg.SkSolution()
-    rc = lib.sk_gpu_solve(model.ptr(), C.byref(o), C.byref(g), C.byref(s))
-    return rc, s, g
+    st = sg.PdlpStats()
+    rc = lib.sk_gpu_solve(model.ptr(), C.byref(o), C.byref(g), C.byref(s), C.byref(st))
+    return rc, s, st
 
 
 def test_gpu_verify_matches_oracle_on_simplex_certificate():
@@ -64,4 +65,4 @@ def test_solve_rejects_bad_engine_and_structure():
     lib.sk_gpu_options_default(C.byref(o))
     o.lp_engine = 99
     s = sg.SkSolution()
-    assert lib.sk_gpu_solve(model.ptr(), C.byref(o), None, C.byref(s)) == sg.SK_ERR_ARG
+    assert lib.sk_gpu_solve(model.ptr(), C.byref(o), None, C.byref(s), None) == sg.SK_ERR_ARG

@@ -132,6 +132,50 @@ int sk_gpu_spmv_f64(int rows, int cols, int nnz, const int *row_offsets, const i
     return rc;
 }
 
+/* The mixed engine's FP32 product on host arrays: the FP64 values are
+ * rounded to FP32 once, x is FP32, products and sums are FP32 (the kernel
+ * of the FP32 iterations).  transpose = 1 multiplies by A^T through the
+ * stored CSR(A^T), as the engine does. */
+int sk_gpu_spmv_f32(int rows, int cols, int nnz, const int *row_offsets, const int *column_indices,
+                    const double *values, int transpose, const float *x, float *y)
+{
+    hcsr a, at;
+    dcsr d;
+    float *dx = NULL, *dy = NULL;
+    int rc, in, out;
+    skg_clear_error();
+    if (rows < 0 || cols < 0 || nnz < 0) return skg_fail("negative CSR dimension");
+    if (!row_offsets || !y || (nnz > 0 && (!column_indices || !values)) || !x)
+        return skg_fail("null CSR or vector pointer");
+    if (hcsr_validate(rows, cols, nnz, row_offsets, column_indices, values)) return -1;
+    in = transpose ? rows : cols;
+    out = transpose ? cols : rows;
+    if (out == 0) return 0;
+    if (skg_have_device()) return -1;
+    a.rows = rows; a.cols = cols; a.nnz = nnz;
+    a.p = (int *)row_offsets; a.j = (int *)column_indices; a.v = (double *)values;
+    memset(&at, 0, sizeof(at));
+    if (transpose && hcsr_transpose(&a, &at)) return skg_fail("out of host memory");
+    rc = dcsr_upload(&d, transpose ? &at : &a, 1);
+    hcsr_free(&at);
+    if (rc) return -1;
+    if ((rc = skk_malloc((void **)&dx, sizeof(float) * (size_t)(in ? in : 1))) ||
+        (rc = skk_malloc((void **)&dy, sizeof(float) * (size_t)out)) ||
+        (rc = skk_h2d(dx, x, sizeof(float) * (size_t)in))) {
+        skk_free(dx); skk_free(dy); dcsr_free(&d);
+        return skg_hip_fail("SpMV staging", rc);
+    }
+    if (d.k.ntiles == 0) {   /* no rows to tile: the product is zero */
+        memset(y, 0, sizeof(float) * (size_t)out);
+        skk_free(dx); skk_free(dy); dcsr_free(&d);
+        return 0;
+    }
+    if ((rc = skk_spmv_f32(&d.k, dx, dy))) rc = skg_hip_fail("FP32 SpMV launch", rc);
+    if (!rc && (rc = skk_d2h(y, dy, sizeof(float) * (size_t)out))) rc = skg_hip_fail("copy y", rc);
+    skk_free(dx); skk_free(dy); dcsr_free(&d);
+    return rc;
+}
+
 int sk_gpu_axpy_f64(int n, double alpha, const double *x, double *y)
 {                                                  /* cuda/src/sankhya_cuda.cu:254-284 */
     double *dx = NULL, *dy = NULL;
@@ -562,21 +606,78 @@ static int interval_infeasible(const sk_model *m, double tol)
     return impossible;
 }
 
-/* sk_solve (c/src/sk_solve.c:1009-1046) for continuous LPs on the GPU PDLP
- * engine.  OPTIMAL needs the relative KKT triple of the ORIGINAL model,
- * evaluated on the device in FP64, to be within eps; when the scaled loop
- * converges but the original-space triple is not yet within eps, the solve
- * continues from the same point with a 10x tighter loop tolerance (at most
- * three times). */
-sk_status sk_gpu_solve(const sk_model *m, const sk_options *o, sk_gpu_options *g, sk_solution *s)
+/* Adds one round's statistics to the running totals of an sk_gpu_solve. */
+static void stats_accumulate(sk_gpu_pdlp_stats *acc, const sk_gpu_pdlp_stats *st, int round)
+{
+    int k;
+    if (round == 0) { *acc = *st; return; }
+    for (k = 0; k < 4; ++k)
+        if (!acc->iterations_to_merit[k] && st->iterations_to_merit[k]) {
+            acc->iterations_to_merit[k] = acc->iterations + st->iterations_to_merit[k];
+            acc->seconds_to_merit[k] = acc->loop_seconds + st->seconds_to_merit[k];
+        }
+    acc->iterations += st->iterations;
+    acc->fp32_iterations += st->fp32_iterations;
+    acc->fp64_iterations += st->fp64_iterations;
+    acc->restarts += st->restarts;
+    acc->reanchors += st->reanchors;
+    acc->host_syncs += st->host_syncs;
+    acc->kernel_launches += st->kernel_launches;
+    acc->loop_seconds += st->loop_seconds;
+    acc->setup_seconds += st->setup_seconds;
+    acc->seconds += st->seconds;
+    acc->converged = st->converged;
+    acc->cancelled = st->cancelled;
+    acc->primal_objective = st->primal_objective;
+    acc->dual_objective = st->dual_objective;
+    acc->relative_primal_residual = st->relative_primal_residual;
+    acc->relative_dual_residual = st->relative_dual_residual;
+    acc->relative_gap = st->relative_gap;
+    acc->final_step_size = st->final_step_size;
+    acc->step_size_ratio = st->step_size_ratio;
+    acc->primal_weight = st->primal_weight;
+    acc->rejected_steps += st->rejected_steps;
+}
+
+/* The certificate margins of the reference's first-order path
+ * (c/src/sk_solve.c:987-996): a QP needs sk_verify's primal infeasibility
+ * within 100 primal_tol and its dual infeasibility and complementarity
+ * within 100 dual_tol.  An LP there stops only with its largest row
+ * violation within primal_tol (sk_solve.c:947); this engine stops on the
+ * relative KKT instead, so an LP needs the verified primal infeasibility
+ * within the same 100 primal_tol margin (its dual sign test, with the 1e-9
+ * "at bound" rule, is not an LP first-order certificate: sk_solve.c:983-985). */
+static int certificate_ok(const sk_model *m, const sk_options *o, const sk_solution *s)
+{
+    const double pt = o->primal_tol > 0.0 ? o->primal_tol : 1e-7;
+    const double dt = o->dual_tol > 0.0 ? o->dual_tol : 1e-7;
+    if (!(s->primal_infeasibility <= 100.0 * pt)) return 0;
+    if (m->Q && !(s->dual_infeasibility <= 100.0 * dt && s->complementarity <= 100.0 * dt)) return 0;
+    return 1;
+}
+
+/* sk_solve (c/src/sk_solve.c:1009-1046) for continuous LP / convex QP on the
+ * GPU PDLP engine.  OPTIMAL needs both the relative KKT triple of the
+ * ORIGINAL model and the sk_verify certificate (certificate_ok), each
+ * evaluated on the device in FP64.  When the scaled loop converges but
+ * either is not yet met, the solve continues from the same point with a 10x
+ * tighter loop tolerance (at most three times), within what is left of the
+ * iteration and time limits.  Otherwise the result is ITERATION_LIMIT or
+ * TIME_LIMIT, as the reference reports an unconverged first-order solve
+ * (sk_solve.c:997-999); NUMERIC_FAILURE for a non-finite result. */
+sk_status sk_gpu_solve(const sk_model *m, const sk_options *o, const sk_gpu_options *g, sk_solution *s,
+                       sk_gpu_pdlp_stats *stats)
 {
     sk_options od;
     sk_gpu_options gd;
+    sk_gpu_pdlp_stats acc;
     hcsr A, Qh;
     double *x, *y, kkt[5], t0 = skg_now(), eps;
-    int j, rc, round;
+    int j, rc, round, verified = 0, cert = 0;
     long long iters = 0;
     skg_clear_error();
+    memset(&acc, 0, sizeof(acc));
+    if (stats) memset(stats, 0, sizeof(*stats));
     if (!m || !s) return SK_ERR_ARG;
     if (sk_gpu_model_validate(m) != SK_OK) return SK_ERR_STRUCTURE;
     if (!o) { sk_gpu_options_default(&od); o = &od; }
@@ -595,42 +696,51 @@ sk_status sk_gpu_solve(const sk_model *m, const sk_options *o, sk_gpu_options *g
     if (m->Q) { Qh.rows = Qh.cols = m->ncol; Qh.nnz = m->Q->p[m->ncol]; Qh.p = m->Q->p; Qh.j = m->Q->i; Qh.v = m->Q->x; }
     x = (double *)calloc((size_t)m->ncol + 1, sizeof(double));
     y = (double *)calloc((size_t)m->nrow + 1, sizeof(double));
-    if (!x || !y) { free(x); free(y); hcsr_free(&A); return SK_ERR_MEMORY; }
+    sk_gpu_solution_init(s);
+    s->rowact = (double *)calloc((size_t)m->nrow + 1, sizeof(double));
+    s->rc = (double *)calloc((size_t)m->ncol + 1, sizeof(double));
+    if (!x || !y || !s->rowact || !s->rc) {
+        free(x); free(y); free(s->rowact); free(s->rc); s->rowact = s->rc = NULL;
+        hcsr_free(&A); return SK_ERR_MEMORY;
+    }
+    s->x = x; s->y = y;
+    s->ncol = m->ncol; s->nrow = m->nrow;
     {
         sk_gpu_pdlp_params P = g->pdlp;
         const double *wx = g->x0, *wy = g->y0;
         if (o->iteration_limit > 0 && o->iteration_limit < P.max_iterations) P.max_iterations = (int)o->iteration_limit;
-        if (o->time_limit > 0.0) P.time_limit_seconds = o->time_limit;
         eps = P.eps_optimal;
-        memset(&g->stats, 0, sizeof(g->stats));
         kkt[0] = kkt[1] = kkt[2] = INFINITY;
         for (round = 0; round < 4; ++round) {
             sk_gpu_pdlp_stats st;
+            if (o->time_limit > 0.0) {
+                /* what is left of the caller's limit for this round */
+                P.time_limit_seconds = o->time_limit - (skg_now() - t0);
+                if (P.time_limit_seconds <= 0.0) break;
+            }
             /* the full symmetric Q's CSC arrays read as CSR are Q itself */
             rc = sk_gpu_pdlp_solve(m->nrow, m->ncol, A.nnz, A.p, A.j, A.v, m->Q ? m->Q->p : NULL,
                                    m->Q ? m->Q->i : NULL, m->Q ? m->Q->x : NULL,
                                    m->Q ? m->Q->p[m->ncol] : 0, m->c, m->rlow, m->rupp, m->clow,
                                    m->cupp, wx, wy, &P, x, y, &st);
-            if (rc == -1) { free(x); free(y); hcsr_free(&A); return SK_ERR_NUMERIC; }
+            if (rc == -1) { hcsr_free(&A); sk_gpu_solution_free(s); return SK_ERR_NUMERIC; }
             iters += st.iterations;
-            if (round == 0) g->stats = st;
-            else {
-                g->stats.iterations += st.iterations;
-                g->stats.fp32_iterations += st.fp32_iterations;
-                g->stats.fp64_iterations += st.fp64_iterations;
-                g->stats.restarts += st.restarts;
-                g->stats.host_syncs += st.host_syncs;
-                g->stats.kernel_launches += st.kernel_launches;
-                g->stats.loop_seconds += st.loop_seconds;
-                g->stats.setup_seconds += st.setup_seconds;
-                g->stats.converged = st.converged;
-            }
+            stats_accumulate(&acc, &st, round);
+            verified = 0;
             if (rc == -2) break;
             if (relkkt_hcsr(&A, m->Q ? &Qh : NULL, m->c, m->rlow, m->rupp, m->clow, m->cupp, x, y, kkt)) {
-                free(x); free(y); hcsr_free(&A); return SK_ERR_NUMERIC;
+                hcsr_free(&A); sk_gpu_solution_free(s); return SK_ERR_NUMERIC;
             }
-            if (!st.converged) break;
-            if (kkt[0] <= eps && kkt[1] <= eps && kkt[2] <= eps) break;
+            if (!st.converged || st.cancelled) break;
+            if (sk_gpu_verify(m, s) != SK_OK) break;
+            verified = 1;
+            cert = certificate_ok(m, o, s);
+            if (P.verbosity > 0 || o->verbosity > 0)
+                fprintf(stderr, "sk_gpu_solve round %d: eps %.1e, %d iterations, relative KKT %.3e %.3e %.3e, "
+                        "verify pinf %.3e dinf %.3e comp %.3e -> %s\n", round, P.eps_optimal, st.iterations,
+                        kkt[0], kkt[1], kkt[2], s->primal_infeasibility, s->dual_infeasibility,
+                        s->complementarity, cert ? "certified" : "not certified");
+            if (kkt[0] <= eps && kkt[1] <= eps && kkt[2] <= eps && cert) break;
             P.eps_optimal *= 0.1;
             P.max_iterations -= st.iterations;
             if (P.max_iterations <= 0) break;
@@ -638,19 +748,20 @@ sk_status sk_gpu_solve(const sk_model *m, const sk_options *o, sk_gpu_options *g
         }
     }
     hcsr_free(&A);
-    sk_gpu_solution_init(s);
-    s->x = x; s->y = y;
-    s->ncol = m->ncol; s->nrow = m->nrow;
-    s->rowact = (double *)calloc((size_t)m->nrow + 1, sizeof(double));
-    s->rc = (double *)calloc((size_t)m->ncol + 1, sizeof(double));
     s->iterations = iters;
-    if (sk_gpu_verify(m, s) != SK_OK) { s->result = SK_RESULT_NUMERIC_FAILURE; return SK_OK; }
+    if (!verified && sk_gpu_verify(m, s) != SK_OK) {
+        s->result = SK_RESULT_NUMERIC_FAILURE;
+        if (stats) *stats = acc;
+        return SK_OK;
+    }
+    cert = certificate_ok(m, o, s);
     s->solve_seconds = skg_now() - t0;
     if (!isfinite(s->objective) || !isfinite(s->primal_infeasibility)) s->result = SK_RESULT_NUMERIC_FAILURE;
-    else if (kkt[0] <= eps && kkt[1] <= eps && kkt[2] <= eps) s->result = SK_RESULT_OPTIMAL;
+    else if (kkt[0] <= eps && kkt[1] <= eps && kkt[2] <= eps && cert) s->result = SK_RESULT_OPTIMAL;
     else if (o->time_limit > 0.0 && s->solve_seconds >= o->time_limit) s->result = SK_RESULT_TIME_LIMIT;
     else s->result = SK_RESULT_ITERATION_LIMIT;
     s->dual_bound = s->objective;
+    if (stats) *stats = acc;
     return SK_OK;
 }
 

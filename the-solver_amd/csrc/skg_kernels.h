@@ -107,6 +107,9 @@ typedef struct skk_csr {
 
 /* y = A x in FP64 (plain product, public ABI). */
 int skk_spmv_f64(const skk_csr *A, const double *x, double *y);
+/* y = A x with the iteration's FP32 values, FP32 gathers and FP32 sums
+ * (the mixed engine's product kernel); A->vf must be present. */
+int skk_spmv_f32(const skk_csr *A, const float *x, float *y);
 int skk_axpy_f64(int n, double alpha, const double *x, double *y);
 
 /* --------------------------------------------------- PDHG device state */
@@ -156,6 +159,17 @@ typedef struct skk_pdhg {
     int elem_grid;                           /* fixed grid for reductions   */
     int probe_grid;                          /* workgroups of the step probe
                                                 (A, or Q and A in one launch) */
+    /* Anchored refinement of the FP32 set (MIXED): its iterates hold the
+     * correction (dx, dy) to an FP64 anchor (hx, hy), x = hx + dx,
+     * y = hy + dy.  The FP32 data then describe the correction problem:
+     * cf = c + A^T hy + Q hx, lf/uf = l/u - hx, rlf/ruf = rl/ru - A hx,
+     * ycf = hy (the dual prox centre), all formed in FP64 and rounded
+     * once.  In exact arithmetic the iterates are those of the unshifted
+     * problem; in FP32 the rounding error scales with the correction. */
+    int anchored;
+    double *hx, *hy;                         /* anchor, FP64                */
+    double *hax, *hg, *hqx;                  /* A hx, c + A^T hy + Q hx, Q hx */
+    float *ycf;                              /* FP32 copy of hy             */
 } skk_pdhg;
 
 enum { SKK_F64 = 0, SKK_F32 = 1 };
@@ -185,8 +199,15 @@ int skk_pdhg_flush(const skk_pdhg *P, int prec);
 int skk_pdhg_eval(const skk_pdhg *P, int prec, int which, int adaptive, double fixed_terms);
 int skk_pdhg_restart(const skk_pdhg *P, int prec, int use_average, int adaptive);
 int skk_pdhg_set_omega(const skk_pdhg *P, double omega);
-/* Copy the FP32 iterate into the FP64 set and continue from it. */
+/* Copy the FP32 iterate into the FP64 set and continue from it (with an
+ * anchor, the FP64 set receives anchor + correction). */
 int skk_pdhg_promote(const skk_pdhg *P);
+/* Anchored refinement: the anchor absorbs the FP32 set's current iterate
+ * (a restart has just put the restart point there), the FP32 corrections
+ * return to zero, and the correction problem's FP32 data are formed again
+ * from FP64 products at the new anchor.  absorb = 0 keeps the anchor
+ * (set-up from hx, hy as uploaded). */
+int skk_pdhg_reanchor(const skk_pdhg *P, int absorb);
 /* Store the final iterate (or the average) as FP64 into x_out/y_out. */
 int skk_pdhg_export(const skk_pdhg *P, int prec, int use_average, double *x_out, double *y_out);
 /* ||A||_2 by power iteration on A'A in FP64; v, v2 length n, w length m.

@@ -45,9 +45,12 @@ hipStream_t stream()
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = 0;
     if (!g_stream[dev]) {
-        // default priority only (no priority streams on these boxes)
-        if (hipStreamCreateWithFlags(&g_stream[dev], hipStreamNonBlocking) != hipSuccess)
-            g_stream[dev] = nullptr;
+        // A blocking stream (default flags, default priority): it orders with
+        // the null stream both ways, so device-pointer operations keep the
+        // reference's default-stream contract (cuda/include/sankhya_cuda.h:
+        // 35-37) -- a caller's earlier null-stream copy into the input is
+        // seen, and a later hipMemcpy of the output waits for the product.
+        if (hipStreamCreate(&g_stream[dev]) != hipSuccess) g_stream[dev] = nullptr;
     }
     return g_stream[dev];
 }
@@ -425,13 +428,41 @@ template <typename T> struct Iter {
     __device__ __forceinline__ T *ax(int k) const { return k == 0 ? AX[0] : (k == 1 ? AX[1] : AX[2]); }
 };
 
+// yc: the dual prox centre of an anchored FP32 set (null otherwise).
 template <typename T> struct Data {
-    const T *c, *l, *u, *rl, *ru;
+    const T *c, *l, *u, *rl, *ru, *yc;
 };
 
 template <typename T> Data<T> data_of(const skk_pdhg &P);
-template <> Data<float> data_of<float>(const skk_pdhg &P) { return {P.cf, P.lf, P.uf, P.rlf, P.ruf}; }
-template <> Data<double> data_of<double>(const skk_pdhg &P) { return {P.cd, P.ld, P.ud, P.rld, P.rud}; }
+template <> Data<float> data_of<float>(const skk_pdhg &P)
+{
+    return {P.cf, P.lf, P.uf, P.rlf, P.ruf, P.anchored ? P.ycf : nullptr};
+}
+template <> Data<double> data_of<double>(const skk_pdhg &P) { return {P.cd, P.ld, P.ud, P.rld, P.rud, nullptr}; }
+
+// The dual prox of a PDHG step, y+ = v - sigma proj_[rl,ru](v / sigma)
+// (PdlpKernels.cu:377-389).  With a centre yc (anchored set: y = yc + dy,
+// v and the bounds already shifted by the anchor) the same map is written
+// for the correction, dy+ = clamp(-yc, v - sigma ru, v - sigma rl): the
+// three cases of the prox (below, between, above the bounds) give
+// v - sigma rl, -yc and v - sigma ru, so no term of the anchor's size is
+// added and then cancelled in FP32.  Infinite bounds never bind.
+template <typename T> __device__ __forceinline__ T dual_prox(T v, T sigma, const Data<T> &D, int i)
+{
+    if (D.yc) return fmin(fmax(-D.yc[i], v - sigma * D.ru[i]), v - sigma * D.rl[i]);
+    return v - sigma * clampv(v / sigma, D.rl[i], D.ru[i]);
+}
+
+// Anchor of an FP32 set in evaluations and exports (null x: none).
+struct Anchor {
+    const double *x, *y, *ax, *g, *qx;
+};
+
+Anchor anchor_of(const skk_pdhg &P, bool f32)
+{
+    if (!f32 || !P.anchored) return Anchor{nullptr, nullptr, nullptr, nullptr, nullptr};
+    return Anchor{P.hx, P.hy, P.hax, P.hg, P.has_q ? P.hqx : nullptr};
+}
 
 template <typename T> const T *vals_of(const skk_csr &A);
 template <> const float *vals_of<float>(const skk_csr &A) { return A.vf; }
@@ -513,7 +544,7 @@ __global__ __launch_bounds__(kBlock) void k_dual_adaptive(Iter<T> I, Data<T> D,
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
         const T yi = y[i];
         const T v = yi + sigma * (T(2) * ax[i] - axp[i]);
-        yc[i] = v - sigma * clampv(v / sigma, D.rl[i], D.ru[i]);
+        yc[i] = dual_prox(v, sigma, D, i);
         if (pending) I.ysum.add(i, yi);
     }
 }
@@ -700,7 +731,7 @@ __global__ __launch_bounds__(kBlock) void k_dual_decide(Iter<T> I, Data<T> D, co
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride_e) {
         const T yi = y[i];
         const T v = yi + sigma * (T(2) * ax[i] - axp[i]);
-        yc[i] = v - sigma * clampv(v / sigma, D.rl[i], D.ru[i]);
+        yc[i] = dual_prox(v, sigma, D, i);
         if (pending) I.ysum.add(i, yi);
     }
 }
@@ -750,7 +781,7 @@ template <typename T, typename AT> struct EpiDualFixed {
     __device__ void row(int i, AT s, double *) const
     {
         const T v = y[i] + sigma * static_cast<T>(s);
-        const T yi = v - sigma * clampv(v / sigma, D.rl[i], D.ru[i]);
+        const T yi = dual_prox(v, sigma, D, i);
         y[i] = yi;
         I.ysum.add(i, yi);
     }
@@ -812,7 +843,10 @@ __global__ void k_average(Iter<T> I, const skk_state *__restrict__ st, int n, in
 // The eight sums of PdlpKernels.cu:101-209 in FP64: ||Ax - proj Ax||^2,
 // dual violation^2, c'x, dual objective, ||x||^2, ||y||^2, ||x - x_r||^2,
 // ||y - y_r||^2.  Column terms use the FP64 data and FP64 products.
-// `which` 0 evaluates X[cur], Y[cur]; 1 the running average.
+// `which` 0 evaluates X[cur], Y[cur]; 1 the running average.  With an
+// anchor H the set holds corrections and the products are those of the
+// corrections: the point evaluated is x = H.x + dx, y = H.y + dy, with
+// A x = H.ax + A dx, c + A^T y + Q x = H.g + A^T dy + Q dx, Q x = H.qx + Q dx.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_residuals_sel(Iter<T> I, int which,
                                                           const double *__restrict__ ax,
@@ -820,7 +854,7 @@ __global__ __launch_bounds__(kBlock) void k_residuals_sel(Iter<T> I, int which,
                                                           Data<double> D, int n, int m,
                                                           double *part, int stride,
                                                           const skk_state *__restrict__ st,
-                                                          const double *__restrict__ qx)
+                                                          const double *__restrict__ qx, Anchor H)
 {
     const int cur = st->cur;
     const T *__restrict__ x = which ? I.xavg : I.x(cur);
@@ -830,9 +864,12 @@ __global__ __launch_bounds__(kBlock) void k_residuals_sel(Iter<T> I, int which,
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const int gs = gridDim.x * blockDim.x;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gs) {
-        const double q = qx ? qx[j] : 0.0;
-        const double xj = x[j], lo = D.l[j], hi = D.u[j], r = D.c[j] + aty[j] + q;
-        const double dx = xj - static_cast<double>(xr[j]);
+        const double qd = qx ? qx[j] : 0.0;
+        const double q = H.qx ? H.qx[j] + qd : qd;
+        const double xd = x[j];
+        const double xj = H.x ? H.x[j] + xd : xd, lo = D.l[j], hi = D.u[j];
+        const double r = (H.x ? H.g[j] : D.c[j]) + aty[j] + qd;
+        const double dx = xd - static_cast<double>(xr[j]);
         // with Q: primal c'x + x'Qx/2, dual terms of r = c + Qx + A'y minus x'Qx/2
         acc[2] += D.c[j] * xj + 0.5 * xj * q;
         acc[3] -= 0.5 * xj * q;
@@ -848,9 +885,10 @@ __global__ __launch_bounds__(kBlock) void k_residuals_sel(Iter<T> I, int which,
         acc[3] += term;
     }
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gs) {
-        const double a = ax[i], rl = D.rl[i], ru = D.ru[i];
+        const double a = H.x ? H.ax[i] + ax[i] : ax[i], rl = D.rl[i], ru = D.ru[i];
         const double pr = a - fmin(fmax(a, rl), ru);
-        const double yi = y[i], dy = yi - static_cast<double>(yr[i]);
+        const double yd = y[i];
+        const double yi = H.x ? H.y[i] + yd : yd, dy = yd - static_cast<double>(yr[i]);
         acc[0] += pr * pr;
         acc[5] += yi * yi;
         acc[7] += dy * dy;
@@ -906,31 +944,94 @@ __global__ void k_restart(Iter<T> I, skk_state *st, const double *__restrict__ a
 }
 
 // FP32 -> FP64 copy of every iterate buffer (the state indices are shared).
-__global__ void k_promote(Iter<double> D, Iter<float> F, int n, int m)
+// An anchored FP32 set holds corrections: the anchor is added to every
+// point and product (the running sums are empty, a restart precedes this).
+__global__ void k_promote(Iter<double> D, Iter<float> F, int n, int m, Anchor H)
 {
     const int stride = gridDim.x * blockDim.x;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
-        D.X[0][j] = F.X[0][j]; D.X[1][j] = F.X[1][j];
-        static_cast<double *>(D.xsum.p)[j] = F.xsum.get(j); D.xr[j] = F.xr[j]; D.xbar[j] = F.xbar[j]; D.xavg[j] = F.xavg[j];
-        if (D.QX[0]) { D.QX[0][j] = F.QX[0][j]; D.QX[1][j] = F.QX[1][j]; }
+        const double h = H.x ? H.x[j] : 0.0, hq = H.qx ? H.qx[j] : 0.0;
+        D.X[0][j] = h + F.X[0][j]; D.X[1][j] = h + F.X[1][j];
+        static_cast<double *>(D.xsum.p)[j] = F.xsum.get(j);
+        D.xr[j] = h + F.xr[j]; D.xbar[j] = h + F.xbar[j]; D.xavg[j] = h + F.xavg[j];
+        if (D.QX[0]) { D.QX[0][j] = hq + F.QX[0][j]; D.QX[1][j] = hq + F.QX[1][j]; }
     }
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
-        D.Y[0][i] = F.Y[0][i]; D.Y[1][i] = F.Y[1][i];
-        D.AX[0][i] = F.AX[0][i]; D.AX[1][i] = F.AX[1][i]; D.AX[2][i] = F.AX[2][i];
-        static_cast<double *>(D.ysum.p)[i] = F.ysum.get(i); D.yr[i] = F.yr[i]; D.yavg[i] = F.yavg[i];
+        const double h = H.x ? H.y[i] : 0.0, ha = H.x ? H.ax[i] : 0.0;
+        D.Y[0][i] = h + F.Y[0][i]; D.Y[1][i] = h + F.Y[1][i];
+        D.AX[0][i] = ha + F.AX[0][i]; D.AX[1][i] = ha + F.AX[1][i]; D.AX[2][i] = ha + F.AX[2][i];
+        static_cast<double *>(D.ysum.p)[i] = F.ysum.get(i); D.yr[i] = h + F.yr[i]; D.yavg[i] = h + F.yavg[i];
     }
 }
 
-template <typename T>
-__global__ void k_export(Iter<T> I, const skk_state *__restrict__ st, int n, int m, int use_avg,
-                         double *xo, double *yo)
+// Anchored refinement, step 1: the anchor absorbs the current correction
+// (a restart has just made it the restart point) and every correction
+// buffer returns to zero.  The anchor's x is kept inside [l, u] (the
+// exact iterate is; FP32 rounding of the shifted bounds may leave the sum
+// an ulp outside).
+__global__ void k_anchor_absorb(Iter<float> F, const skk_state *__restrict__ st, double *hx, double *hy,
+                                const double *__restrict__ l, const double *__restrict__ u, int n, int m,
+                                int absorb)
 {
     const int cur = st->cur;
     const int stride = gridDim.x * blockDim.x;
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride)
-        xo[j] = static_cast<double>(use_avg ? I.xavg[j] : I.x(cur)[j]);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride)
-        yo[i] = static_cast<double>(use_avg ? I.yavg[i] : I.y(cur)[i]);
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        if (absorb) hx[j] = fmin(fmax(hx[j] + static_cast<double>(F.x(cur)[j]), l[j]), u[j]);
+        F.X[0][j] = 0.0f; F.X[1][j] = 0.0f; F.xr[j] = 0.0f; F.xbar[j] = 0.0f; F.xavg[j] = 0.0f;
+        F.xsum.zero(j);
+        if (F.QX[0]) { F.QX[0][j] = 0.0f; F.QX[1][j] = 0.0f; }
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        if (absorb) hy[i] += static_cast<double>(F.y(cur)[i]);
+        F.Y[0][i] = 0.0f; F.Y[1][i] = 0.0f; F.yr[i] = 0.0f; F.yavg[i] = 0.0f;
+        F.ysum.zero(i);
+        F.AX[0][i] = 0.0f; F.AX[1][i] = 0.0f; F.AX[2][i] = 0.0f;
+    }
+}
+
+// Anchored refinement, step 2: the correction problem's FP32 data from the
+// FP64 products at the anchor (hax = A hx, hg = A^T hy on entry, hqx = Q hx).
+// The column bounds are rounded inward, so a correction at its bound keeps
+// hx + dx inside [l, u].
+__global__ void k_anchor_data(const double *__restrict__ c, const double *__restrict__ l,
+                              const double *__restrict__ u, const double *__restrict__ rl,
+                              const double *__restrict__ ru, const double *__restrict__ hx,
+                              const double *__restrict__ hy, const double *__restrict__ hax, double *hg,
+                              const double *__restrict__ hqx, float *cf, float *lf, float *uf, float *rlf,
+                              float *ruf, float *ycf, int n, int m)
+{
+    const int stride = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        const double g = c[j] + hg[j] + (hqx ? hqx[j] : 0.0);
+        hg[j] = g;
+        cf[j] = static_cast<float>(g);
+        lf[j] = __double2float_ru(l[j] - hx[j]);
+        uf[j] = __double2float_rd(u[j] - hx[j]);
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        rlf[i] = static_cast<float>(rl[i] - hax[i]);
+        ruf[i] = static_cast<float>(ru[i] - hax[i]);
+        ycf[i] = static_cast<float>(hy[i]);
+    }
+}
+
+// With an anchor the exported point is anchor + correction, x kept inside
+// [l, u] (l, u: the FP64 column bounds).
+template <typename T>
+__global__ void k_export(Iter<T> I, const skk_state *__restrict__ st, int n, int m, int use_avg,
+                         double *xo, double *yo, Anchor H, const double *__restrict__ l,
+                         const double *__restrict__ u)
+{
+    const int cur = st->cur;
+    const int stride = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        const double v = static_cast<double>(use_avg ? I.xavg[j] : I.x(cur)[j]);
+        xo[j] = H.x ? fmin(fmax(H.x[j] + v, l[j]), u[j]) : v;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        const double v = static_cast<double>(use_avg ? I.yavg[i] : I.y(cur)[i]);
+        yo[i] = H.x ? H.y[i] + v : v;
+    }
 }
 
 // ------------------------------------------------------- power iteration
@@ -1390,7 +1491,7 @@ template <typename T> int eval(const skk_pdhg &P, int which, int adaptive, doubl
     double *rp = P.part + (size_t)R_EVAL * P.part_capacity;
     hipLaunchKernelGGL(k_residuals_sel<T>, dim3(P.elem_grid), dim3(kBlock), 0, stream(), I, which,
                        P.ax_d, P.aty_d, D, P.n, P.m, rp, P.part_capacity, P.st,
-                       P.has_q ? P.qx_d : nullptr);
+                       P.has_q ? P.qx_d : nullptr, anchor_of(P, prec_index<T>() == SKK_F32));
     if ((rc = launch_status())) return rc;
     if ((rc = finalize(rp, P.part_capacity, P.elem_grid, 8, 0, P.eval_out + 8 * which))) return rc;
     if (!which && adaptive) {
@@ -1575,6 +1676,13 @@ int skk_spmv_f64(const skk_csr *A, const double *x, double *y)
     return launch_spmv<double, double, double>(*A, A->vd, x, x, nullptr, e);
 }
 
+int skk_spmv_f32(const skk_csr *A, const float *x, float *y)
+{
+    if (!A->vf) return -1;
+    EpiStore<float, float> e{y};
+    return launch_spmv<float, float, float>(*A, A->vf, x, x, nullptr, e);
+}
+
 int skk_axpy_f64(int n, double alpha, const double *x, double *y)
 {
     if (n <= 0) return 0;
@@ -1719,7 +1827,37 @@ int skk_pdhg_promote(const skk_pdhg *P)
 {
     Iter<double> D(P->it[SKK_F64]);
     Iter<float> F(P->it[SKK_F32]);
-    hipLaunchKernelGGL(k_promote, dim3(P->elem_grid), dim3(kBlock), 0, stream(), D, F, P->n, P->m);
+    hipLaunchKernelGGL(k_promote, dim3(P->elem_grid), dim3(kBlock), 0, stream(), D, F, P->n, P->m,
+                       anchor_of(*P, true));
+    return launch_status();
+}
+
+int skk_pdhg_reanchor(const skk_pdhg *P, int absorb)
+{
+    if (!P->anchored) return -1;
+    int rc;
+    Iter<float> F(P->it[SKK_F32]);
+    hipLaunchKernelGGL(k_anchor_absorb, dim3(P->elem_grid), dim3(kBlock), 0, stream(), F, P->st, P->hx, P->hy,
+                       P->ld, P->ud, P->n, P->m, absorb);
+    if ((rc = launch_status())) return rc;
+    // FP64 products at the anchor; an empty operator leaves its product 0
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(P->m, P->elem_grid)), dim3(kBlock), 0, stream(), P->hax, 0.0, P->m);
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(P->n, P->elem_grid)), dim3(kBlock), 0, stream(), P->hg, 0.0, P->n);
+    if ((rc = launch_status())) return rc;
+    EpiStore<double, double> ea{P->hax};
+    if ((rc = launch_spmv<double, double, double>(P->A, P->A.vd, P->hx, P->hx, nullptr, ea))) return rc;
+    EpiStore<double, double> eg{P->hg};
+    if ((rc = launch_spmv<double, double, double>(P->AT, P->AT.vd, P->hy, P->hy, nullptr, eg))) return rc;
+    if (P->has_q) {
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(P->n, P->elem_grid)), dim3(kBlock), 0, stream(), P->hqx, 0.0, P->n);
+        if ((rc = launch_status())) return rc;
+        EpiStore<double, double> eq{P->hqx};
+        if ((rc = launch_spmv<double, double, double>(P->Q, P->Q.vd, P->hx, P->hx, nullptr, eq))) return rc;
+    }
+    hipLaunchKernelGGL(k_anchor_data, dim3(P->elem_grid), dim3(kBlock), 0, stream(), P->cd, P->ld, P->ud, P->rld,
+                       P->rud, P->hx, P->hy, P->hax, P->hg, P->has_q ? P->hqx : nullptr,
+                       const_cast<float *>(P->cf), const_cast<float *>(P->lf), const_cast<float *>(P->uf),
+                       const_cast<float *>(P->rlf), const_cast<float *>(P->ruf), P->ycf, P->n, P->m);
     return launch_status();
 }
 
@@ -1728,11 +1866,11 @@ int skk_pdhg_export(const skk_pdhg *P, int prec, int use_average, double *x_out,
     if (prec == SKK_F32) {
         Iter<float> I(P->it[SKK_F32]);
         hipLaunchKernelGGL(k_export<float>, dim3(P->elem_grid), dim3(kBlock), 0, stream(), I, P->st,
-                           P->n, P->m, use_average, x_out, y_out);
+                           P->n, P->m, use_average, x_out, y_out, anchor_of(*P, true), P->ld, P->ud);
     } else {
         Iter<double> I(P->it[SKK_F64]);
         hipLaunchKernelGGL(k_export<double>, dim3(P->elem_grid), dim3(kBlock), 0, stream(), I, P->st,
-                           P->n, P->m, use_average, x_out, y_out);
+                           P->n, P->m, use_average, x_out, y_out, anchor_of(*P, false), P->ld, P->ud);
     }
     return launch_status();
 }

@@ -41,6 +41,9 @@ void sk_gpu_pdlp_params_default(sk_gpu_pdlp_params *p)
     p->ruiz_tol = 1e-2;
     p->use_graph = 0;
     p->verbosity = 0;
+    p->refine = 1;
+    p->cancel = NULL;
+    p->restart_rule = SK_GPU_RESTART_KKT;
 }
 
 static double to_ieee(double b)
@@ -137,7 +140,7 @@ typedef struct engine {
     int *perm_r, *perm_c;        /* slot -> original index */
     double *R, *C;               /* scale factors, original indexing */
     double cost_norm, bound_norm;
-    void *dev[64];               /* every device allocation, freed together */
+    void *dev[96];               /* every device allocation, freed together */
     int ndev;
     skk_pdhg P;
     double *pv, *pw, *pv2, *pnorm;
@@ -152,7 +155,8 @@ static int dalloc(engine *E, void **p, size_t bytes)
 {
     int rc = skk_malloc(p, bytes);
     if (rc) return skg_hip_fail("hipMalloc", rc);
-    if (E->ndev < 64) E->dev[E->ndev++] = *p;
+    if (E->ndev < 96) E->dev[E->ndev++] = *p;
+    else { skk_free(*p); *p = NULL; return skg_fail("too many device allocations"); }
     return 0;
 }
 
@@ -371,6 +375,14 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
            the config-4 family) FP32 sums were 4-10% faster, and they keep
            config 1's iterations lighter. */
         if (E->mixed && (rc = alloc_iter(E, &Q->it[SKK_F32], sizeof(float), E->has_q))) goto done;
+        if (E->mixed && P->refine) {
+            Q->anchored = 1;
+            if ((rc = dalloc(E, (void **)&Q->hx, 8 * nn)) || (rc = dalloc(E, (void **)&Q->hy, 8 * mm)) ||
+                (rc = dalloc(E, (void **)&Q->hax, 8 * mm)) || (rc = dalloc(E, (void **)&Q->hg, 8 * nn)) ||
+                (rc = dalloc(E, (void **)&Q->ycf, 4 * mm)) ||
+                (E->has_q && (rc = dalloc(E, (void **)&Q->hqx, 8 * nn))))
+                goto done;
+        }
         eg = ((m > n ? m : n) + SKK_BLOCK - 1) / SKK_BLOCK;
         if (eg < 1) eg = 1;
         if (eg > 2048) eg = 2048;
@@ -587,6 +599,19 @@ int sk_gpu_pdlp_session_update(sk_gpu_pdlp_session *S, const double *c, const do
     return engine_set_data(&S->E, c, row_lower, row_upper, col_lower, col_upper) ? -1 : 0;
 }
 
+int sk_gpu_pdlp_session_set_limits(sk_gpu_pdlp_session *S, int max_iterations, double time_limit_seconds,
+                                   double eps_optimal)
+{
+    skg_clear_error();
+    if (!S || !S->ready) return skg_fail("invalid session");
+    if (max_iterations <= 0 || time_limit_seconds < 0.0 || !(eps_optimal > 0.0))
+        return skg_fail("invalid limits");
+    S->P.max_iterations = max_iterations;
+    S->P.time_limit_seconds = time_limit_seconds;
+    S->P.eps_optimal = eps_optimal;
+    return 0;
+}
+
 void sk_gpu_pdlp_session_destroy(sk_gpu_pdlp_session *S)
 {
     if (!S) return;
@@ -606,6 +631,28 @@ static int session_start(sk_gpu_pdlp_session *S, int prec, const double *x0, con
         omega = fmin(1e6, fmax(1e-6, S->P.primal_weight));
     else if (E->cost_norm > 0.0 && E->bound_norm > 0.0)
         omega = fmin(1e6, fmax(1e-6, E->cost_norm / E->bound_norm));
+    if (prec == SKK_F32 && E->P.anchored) {
+        /* anchored FP32 set: the start point (warm or 0) is the first anchor,
+           in FP64; the corrections start at 0 */
+        double *tx = (double *)calloc((size_t)cols + 1, sizeof(double));
+        double *ty = (double *)calloc((size_t)rows + 1, sizeof(double));
+        if (!tx || !ty) { free(tx); free(ty); return skg_fail("out of host memory"); }
+        for (k = 0; k < cols && x0; ++k) {
+            const int col = E->perm_c[k];
+            tx[k] = x0[col] / E->C[col];
+        }
+        for (k = 0; k < rows && y0; ++k) ty[k] = y0[E->perm_r[k]] / E->R[E->perm_r[k]];
+        rc = skk_h2d(E->P.hx, tx, sizeof(double) * (size_t)cols);
+        if (!rc) rc = skk_h2d(E->P.hy, ty, sizeof(double) * (size_t)rows);
+        free(tx); free(ty);
+        if (rc) return skg_hip_fail("warm start upload", rc);
+        if ((rc = skk_pdhg_init(&E->P, prec, S->eta_global, omega, S->P.eta_min_factor / S->norm,
+                                S->P.eta_max_factor / S->norm, 0, 0)) ||
+            (rc = skk_pdhg_reanchor(&E->P, 0)))
+            return skg_hip_fail("PDHG init", rc);
+        *omega_out = omega;
+        return 0;
+    }
     if (x0 || y0) {
         double *tx = (double *)malloc(sizeof(double) * (size_t)(cols + 1));
         double *ty = (double *)malloc(sizeof(double) * (size_t)(rows + 1));
@@ -733,6 +780,15 @@ int sk_gpu_pdlp_session_solve(sk_gpu_pdlp_session *S, const double *x0, const do
                         iteration, prec == SKK_F32 ? "f32" : "f64", kkt_worst(&cur), kkt_worst(&avg),
                         best.primal, best.dual, best.gap, E->h_state->eta, E->h_state->omega);
             if (!isfinite(kkt_worst(&cur)) && !isfinite(kkt_worst(&avg))) { diverged = 1; break; }
+            {
+                static const double marks[4] = {1e-3, 1e-4, 1e-5, 1e-6};
+                const double merit = kkt_worst(&best);
+                for (k = 0; k < 4; ++k)
+                    if (!stats->iterations_to_merit[k] && merit <= marks[k]) {
+                        stats->iterations_to_merit[k] = iteration;
+                        stats->seconds_to_merit[k] = skg_now() - tl0;
+                    }
+            }
             if (best.primal <= P.eps_optimal && best.dual <= P.eps_optimal && best.gap <= P.eps_optimal) {
                 use_avg_final = average_better;
                 stats->converged = 1;
@@ -740,11 +796,20 @@ int sk_gpu_pdlp_session_solve(sk_gpu_pdlp_session *S, const double *x0, const do
             }
             use_avg_final = average_better;
             if (P.time_limit_seconds > 0.0 && skg_now() - t0 > P.time_limit_seconds) break;
+            /* GpuPdlp.cpp:430-434: cooperative cancel at the window sync */
+            if (P.cancel && *P.cancel) { stats->cancelled = 1; break; }
             mu = kkt_worst(&best);
-            /* mixed precision: hand over to FP64 once FP32 is near its floor:
-               at tail_switch, or when within 100x of it no window has improved
-               the best merit by 10% for tail_stall_windows windows */
-            if (prec == SKK_F32 && !tail) {
+            /* mixed precision: hand over to FP64 once FP32 is near its floor.
+               Plain FP32 iterates: at tail_switch, or when within 100x of it
+               no window has improved the best merit by 10% for
+               tail_stall_windows windows.  Anchored iterates have no floor of
+               their own (the rounding shrinks with the correction, and every
+               restart moves the anchor) and follow the FP64 engine's
+               iteration counts (the 500k QP 9,472 vs 9,664, the 20k x 50k
+               LP family 14,912 vs 14,912); where they stall, the FP64 engine
+               stalls at the same merit (the 1M x 2M LP: relative gap 1.27e-3
+               in both), so they never hand over. */
+            if (prec == SKK_F32 && !tail && !E->P.anchored) {
                 if (mu < 0.9 * best_mu_seen) { best_mu_seen = mu; stall = 0; }
                 else ++stall;
                 if (mu <= P.tail_switch || (mu <= 100.0 * P.tail_switch && stall >= P.tail_stall_windows)) {
@@ -773,6 +838,12 @@ int sk_gpu_pdlp_session_solve(sk_gpu_pdlp_session *S, const double *x0, const do
             if ((rc = skk_pdhg_restart(&E->P, prec, average_better, P.adaptive_step)))
                 return skg_hip_fail("restart launch", rc);
             E->launches += 1;
+            if (prec == SKK_F32 && E->P.anchored) {
+                /* the restart point becomes the new anchor */
+                if ((rc = skk_pdhg_reanchor(&E->P, 1))) return skg_hip_fail("re-anchor", rc);
+                E->launches += 5 + (E->has_q ? 2 : 0);
+                stats->reanchors++;
+            }
             use_avg_final = 0;
             windows_since_restart = 0;
             its_since_restart = 0;

@@ -32,6 +32,7 @@ SK_RESULT_NOT_RUN = 6
 SK_LP_AUTO, SK_LP_SIMPLEX, SK_LP_FIRST_ORDER = 0, 1, 2
 SK_INFINITY = 1.0e30
 PREC_FP64, PREC_MIXED = 0, 1
+RESTART_KKT, RESTART_NDG = 0, 1
 
 
 class SkCsc(C.Structure):
@@ -98,7 +99,8 @@ class PdlpParams(C.Structure):
                 ("tail_stall_windows", C.c_int), ("scaling", C.c_int),
                 ("ruiz_iterations", C.c_int), ("ruiz_tol", C.c_double),
                 ("use_graph", C.c_int), ("verbosity", C.c_int), ("primal_weight", C.c_double),
-                ("unfused", C.c_int), ("pad", C.c_int)]
+                ("unfused", C.c_int), ("refine", C.c_int), ("cancel", C.POINTER(C.c_int)),
+                ("restart_rule", C.c_int), ("pad", C.c_int)]
 
 
 class PdlpStats(C.Structure):
@@ -111,15 +113,19 @@ class PdlpStats(C.Structure):
                 ("step_size_ratio", C.c_double), ("host_syncs", C.c_int),
                 ("fp32_iterations", C.c_int), ("fp64_iterations", C.c_int),
                 ("kernel_launches", C.c_longlong), ("loop_seconds", C.c_double),
-                ("setup_seconds", C.c_double), ("primal_weight", C.c_double)]
+                ("setup_seconds", C.c_double), ("primal_weight", C.c_double),
+                ("reanchors", C.c_int), ("iterations_to_merit", C.c_int * 4),
+                ("seconds_to_merit", C.c_double * 4)]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["iterations_to_merit"] = list(self.iterations_to_merit)
+        d["seconds_to_merit"] = list(self.seconds_to_merit)
+        return d
 
 
 class GpuOptions(C.Structure):
-    _fields_ = [("pdlp", PdlpParams), ("x0", c_double_p), ("y0", c_double_p),
-                ("stats", PdlpStats)]
+    _fields_ = [("pdlp", PdlpParams), ("x0", c_double_p), ("y0", c_double_p)]
 
 
 class BatchOptions(C.Structure):
@@ -152,7 +158,8 @@ def default_pdlp_params(**overrides) -> PdlpParams:
                    time_limit_seconds=0.0, power_iterations=30, adaptive_step=1,
                    eta_min_factor=1e-6, eta_max_factor=1e4, precision=PREC_FP64,
                    tail_switch=1e-5, tail_stall_windows=32, scaling=1,
-                   ruiz_iterations=100, ruiz_tol=1e-2, use_graph=0, verbosity=0)
+                   ruiz_iterations=100, ruiz_tol=1e-2, use_graph=0, verbosity=0, refine=1,
+                   restart_rule=RESTART_KKT)
     for k, v in overrides.items():
         setattr(p, k, v)
     return p
@@ -228,6 +235,8 @@ def load_gpu():
         _declare(lib, "sk_gpu_spmv_f64", C.c_int, [C.c_int, C.c_int, C.c_int, c_int_p, c_int_p,
                                                    c_double_p, c_double_p, c_double_p])
         _declare(lib, "sk_gpu_axpy_f64", C.c_int, [C.c_int, C.c_double, c_double_p, c_double_p])
+        _declare(lib, "sk_gpu_spmv_f32", C.c_int, [C.c_int, C.c_int, C.c_int, c_int_p, c_int_p, c_double_p,
+                                                   C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)])
         lp_args = [H, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p, LpSettings,
                    c_double_p, C.POINTER(LpResult)]
         _declare(lib, "sk_gpu_lp_pdhg", C.c_int, lp_args)
@@ -251,7 +260,8 @@ def load_gpu():
                   c_double_p, c_double_p, C.POINTER(PdlpParams),
                   c_double_p, c_double_p, C.POINTER(PdlpStats)])
         _declare(lib, "sk_gpu_solver_options_default", None, [C.POINTER(GpuOptions)])
-        _declare(lib, "sk_gpu_solve", C.c_int, [M, C.POINTER(SkOptions), C.POINTER(GpuOptions), S])
+        _declare(lib, "sk_gpu_solve", C.c_int, [M, C.POINTER(SkOptions), C.POINTER(GpuOptions), S,
+                                                C.POINTER(PdlpStats)])
         _declare(lib, "sk_gpu_verify", C.c_int, [M, S])
         _declare(lib, "sk_gpu_relative_kkt", C.c_int,
                  [C.c_int, C.c_int, C.c_int, c_int_p, c_int_p, c_double_p, c_double_p,
@@ -267,6 +277,7 @@ def load_gpu():
         _declare(lib, "sk_gpu_pdlp_session_solve", C.c_int,
                  [vp, c_double_p, c_double_p, c_double_p, c_double_p, C.POINTER(PdlpStats)])
         _declare(lib, "sk_gpu_pdlp_session_destroy", None, [vp])
+        _declare(lib, "sk_gpu_pdlp_session_set_limits", C.c_int, [vp, C.c_int, C.c_double, C.c_double])
         _declare(lib, "sk_gpu_gather_rate", C.c_int, [C.c_int, C.c_int, C.c_int, c_double_p])
         _declare(lib, "sk_gpu_read_mps", C.c_int, [C.c_char_p, C.POINTER(SkModel)])
         _declare(lib, "sk_gpu_model_free", None, [C.POINTER(SkModel)])
@@ -454,6 +465,7 @@ def _hip_runtime():
         _hip.hipFree.argtypes = [C.c_void_p]
         _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         _hip.hipDeviceSynchronize.argtypes = []
+        _hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
     return _hip
 
 
