@@ -427,8 +427,18 @@ int  sk_gpu_pdlp_session_solve(sk_gpu_pdlp_session *session, const double *x0,
                                sk_gpu_pdlp_stats *stats);
 int  sk_gpu_pdlp_session_iterate(sk_gpu_pdlp_session *session, int iterations,
                                  double *kernel_ms);
-/* The stopping limits of later solves (GpuPdlp.hpp:19-37 max_iterations,
- * time_limit_seconds, eps_optimal); the resident operator is kept. */
+/* Continue the session's last solve from exactly where it stopped (same
+ * iterate, step size, primal weight, running averages and restart state)
+ * under the current limits: at most max_iterations more iterations, the
+ * time limit counted from this call.  The typical use is a tighter
+ * eps_optimal after a converged solve (set_limits, then resume).  Stats
+ * are cumulative over the solve and its resumes.  No reference counterpart
+ * (GpuPdlp::solve always starts over); -1 when no solve ran. */
+int  sk_gpu_pdlp_session_resume(sk_gpu_pdlp_session *session, double *x, double *y,
+                                sk_gpu_pdlp_stats *stats);
+/* The stopping limits of later solves and resumes (GpuPdlp.hpp:19-37
+ * max_iterations, time_limit_seconds, eps_optimal); the resident operator
+ * and the loop state are kept. */
 int  sk_gpu_pdlp_session_set_limits(sk_gpu_pdlp_session *session, int max_iterations,
                                     double time_limit_seconds, double eps_optimal);
 void sk_gpu_pdlp_session_destroy(sk_gpu_pdlp_session *session);

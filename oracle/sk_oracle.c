@@ -25,6 +25,7 @@
 #include "sk_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -994,6 +995,11 @@ int orc_pdlp_solve(const orc_csr *A, const double *c, const double *l, const dou
         necessary = mu <= 0.8 * mu_at_restart && mu > mu_previous;
         artificial = (double)its_since_restart >= 0.36 * (double)iteration;
         mu_previous = mu;
+        if (P->verbosity > 1)   /* window trace for diagnosis */
+            fprintf(stderr, "win %d cur %.4e avg %.4e p %.4e g %.4e pobj %.10e dobj %.10e omega %.4e eta %.4e "
+                    "pmove %.4e dmove %.4e restart %d%d%d avg %d\n", iteration, kkt_worst(&cur), kkt_worst(&avg),
+                    best.primal, best.gap, best.pobj, best.dobj, omega, eta, best.pmove, best.dmove, sufficient,
+                    necessary, artificial, average_better);
         if (!(sufficient || necessary || artificial)) continue;
         if (average_better) {
             memcpy(x, xa, (size_t)n * sizeof(double));

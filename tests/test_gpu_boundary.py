@@ -117,3 +117,46 @@ def test_cancel_flag_stops_at_the_next_window(prec):
     flag.value = 0
     rc, x, y, st = gpu_pdlp(model, precision=prec, eps_optimal=1e-7, cancel=C.pointer(flag))
     assert rc == 0 and st.cancelled == 0 and st.converged == 1
+
+
+@pytest.mark.parametrize("prec", [sg.PREC_FP64, sg.PREC_MIXED])
+def test_session_resume_continues_the_same_solve(prec):
+    """A solve to 1e-4 resumed with a 1e-7 tolerance reaches 1e-7 (the
+    oracle's FP64 relative KKT of the point) in about the iterations of a
+    cold solve to 1e-7: it continues the same iterate, step size and primal
+    weight instead of starting over."""
+    lib = sg.load_gpu()
+    model = sg.Model.generated("skw_bounded_lp", 400, 800, 5, 11)
+    m, n = model.nrow, model.ncol
+    p, i, v = model.csc()
+    rp, cj, vv = sg.csc_to_csr(m, n, p, i, v)
+    vec = [model.vec(k) for k in ("c", "rlow", "rupp", "clow", "cupp")]
+
+    def session(**kw):
+        P = sg.default_pdlp_params(precision=prec, **kw)
+        h = C.c_void_p()
+        assert lib.sk_gpu_pdlp_session_create(m, n, int(rp[-1]), sg.iptr(rp), sg.iptr(cj), sg.dptr(vv),
+                                              *[sg.dptr(a) for a in vec], C.byref(P), C.byref(h)) == 0
+        return h
+
+    x, y = np.zeros(n), np.zeros(m)
+    st = sg.PdlpStats()
+    h = session(eps_optimal=1e-4)
+    assert lib.sk_gpu_pdlp_session_solve(h, None, None, sg.dptr(x), sg.dptr(y), C.byref(st)) == 0
+    assert st.converged
+    first = st.iterations
+    assert lib.sk_gpu_pdlp_session_set_limits(h, 500000, 0.0, 1e-7) == 0
+    st2 = sg.PdlpStats()
+    assert lib.sk_gpu_pdlp_session_resume(h, sg.dptr(x), sg.dptr(y), C.byref(st2)) == 0
+    assert st2.converged and st2.iterations > first
+    kkt = so.relative_kkt(model, x, y)
+    assert max(kkt[:3]) <= 1.05e-7, kkt
+    stc = sg.PdlpStats()
+    assert lib.sk_gpu_pdlp_session_solve(h, None, None, sg.dptr(x), sg.dptr(y), C.byref(stc)) == 0
+    assert stc.converged
+    assert st2.iterations <= 1.5 * stc.iterations + 640, (st2.iterations, stc.iterations)
+    lib.sk_gpu_pdlp_session_destroy(h)
+    # nothing to resume on a fresh session
+    h = session()
+    assert lib.sk_gpu_pdlp_session_resume(h, sg.dptr(x), sg.dptr(y), C.byref(st2)) == -1
+    lib.sk_gpu_pdlp_session_destroy(h)

@@ -606,39 +606,6 @@ static int interval_infeasible(const sk_model *m, double tol)
     return impossible;
 }
 
-/* Adds one round's statistics to the running totals of an sk_gpu_solve. */
-static void stats_accumulate(sk_gpu_pdlp_stats *acc, const sk_gpu_pdlp_stats *st, int round)
-{
-    int k;
-    if (round == 0) { *acc = *st; return; }
-    for (k = 0; k < 4; ++k)
-        if (!acc->iterations_to_merit[k] && st->iterations_to_merit[k]) {
-            acc->iterations_to_merit[k] = acc->iterations + st->iterations_to_merit[k];
-            acc->seconds_to_merit[k] = acc->loop_seconds + st->seconds_to_merit[k];
-        }
-    acc->iterations += st->iterations;
-    acc->fp32_iterations += st->fp32_iterations;
-    acc->fp64_iterations += st->fp64_iterations;
-    acc->restarts += st->restarts;
-    acc->reanchors += st->reanchors;
-    acc->host_syncs += st->host_syncs;
-    acc->kernel_launches += st->kernel_launches;
-    acc->loop_seconds += st->loop_seconds;
-    acc->setup_seconds += st->setup_seconds;
-    acc->seconds += st->seconds;
-    acc->converged = st->converged;
-    acc->cancelled = st->cancelled;
-    acc->primal_objective = st->primal_objective;
-    acc->dual_objective = st->dual_objective;
-    acc->relative_primal_residual = st->relative_primal_residual;
-    acc->relative_dual_residual = st->relative_dual_residual;
-    acc->relative_gap = st->relative_gap;
-    acc->final_step_size = st->final_step_size;
-    acc->step_size_ratio = st->step_size_ratio;
-    acc->primal_weight = st->primal_weight;
-    acc->rejected_steps += st->rejected_steps;
-}
-
 /* The certificate margins of the reference's first-order path
  * (c/src/sk_solve.c:987-996): a QP needs sk_verify's primal infeasibility
  * within 100 primal_tol and its dual infeasibility and complementarity
@@ -707,45 +674,58 @@ sk_status sk_gpu_solve(const sk_model *m, const sk_options *o, const sk_gpu_opti
     s->ncol = m->ncol; s->nrow = m->nrow;
     {
         sk_gpu_pdlp_params P = g->pdlp;
-        const double *wx = g->x0, *wy = g->y0;
+        sk_gpu_pdlp_session *S = NULL;
+        long long budget;
         if (o->iteration_limit > 0 && o->iteration_limit < P.max_iterations) P.max_iterations = (int)o->iteration_limit;
+        budget = P.max_iterations;
+        if (o->time_limit > 0.0) P.time_limit_seconds = o->time_limit - (skg_now() - t0);
         eps = P.eps_optimal;
         kkt[0] = kkt[1] = kkt[2] = INFINITY;
+        /* the full symmetric Q's CSC arrays read as CSR are Q itself */
+        if (sk_gpu_pdlp_session_create_qp(m->nrow, m->ncol, A.nnz, A.p, A.j, A.v, m->Q ? m->Q->p : NULL,
+                                          m->Q ? m->Q->i : NULL, m->Q ? m->Q->x : NULL,
+                                          m->Q ? m->Q->p[m->ncol] : 0, m->c, m->rlow, m->rupp, m->clow,
+                                          m->cupp, &P, &S)) {
+            hcsr_free(&A); sk_gpu_solution_free(s); return SK_ERR_NUMERIC;
+        }
         for (round = 0; round < 4; ++round) {
-            sk_gpu_pdlp_stats st;
-            if (o->time_limit > 0.0) {
-                /* what is left of the caller's limit for this round */
-                P.time_limit_seconds = o->time_limit - (skg_now() - t0);
-                if (P.time_limit_seconds <= 0.0) break;
+            if (round > 0) {
+                /* continue the same loop (iterate, step, primal weight,
+                   averages) with a 10x tighter tolerance, within what is
+                   left of the iteration and time limits */
+                const double left = o->time_limit > 0.0 ? o->time_limit - (skg_now() - t0) : 0.0;
+                const long long left_it = budget - acc.iterations;
+                if (left_it <= 0 || (o->time_limit > 0.0 && left <= 0.0)) break;
+                P.eps_optimal *= 0.1;
+                if (sk_gpu_pdlp_session_set_limits(S, (int)left_it, left, P.eps_optimal)) break;
+                rc = sk_gpu_pdlp_session_resume(S, x, y, &acc);
+            } else {
+                if (o->time_limit > 0.0 && P.time_limit_seconds <= 0.0) break;
+                rc = sk_gpu_pdlp_session_solve(S, g->x0, g->y0, x, y, &acc);
             }
-            /* the full symmetric Q's CSC arrays read as CSR are Q itself */
-            rc = sk_gpu_pdlp_solve(m->nrow, m->ncol, A.nnz, A.p, A.j, A.v, m->Q ? m->Q->p : NULL,
-                                   m->Q ? m->Q->i : NULL, m->Q ? m->Q->x : NULL,
-                                   m->Q ? m->Q->p[m->ncol] : 0, m->c, m->rlow, m->rupp, m->clow,
-                                   m->cupp, wx, wy, &P, x, y, &st);
-            if (rc == -1) { hcsr_free(&A); sk_gpu_solution_free(s); return SK_ERR_NUMERIC; }
-            iters += st.iterations;
-            stats_accumulate(&acc, &st, round);
+            if (rc == -1) { sk_gpu_pdlp_session_destroy(S); hcsr_free(&A); sk_gpu_solution_free(s); return SK_ERR_NUMERIC; }
+            iters = acc.iterations;
             verified = 0;
             if (rc == -2) break;
             if (relkkt_hcsr(&A, m->Q ? &Qh : NULL, m->c, m->rlow, m->rupp, m->clow, m->cupp, x, y, kkt)) {
-                hcsr_free(&A); sk_gpu_solution_free(s); return SK_ERR_NUMERIC;
+                sk_gpu_pdlp_session_destroy(S); hcsr_free(&A); sk_gpu_solution_free(s); return SK_ERR_NUMERIC;
             }
-            if (!st.converged || st.cancelled) break;
+            if ((P.verbosity > 0 || o->verbosity > 0) && (!acc.converged || acc.cancelled))
+                fprintf(stderr, "sk_gpu_solve round %d: eps %.1e, %d iterations, not converged (rc %d, cancelled "
+                        "%d), relative KKT %.3e %.3e %.3e\n", round, P.eps_optimal, acc.iterations, rc,
+                        acc.cancelled, kkt[0], kkt[1], kkt[2]);
+            if (!acc.converged || acc.cancelled) break;
             if (sk_gpu_verify(m, s) != SK_OK) break;
             verified = 1;
             cert = certificate_ok(m, o, s);
             if (P.verbosity > 0 || o->verbosity > 0)
                 fprintf(stderr, "sk_gpu_solve round %d: eps %.1e, %d iterations, relative KKT %.3e %.3e %.3e, "
-                        "verify pinf %.3e dinf %.3e comp %.3e -> %s\n", round, P.eps_optimal, st.iterations,
+                        "verify pinf %.3e dinf %.3e comp %.3e -> %s\n", round, P.eps_optimal, acc.iterations,
                         kkt[0], kkt[1], kkt[2], s->primal_infeasibility, s->dual_infeasibility,
                         s->complementarity, cert ? "certified" : "not certified");
             if (kkt[0] <= eps && kkt[1] <= eps && kkt[2] <= eps && cert) break;
-            P.eps_optimal *= 0.1;
-            P.max_iterations -= st.iterations;
-            if (P.max_iterations <= 0) break;
-            wx = x; wy = y;
         }
+        sk_gpu_pdlp_session_destroy(S);
     }
     hcsr_free(&A);
     s->iterations = iters;

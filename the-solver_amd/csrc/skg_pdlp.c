@@ -11,6 +11,7 @@
 #define _POSIX_C_SOURCE 200809L
 #include "skg_internal.h"
 
+#include <limits.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -500,11 +501,24 @@ static int run_window(engine *E, int prec, int period, int adaptive, double fixe
     return 0;
 }
 
+/* Host state of the restarted loop, kept in the session so that a solve
+ * can be resumed with tighter limits from exactly where it stopped (same
+ * iterate, step size, primal weight, running averages and restart state). */
+typedef struct loop_state {
+    int active;                  /* a solve ran and can be resumed          */
+    int prec, tail, stall, use_avg_final;
+    int iteration, windows_since_restart, its_since_restart;
+    double omega, mu_at_restart, mu_previous, best_mu_seen, loop_seconds;
+    kkt best;
+} loop_state;
+
 struct sk_gpu_pdlp_session {
     engine E;
     sk_gpu_pdlp_params P;
     double norm, eta_global;
     int ready;
+    loop_state L;
+    sk_gpu_pdlp_stats stats;     /* cumulative over a solve and its resumes */
 };
 
 static int check_problem(int rows, int cols, int nnz, const int *rp, const int *ci, const double *v,
@@ -727,145 +741,129 @@ int sk_gpu_pdlp_session_iterate(sk_gpu_pdlp_session *S, int iterations, double *
     return 0;
 }
 
-int sk_gpu_pdlp_session_solve(sk_gpu_pdlp_session *S, const double *x0, const double *y0,
-                              double *x_out, double *y_out, sk_gpu_pdlp_stats *stats)
+/* The restarted loop (GpuPdlp.cpp:311-499) from the session's loop state,
+ * for at most `budget` more iterations, until convergence, the time limit
+ * (counted from this call), a cancel or divergence; then the export of the
+ * result to x_out, y_out in the original space.  S->stats accumulates. */
+static int loop_run(sk_gpu_pdlp_session *S, int budget, double *x_out, double *y_out)
 {
-    engine *E;
-    sk_gpu_pdlp_params P;
+    engine *E = &S->E;
+    const sk_gpu_pdlp_params *P = &S->P;
+    loop_state *L = &S->L;
+    sk_gpu_pdlp_stats *stats = &S->stats;
     const double t0 = skg_now();
-    double omega = 1.0;
-    double mu_at_restart = INFINITY, mu_previous = INFINITY, best_mu_seen = INFINITY;
-    int iteration = 0, windows_since_restart = 0, its_since_restart = 0, prec, rc = 0, k, rows, cols;
-    int stall = 0, tail = 0, diverged = 0, use_avg_final = 0;
-    kkt best;
-    skg_clear_error();
-    if (!stats) return skg_fail("null stats");
-    memset(stats, 0, sizeof(*stats));
-    if (!S || !S->ready) return skg_fail("invalid session");
-    E = &S->E;
-    P = S->P;
-    rows = E->m; cols = E->n;
-    if (!x_out || (rows > 0 && !y_out)) return skg_fail("null output pointer");
-    memset(&best, 0, sizeof(best));
-    best.primal = best.dual = best.gap = INFINITY;
-    E->launches = 0;
-    stats->matrix_norm = S->norm;
-    prec = E->mixed ? SKK_F32 : SKK_F64;
-    if (session_start(S, prec, x0, y0, &omega)) return -1;
-    if ((rc = skk_sync())) return skg_hip_fail("setup", rc);
-    stats->setup_seconds = skg_now() - t0;
-    {
-        const double tl0 = skg_now();
-        while (iteration < P.max_iterations) {
-            const int period = P.restart_period < P.max_iterations - iteration ? P.restart_period
-                                                                              : P.max_iterations - iteration;
-            kkt cur, avg;
-            int average_better, sufficient, necessary, artificial;
-            double mu;
-            if (period <= 0) break;
-            if (run_window(E, prec, period, P.adaptive_step,
-                           (double)(windows_since_restart + 1) * (double)P.restart_period, P.use_graph,
-                           P.restart_period)) return -1;
-            stats->host_syncs++;
-            iteration += period;
-            if (prec == SKK_F32) stats->fp32_iterations += period; else stats->fp64_iterations += period;
-            its_since_restart += period;
-            ++windows_since_restart;
-            cur = kkt_from(E->h_eval, E->cost_norm, E->bound_norm);
-            avg = kkt_from(E->h_eval + 8, E->cost_norm, E->bound_norm);
-            average_better = kkt_worst(&avg) < kkt_worst(&cur);
-            best = average_better ? avg : cur;
-            if (P.verbosity > 1)
-                fprintf(stderr, "it %d prec %s cur %.3e avg %.3e (p %.2e d %.2e g %.2e) eta %.3e omega %.3e\n",
-                        iteration, prec == SKK_F32 ? "f32" : "f64", kkt_worst(&cur), kkt_worst(&avg),
-                        best.primal, best.dual, best.gap, E->h_state->eta, E->h_state->omega);
-            if (!isfinite(kkt_worst(&cur)) && !isfinite(kkt_worst(&avg))) { diverged = 1; break; }
-            {
-                static const double marks[4] = {1e-3, 1e-4, 1e-5, 1e-6};
-                const double merit = kkt_worst(&best);
-                for (k = 0; k < 4; ++k)
-                    if (!stats->iterations_to_merit[k] && merit <= marks[k]) {
-                        stats->iterations_to_merit[k] = iteration;
-                        stats->seconds_to_merit[k] = skg_now() - tl0;
-                    }
-            }
-            if (best.primal <= P.eps_optimal && best.dual <= P.eps_optimal && best.gap <= P.eps_optimal) {
-                use_avg_final = average_better;
-                stats->converged = 1;
-                break;
-            }
-            use_avg_final = average_better;
-            if (P.time_limit_seconds > 0.0 && skg_now() - t0 > P.time_limit_seconds) break;
-            /* GpuPdlp.cpp:430-434: cooperative cancel at the window sync */
-            if (P.cancel && *P.cancel) { stats->cancelled = 1; break; }
-            mu = kkt_worst(&best);
-            /* mixed precision: hand over to FP64 once FP32 is near its floor.
-               Plain FP32 iterates: at tail_switch, or when within 100x of it
-               no window has improved the best merit by 10% for
-               tail_stall_windows windows.  Anchored iterates have no floor of
-               their own (the rounding shrinks with the correction, and every
-               restart moves the anchor) and follow the FP64 engine's
-               iteration counts (the 500k QP 9,472 vs 9,664, the 20k x 50k
-               LP family 14,912 vs 14,912); where they stall, the FP64 engine
-               stalls at the same merit (the 1M x 2M LP: relative gap 1.27e-3
-               in both), so they never hand over. */
-            if (prec == SKK_F32 && !tail && !E->P.anchored) {
-                if (mu < 0.9 * best_mu_seen) { best_mu_seen = mu; stall = 0; }
-                else ++stall;
-                if (mu <= P.tail_switch || (mu <= 100.0 * P.tail_switch && stall >= P.tail_stall_windows)) {
-                    if ((rc = skk_pdhg_restart(&E->P, SKK_F32, average_better, P.adaptive_step)) ||
-                        (rc = skk_pdhg_promote(&E->P)) ||
-                        (rc = skk_pdhg_eval(&E->P, SKK_F64, 0, P.adaptive_step, 1.0)) ||
-                        (rc = skk_pdhg_restart(&E->P, SKK_F64, 0, P.adaptive_step)))
-                        return skg_hip_fail("precision switch", rc);
-                    E->launches += 8;
-                    prec = SKK_F64;
-                    tail = 1;
-                    windows_since_restart = 0;
-                    its_since_restart = 0;
-                    mu_at_restart = mu;
-                    mu_previous = INFINITY;
-                    use_avg_final = 0;
-                    continue;
+    const int rows = E->m, cols = E->n;
+    const int stop_at = budget > INT_MAX - L->iteration ? INT_MAX : L->iteration + budget;
+    int rc, k, diverged = 0;
+    stats->converged = 0;
+    stats->cancelled = 0;
+    while (L->iteration < stop_at) {
+        const int period = P->restart_period < stop_at - L->iteration ? P->restart_period : stop_at - L->iteration;
+        kkt cur, avg;
+        int average_better, sufficient, necessary, artificial;
+        double mu;
+        if (period <= 0) break;
+        if (run_window(E, L->prec, period, P->adaptive_step,
+                       (double)(L->windows_since_restart + 1) * (double)P->restart_period, P->use_graph,
+                       P->restart_period)) return -1;
+        stats->host_syncs++;
+        L->iteration += period;
+        if (L->prec == SKK_F32) stats->fp32_iterations += period; else stats->fp64_iterations += period;
+        L->its_since_restart += period;
+        ++L->windows_since_restart;
+        cur = kkt_from(E->h_eval, E->cost_norm, E->bound_norm);
+        avg = kkt_from(E->h_eval + 8, E->cost_norm, E->bound_norm);
+        average_better = kkt_worst(&avg) < kkt_worst(&cur);
+        L->best = average_better ? avg : cur;
+        if (P->verbosity > 1)
+            fprintf(stderr, "it %d prec %s cur %.3e avg %.3e (p %.2e d %.2e g %.2e) eta %.3e omega %.3e\n",
+                    L->iteration, L->prec == SKK_F32 ? "f32" : "f64", kkt_worst(&cur), kkt_worst(&avg),
+                    L->best.primal, L->best.dual, L->best.gap, E->h_state->eta, E->h_state->omega);
+        if (!isfinite(kkt_worst(&cur)) && !isfinite(kkt_worst(&avg))) { diverged = 1; break; }
+        {
+            static const double marks[4] = {1e-3, 1e-4, 1e-5, 1e-6};
+            const double merit = kkt_worst(&L->best);
+            for (k = 0; k < 4; ++k)
+                if (!stats->iterations_to_merit[k] && merit <= marks[k]) {
+                    stats->iterations_to_merit[k] = L->iteration;
+                    stats->seconds_to_merit[k] = L->loop_seconds + (skg_now() - t0);
                 }
-            }
-            /* restart only when earned (GpuPdlp.cpp:437-478) */
-            sufficient = mu <= 0.2 * mu_at_restart;
-            necessary = mu <= 0.8 * mu_at_restart && mu > mu_previous;
-            artificial = (double)its_since_restart >= 0.36 * (double)iteration;
-            mu_previous = mu;
-            if (!(sufficient || necessary || artificial)) continue;
-            if ((rc = skk_pdhg_restart(&E->P, prec, average_better, P.adaptive_step)))
-                return skg_hip_fail("restart launch", rc);
-            E->launches += 1;
-            if (prec == SKK_F32 && E->P.anchored) {
-                /* the restart point becomes the new anchor */
-                if ((rc = skk_pdhg_reanchor(&E->P, 1))) return skg_hip_fail("re-anchor", rc);
-                E->launches += 5 + (E->has_q ? 2 : 0);
-                stats->reanchors++;
-            }
-            use_avg_final = 0;
-            windows_since_restart = 0;
-            its_since_restart = 0;
-            mu_at_restart = mu;
-            mu_previous = INFINITY;
-            stats->restarts++;
-            /* primal weight (GpuPdlp.cpp:485-493) */
-            if (best.pmove > 1e-14 && best.dmove > 1e-14) {
-                const double target = best.dmove / best.pmove;
-                const double updated = exp(0.5 * log(target) + 0.5 * log(omega));
-                if (isfinite(updated)) {
-                    omega = fmin(1e6, fmax(1e-6, updated));
-                    if ((rc = skk_pdhg_set_omega(&E->P, omega))) return skg_hip_fail("primal weight update", rc);
-                    E->launches += 1;
-                }
+        }
+        L->use_avg_final = average_better;
+        if (L->best.primal <= P->eps_optimal && L->best.dual <= P->eps_optimal && L->best.gap <= P->eps_optimal) {
+            stats->converged = 1;
+            break;
+        }
+        if (P->time_limit_seconds > 0.0 && skg_now() - t0 > P->time_limit_seconds) break;
+        /* GpuPdlp.cpp:430-434: cooperative cancel at the window sync */
+        if (P->cancel && *P->cancel) { stats->cancelled = 1; break; }
+        mu = kkt_worst(&L->best);
+        /* mixed precision: hand over to FP64 once FP32 is near its floor.
+           Plain FP32 iterates: at tail_switch, or when within 100x of it
+           no window has improved the best merit by 10% for
+           tail_stall_windows windows.  Anchored iterates have no floor of
+           their own (the rounding shrinks with the correction, and every
+           restart moves the anchor) and follow the FP64 engine's
+           iteration counts (the 500k QP 9,472 vs 9,664, the 20k x 50k
+           LP family 14,912 vs 14,912); where they stall, the FP64 engine
+           stalls at the same merit (the 1M x 2M LP: relative gap 1.27e-3
+           in both), so they never hand over. */
+        if (L->prec == SKK_F32 && !L->tail && !E->P.anchored) {
+            if (mu < 0.9 * L->best_mu_seen) { L->best_mu_seen = mu; L->stall = 0; }
+            else ++L->stall;
+            if (mu <= P->tail_switch || (mu <= 100.0 * P->tail_switch && L->stall >= P->tail_stall_windows)) {
+                if ((rc = skk_pdhg_restart(&E->P, SKK_F32, average_better, P->adaptive_step)) ||
+                    (rc = skk_pdhg_promote(&E->P)) ||
+                    (rc = skk_pdhg_eval(&E->P, SKK_F64, 0, P->adaptive_step, 1.0)) ||
+                    (rc = skk_pdhg_restart(&E->P, SKK_F64, 0, P->adaptive_step)))
+                    return skg_hip_fail("precision switch", rc);
+                E->launches += 8;
+                L->prec = SKK_F64;
+                L->tail = 1;
+                L->windows_since_restart = 0;
+                L->its_since_restart = 0;
+                L->mu_at_restart = mu;
+                L->mu_previous = INFINITY;
+                L->use_avg_final = 0;
+                continue;
             }
         }
-        if ((rc = skk_pdhg_export(&E->P, prec, use_avg_final, E->pv, E->pw)) ||
-            (rc = skk_d2h_async(E->h_state, E->P.st, sizeof(skk_state))) || (rc = skk_sync()))
-            return skg_hip_fail("export", rc);
-        stats->loop_seconds = skg_now() - tl0;
+        /* restart only when earned (GpuPdlp.cpp:437-478) */
+        sufficient = mu <= 0.2 * L->mu_at_restart;
+        necessary = mu <= 0.8 * L->mu_at_restart && mu > L->mu_previous;
+        artificial = (double)L->its_since_restart >= 0.36 * (double)L->iteration;
+        L->mu_previous = mu;
+        if (!(sufficient || necessary || artificial)) continue;
+        if ((rc = skk_pdhg_restart(&E->P, L->prec, average_better, P->adaptive_step)))
+            return skg_hip_fail("restart launch", rc);
+        E->launches += 1;
+        if (L->prec == SKK_F32 && E->P.anchored) {
+            /* the restart point becomes the new anchor */
+            if ((rc = skk_pdhg_reanchor(&E->P, 1))) return skg_hip_fail("re-anchor", rc);
+            E->launches += 5 + (E->has_q ? 2 : 0);
+            stats->reanchors++;
+        }
+        L->use_avg_final = 0;
+        L->windows_since_restart = 0;
+        L->its_since_restart = 0;
+        L->mu_at_restart = mu;
+        L->mu_previous = INFINITY;
+        stats->restarts++;
+        /* primal weight (GpuPdlp.cpp:485-493) */
+        if (L->best.pmove > 1e-14 && L->best.dmove > 1e-14) {
+            const double target = L->best.dmove / L->best.pmove;
+            const double updated = exp(0.5 * log(target) + 0.5 * log(L->omega));
+            if (isfinite(updated)) {
+                L->omega = fmin(1e6, fmax(1e-6, updated));
+                if ((rc = skk_pdhg_set_omega(&E->P, L->omega))) return skg_hip_fail("primal weight update", rc);
+                E->launches += 1;
+            }
+        }
     }
+    if ((rc = skk_pdhg_export(&E->P, L->prec, L->use_avg_final, E->pv, E->pw)) ||
+        (rc = skk_d2h_async(E->h_state, E->P.st, sizeof(skk_state))) || (rc = skk_sync()))
+        return skg_hip_fail("export", rc);
+    L->loop_seconds += skg_now() - t0;
     {
         double *xs = (double *)malloc(sizeof(double) * (size_t)(cols + 1));
         double *ys = (double *)malloc(sizeof(double) * (size_t)(rows + 1));
@@ -879,14 +877,15 @@ int sk_gpu_pdlp_session_solve(sk_gpu_pdlp_session *S, const double *x0, const do
         for (k = 0; k < rows; ++k) y_out[E->perm_r[k]] = ys[k] * E->R[E->perm_r[k]];
         free(xs); free(ys);
     }
-    stats->iterations = iteration;
-    stats->primal_objective = best.pobj;
-    stats->dual_objective = best.dobj;
-    stats->relative_primal_residual = best.primal;
-    stats->relative_dual_residual = best.dual;
-    stats->relative_gap = best.gap;
-    stats->primal_weight = omega;
-    if (P.adaptive_step) {
+    stats->loop_seconds = L->loop_seconds;
+    stats->iterations = L->iteration;
+    stats->primal_objective = L->best.pobj;
+    stats->dual_objective = L->best.dobj;
+    stats->relative_primal_residual = L->best.primal;
+    stats->relative_dual_residual = L->best.dual;
+    stats->relative_gap = L->best.gap;
+    stats->primal_weight = L->omega;
+    if (P->adaptive_step) {
         const double rej = E->h_state->updates - E->h_state->accept_count;
         stats->final_step_size = E->h_state->eta;
         stats->step_size_ratio = E->h_state->eta * S->norm;
@@ -896,9 +895,56 @@ int sk_gpu_pdlp_session_solve(sk_gpu_pdlp_session *S, const double *x0, const do
         stats->step_size_ratio = 0.9;
     }
     stats->kernel_launches = E->launches;
-    stats->seconds = skg_now() - t0;
-    if (diverged) { skg_fail("PDHG iterate became non-finite"); return -2; }
+    if (diverged) {
+        L->active = 0;
+        skg_fail("PDHG iterate became non-finite");
+        return -2;
+    }
+    L->active = 1;
     return 0;
+}
+
+int sk_gpu_pdlp_session_solve(sk_gpu_pdlp_session *S, const double *x0, const double *y0,
+                              double *x_out, double *y_out, sk_gpu_pdlp_stats *stats)
+{
+    engine *E;
+    const double t0 = skg_now();
+    int rc;
+    skg_clear_error();
+    if (!stats) return skg_fail("null stats");
+    memset(stats, 0, sizeof(*stats));
+    if (!S || !S->ready) return skg_fail("invalid session");
+    E = &S->E;
+    if (!x_out || (E->m > 0 && !y_out)) return skg_fail("null output pointer");
+    memset(&S->L, 0, sizeof(S->L));
+    memset(&S->stats, 0, sizeof(S->stats));
+    S->L.best.primal = S->L.best.dual = S->L.best.gap = INFINITY;
+    S->L.mu_at_restart = S->L.mu_previous = S->L.best_mu_seen = INFINITY;
+    E->launches = 0;
+    S->stats.matrix_norm = S->norm;
+    S->L.prec = E->mixed ? SKK_F32 : SKK_F64;
+    if (session_start(S, S->L.prec, x0, y0, &S->L.omega)) return -1;
+    if ((rc = skk_sync())) return skg_hip_fail("setup", rc);
+    S->stats.setup_seconds = skg_now() - t0;
+    rc = loop_run(S, S->P.max_iterations, x_out, y_out);
+    S->stats.seconds = skg_now() - t0;
+    *stats = S->stats;
+    return rc;
+}
+
+int sk_gpu_pdlp_session_resume(sk_gpu_pdlp_session *S, double *x_out, double *y_out, sk_gpu_pdlp_stats *stats)
+{
+    const double t0 = skg_now();
+    int rc;
+    skg_clear_error();
+    if (!stats) return skg_fail("null stats");
+    if (!S || !S->ready) return skg_fail("invalid session");
+    if (!S->L.active) return skg_fail("no solve to resume on this session");
+    if (!x_out || (S->E.m > 0 && !y_out)) return skg_fail("null output pointer");
+    rc = loop_run(S, S->P.max_iterations, x_out, y_out);
+    S->stats.seconds += skg_now() - t0;
+    *stats = S->stats;
+    return rc;
 }
 
 int sk_gpu_pdlp_solve(int rows, int cols, int nnz, const int *row_offsets, const int *column_indices,
