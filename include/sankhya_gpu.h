@@ -562,6 +562,26 @@ int sk_gpu_relative_kkt(int rows, int cols, int nnz, const int *row_offsets,
                         const double *row_upper, const double *col_lower,
                         const double *col_upper, const double *x,
                         const double *y, double out[5]);
+/* The relative KKT triple of sk_gpu_relative_kkt for an LP already on the
+ * device, allocating nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  matrix:
+ * a handle from sk_gpu_csr_create; c, bounds (IEEE inf or the 1e30
+ * sentinel), x, y: device vectors in the original space; device_out:
+ * 5 device doubles (primal, dual, gap, primal objective, dual objective).
+ * Queued on the library stream; read device_out after a null-stream copy
+ * or sk_gpu_synchronize.  Scratch: block partials in a static device
+ * array of 256 KB per GPU (two products of at most 4,096 workgroups,
+ * 4 sums each), so there is no limit on rows, columns or nonzeros; calls
+ * that overlap on one GPU share that array and race.  0, -1 (bad
+ * argument) or a positive hipError_t. */
+int sk_gpu_relative_kkt_device_noalloc(const sk_gpu_csr *matrix, const double *device_c,
+                                       const double *device_row_lower,
+                                       const double *device_row_upper,
+                                       const double *device_col_lower,
+                                       const double *device_col_upper,
+                                       const double *device_x, const double *device_y,
+                                       double *device_out);
 
 /* The same triple with an optional full symmetric Hessian (CSR, n x n):
  * primal objective c'x + x'Qx/2, reduced cost c + Qx + A'y. */

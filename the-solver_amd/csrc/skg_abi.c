@@ -567,6 +567,27 @@ int sk_gpu_relative_kkt_qp(int rows, int cols, int nnz, const int *row_offsets, 
     return relkkt_hcsr(&A, q_row_offsets ? &Q : NULL, c, row_lower, row_upper, col_lower, col_upper, x, y, out);
 }
 
+int sk_gpu_relative_kkt_device_noalloc(const sk_gpu_csr *matrix, const double *device_c,
+                                       const double *device_row_lower, const double *device_row_upper,
+                                       const double *device_col_lower, const double *device_col_upper,
+                                       const double *device_x, const double *device_y, double *device_out)
+{
+    const csr_plan *p = plan_of(matrix);
+    int rc;
+    skg_clear_error();
+    if (!matrix || !p) return skg_fail("null device CSR");
+    if (!device_out || (matrix->cols > 0 && (!device_c || !device_col_lower || !device_col_upper || !device_x)) ||
+        (matrix->rows > 0 && (!device_row_lower || !device_row_upper || !device_y)))
+        return skg_fail("null device vector pointer");
+    if (p->A.k.grid > SKK_SPMV_GRID || p->AT.k.grid > SKK_SPMV_GRID)
+        return skg_fail("product grid of %d / %d workgroups exceeds the %d partials of the static scratch",
+                        p->A.k.grid, p->AT.k.grid, SKK_SPMV_GRID);
+    if ((rc = skk_relkkt_noalloc(&p->A.k, &p->AT.k, device_c, device_row_lower, device_row_upper, device_col_lower,
+                                 device_col_upper, device_x, device_y, device_out)))
+        return skg_hip_fail("relative KKT (no allocation)", rc);
+    return 0;
+}
+
 /* -------------------------------------------------- sk_solve mirror */
 
 void sk_gpu_solver_options_default(sk_gpu_options *g)

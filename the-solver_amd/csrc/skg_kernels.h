@@ -233,6 +233,14 @@ typedef struct skk_verify_in {
 } skk_verify_in;
 int skk_verify(const skk_verify_in *V);
 
+/* Relative KKT triple of an LP from device vectors with no allocation: the
+ * block partials live in a static device array of 2 x 4 x SKK_SPMV_GRID
+ * doubles (looked up once per device with hipGetSymbolAddress); out (device,
+ * 5 doubles) receives primal, dual, gap, primal and dual objective.  Calls
+ * that overlap on one device share that array and race. */
+int skk_relkkt_noalloc(const skk_csr *A, const skk_csr *AT, const double *c, const double *rl, const double *ru,
+                       const double *l, const double *u, const double *x, const double *y, double *out);
+
 /* PDLP relative KKT sums in FP64 (same slots as the loop's evaluation):
  * out[0] ||Ax-proj Ax||^2, [1] dual violation^2, [2] c'x (+x'Qx), [3] dual
  * objective, [4] x'Qx.  Bounds with |b| >= 1e30 are absent. */

@@ -1511,6 +1511,89 @@ template <typename T> int restart(const skk_pdhg &P, int use_avg, int adaptive)
     return launch_status();
 }
 
+// ---------------------------------------- relative KKT without allocation
+//
+// The relative KKT triple of an LP (GpuPdlp.cpp:171-176, 226-249) from two
+// products whose epilogues reduce the residual terms directly, so only
+// block partials need device memory: row i of A x gives the primal
+// residual, the support term of y_i and the row-bound norms; row j of
+// A^T y gives the reduced cost c_j + (A^T y)_j and the column terms.
+
+struct EpiKktRow {
+    static constexpr int kSlots = 4;
+    const double *rl, *ru, *y;
+    double *part;
+    int stride;
+    template <class K> __device__ void begin(const K &) {}
+    __device__ void row(int i, double a, double *p) const
+    {
+        const double lo = sk_ninf(rl[i]) ? -INFINITY : rl[i];
+        const double hi = sk_inf(ru[i]) ? INFINITY : ru[i];
+        const double pr = a - fmin(fmax(a, lo), hi);
+        const double yi = y[i];
+        double support = 0.0;
+        if (yi > 0.0) support = isfinite(hi) ? yi * hi : 0.0;
+        else if (yi < 0.0) support = isfinite(lo) ? yi * lo : 0.0;
+        p[0] += pr * pr;
+        p[1] -= support;
+        if (isfinite(lo)) p[2] += lo * lo;
+        if (isfinite(hi)) p[3] += hi * hi;
+    }
+    __device__ void end(double *p) const
+    {
+        double v[4] = {p[0], p[1], p[2], p[3]};
+        store_partials<4>(v, part, stride);
+    }
+};
+
+struct EpiKktCol {
+    static constexpr int kSlots = 4;
+    const double *c, *l, *u, *x;
+    double *part;
+    int stride;
+    template <class K> __device__ void begin(const K &) {}
+    __device__ void row(int j, double s, double *p) const
+    {
+        const double r = c[j] + s, xj = x[j];
+        const double lo = sk_ninf(l[j]) ? -INFINITY : l[j];
+        const double hi = sk_inf(u[j]) ? INFINITY : u[j];
+        double viol = 0.0, term = 0.0;
+        const bool lf = isfinite(lo), hf = isfinite(hi);
+        if (lf && hf) term = fmin(r * lo, r * hi);
+        else if (lf) { viol = fmax(0.0, -r); term = fmax(0.0, r) * lo; }
+        else if (hf) { viol = fmax(0.0, r); term = fmin(0.0, r) * hi; }
+        else viol = fabs(r);
+        p[0] += viol * viol;
+        p[1] += c[j] * xj;
+        p[2] += term;
+        p[3] += c[j] * c[j];
+    }
+    __device__ void end(double *p) const
+    {
+        double v[4] = {p[0], p[1], p[2], p[3]};
+        store_partials<4>(v, part, stride);
+    }
+};
+
+// One workgroup: the two products' partials in a fixed order, then
+// out[0] primal, [1] dual, [2] gap, [3] primal objective, [4] dual objective.
+__global__ __launch_bounds__(kBlock) void k_kkt_finalize(const double *rows, int nrow_blocks, const double *cols,
+                                                         int ncol_blocks, int stride, double *out)
+{
+    double t[8];
+    for (int s = 0; s < 4; ++s) t[s] = combine(rows + (size_t)s * stride, nrow_blocks, false);
+    for (int s = 0; s < 4; ++s) t[4 + s] = combine(cols + (size_t)s * stride, ncol_blocks, false);
+    if (threadIdx.x != 0) return;
+    const double bound_norm = sqrt(fmax(0.0, t[2] + t[3]));
+    const double cost_norm = sqrt(fmax(0.0, t[7]));
+    double primal = sqrt(fmax(0.0, t[0])) / (1.0 + bound_norm);
+    double dual = sqrt(fmax(0.0, t[4])) / (1.0 + cost_norm);
+    const double pobj = t[5], dobj = t[6] + t[1];
+    double gap = fabs(pobj - dobj) / (1.0 + fabs(pobj) + fabs(dobj));
+    if (!isfinite(primal) || !isfinite(dual) || !isfinite(gap)) primal = dual = gap = INFINITY;
+    out[0] = primal; out[1] = dual; out[2] = gap; out[3] = pobj; out[4] = dobj;
+}
+
 }  // namespace
 
 // ======================================================== extern "C" layer
@@ -1549,6 +1632,15 @@ __global__ __launch_bounds__(kBlock) void k_probe_gather(const int *__restrict__
     }
     out[blockIdx.x * kBlock + threadIdx.x] = s;
 }
+
+// Device scratch of skk_relkkt_noalloc: the block partials of its two
+// products, 4 slots of at most SKK_SPMV_GRID workgroups each (256 KB).
+// Every GPU holds its own copy until the process ends; the kernels take
+// its device address, looked up once per device.
+constexpr int kNoAllocSlots = 4;
+constexpr int kNoAllocRegion = kNoAllocSlots * SKK_SPMV_GRID;
+static __device__ __align__(256) double scratch[2 * kNoAllocRegion];
+static double *g_scratch[kMaxDevices] = {};
 
 extern "C" {
 
@@ -1968,6 +2060,29 @@ int skk_legacy_check(const skk_legacy *L, int have_prev)
     hipLaunchKernelGGL(k_legacy_check, dim3(L->grid), dim3(kBlock), 0, stream(), K);
     if ((rc = launch_status())) return rc;
     return finalize(L->part, L->grid, L->grid, 2, 6, L->out);
+}
+
+int skk_relkkt_noalloc(const skk_csr *A, const skk_csr *AT, const double *c, const double *rl, const double *ru,
+                       const double *l, const double *u, const double *x, const double *y, double *out)
+{
+    int dev = 0;
+    if (check(hipGetDevice(&dev))) return g_last;
+    if (dev < 0 || dev >= kMaxDevices) return -1;
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        if (check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)))) return g_last;
+        g_scratch[dev] = static_cast<double *>(p);
+    }
+    if (A->grid > SKK_SPMV_GRID || AT->grid > SKK_SPMV_GRID) return -1;
+    double *rows = g_scratch[dev], *cols = g_scratch[dev] + kNoAllocRegion;
+    int rc;
+    EpiKktRow er{rl, ru, y, rows, SKK_SPMV_GRID};
+    if ((rc = launch_spmv<double, double, double>(*A, A->vd, x, x, nullptr, er))) return rc;
+    EpiKktCol ec{c, l, u, x, cols, SKK_SPMV_GRID};
+    if ((rc = launch_spmv<double, double, double>(*AT, AT->vd, y, y, nullptr, ec))) return rc;
+    hipLaunchKernelGGL(k_kkt_finalize, dim3(1), dim3(kBlock), 0, stream(), rows, A->ntiles ? A->grid : 0, cols,
+                       AT->ntiles ? AT->grid : 0, SKK_SPMV_GRID, out);
+    return launch_status();
 }
 
 /* Random-gather issue rate of this device: `count` 4-byte gathers from a

@@ -232,6 +232,8 @@ def load_gpu():
         _declare(lib, "sk_gpu_spmv_transpose_device_f64", C.c_int, [H, C.c_void_p, C.c_void_p])
         _declare(lib, "sk_gpu_axpy_device_f64", C.c_int, [C.c_int, C.c_double, C.c_void_p, C.c_void_p])
         _declare(lib, "sk_gpu_synchronize", C.c_int, [])
+        _declare(lib, "sk_gpu_relative_kkt_device_noalloc", C.c_int,
+                 [H] + [C.c_void_p] * 8)
         _declare(lib, "sk_gpu_spmv_f64", C.c_int, [C.c_int, C.c_int, C.c_int, c_int_p, c_int_p,
                                                    c_double_p, c_double_p, c_double_p])
         _declare(lib, "sk_gpu_axpy_f64", C.c_int, [C.c_int, C.c_double, c_double_p, c_double_p])
