@@ -371,6 +371,10 @@ typedef struct sk_gpu_pdlp_stats {   /* src/cuda/GpuPdlp.hpp:69-92 PdlpStats */
                                     relative KKT merit max(primal, dual, gap)
                                     is <= 1e-3, 1e-4, 1e-5, 1e-6 (0: never) */
     double seconds_to_merit[4];  /* loop seconds at those windows           */
+    double restart_merit[2];     /* last window's restart merits of the
+                                    current and the averaged point: their
+                                    relative KKT (restart_rule KKT) or their
+                                    normalized duality gap (NDG)            */
 } sk_gpu_pdlp_stats;
 
 void sk_gpu_pdlp_params_default(sk_gpu_pdlp_params *p);

@@ -853,6 +853,80 @@ static double finite_norm(const double *v, int n)   /* GpuPdlp.cpp:84-90 */
     return sqrt(acc);
 }
 
+/* ---------------------------------------------- normalized duality gap
+ *
+ * Restart merit of Applegate et al. 2021 (PDLP, s.3.1), not in the
+ * reference (GpuPdlp.cpp:437-451 restarts on the relative KKT).  For the
+ * LP saddle function L(x, y) = c'x + y'Ax - p(y), p the support function
+ * of [rl, ru], the gap of z = (x, y) within the omega-ball of radius
+ * r = ||z - z_r||_omega around z (z_r: the last restart point) is
+ *     G = max { L(x, yh) - L(xh, y) : xh in [l, u], ||zh - z||_omega <= r }
+ * and the merit is G / r.  With a multiplier on the ball the maximiser is
+ * one projected step of length t: xh = proj(x - t g / omega), g = c + A'y,
+ * and yh = prox_{s p}(y + s A x) with s = t omega; the distance grows with
+ * t, so t is found by bisection on the distance (exact to rounding here;
+ * the device uses a three-stage 32-point search).  Written with
+ * differences, G = sum_j g_j (x_j - xh_j) + sum_i [(yh_i - y_i)(Ax)_i -
+ * p_i(yh_i) + p_i(y_i)], so no objective-sized terms cancel. */
+static double support(double v, double lo, double hi)
+{
+    if (v > 0.0) return isfinite(hi) ? v * hi : 0.0;
+    if (v < 0.0) return isfinite(lo) ? v * lo : 0.0;
+    return 0.0;
+}
+
+static void ndg_at(double t, double omega, int n, int m, const double *x, const double *y, const double *ax,
+                   const double *g, const double *l, const double *u, const double *rl, const double *ru,
+                   double *d2, double *G)
+{
+    double dist = 0.0, gap = 0.0;
+    int j, i;
+    for (j = 0; j < n; ++j) {
+        const double d = clamp_to(x[j] - t * g[j] / omega, l[j], u[j]) - x[j];
+        dist += omega * d * d;
+        gap -= g[j] * d;
+    }
+    for (i = 0; i < m; ++i) {
+        const double s = t * omega, v = y[i] + s * ax[i];
+        const double yh = v - s * clamp_to(v / s, rl[i], ru[i]);
+        const double d = yh - y[i];
+        dist += d * d / omega;
+        gap += d * ax[i] - support(yh, rl[i], ru[i]) + support(y[i], rl[i], ru[i]);
+    }
+    *d2 = dist;
+    *G = gap;
+}
+
+double orc_ndg(int n, int m, const double *x, const double *y, const double *ax, const double *g,
+               const double *l, const double *u, const double *rl, const double *ru, const double *xr,
+               const double *yr, double omega)
+{
+    double r2 = 0.0, r, lo = 0.0, hi, d2, G, gn = 0.0;
+    int j, i, k;
+    for (j = 0; j < n; ++j) r2 += omega * (x[j] - xr[j]) * (x[j] - xr[j]);
+    for (i = 0; i < m; ++i) r2 += (y[i] - yr[i]) * (y[i] - yr[i]) / omega;
+    r = sqrt(r2);
+    if (!(r > 0.0)) return 0.0;
+    for (j = 0; j < n; ++j) gn += g[j] * g[j] / omega;
+    for (i = 0; i < m; ++i) gn += omega * ax[i] * ax[i];
+    /* d(t) <= t sqrt(gn) (projections are nonexpansive), so t >= r / sqrt(gn) */
+    hi = gn > 0.0 ? r / sqrt(gn) : 1.0;
+    for (k = 0; k < 200; ++k) {
+        ndg_at(hi, omega, n, m, x, y, ax, g, l, u, rl, ru, &d2, &G);
+        if (d2 >= r2) break;
+        lo = hi;
+        hi *= 2.0;
+    }
+    for (k = 0; k < 100 && lo > 0.0; ++k) {
+        const double mid = sqrt(lo * hi);
+        ndg_at(mid, omega, n, m, x, y, ax, g, l, u, rl, ru, &d2, &G);
+        if (d2 <= r2) lo = mid; else hi = mid;
+    }
+    if (!(lo > 0.0)) lo = hi;
+    ndg_at(lo, omega, n, m, x, y, ax, g, l, u, rl, ru, &d2, &G);
+    return G / r;
+}
+
 /* GpuPdlp::solve, src/cuda/GpuPdlp.cpp:252-534, with the kernels of
  * src/cuda/PdlpKernels.cu:71-97 (fixed step) and 360-518 (adaptive step). */
 int orc_pdlp_solve(const orc_csr *A, const double *c, const double *l, const double *u,
@@ -981,6 +1055,23 @@ int orc_pdlp_solve(const orc_csr *A, const double *c, const double *l, const dou
         st->host_syncs++;
         average_better = kkt_worst(&avg) < kkt_worst(&cur);
         best = average_better ? avg : cur;
+        st->restart_merit[0] = kkt_worst(&cur);
+        st->restart_merit[1] = kkt_worst(&avg);
+        if (P->restart_rule == SK_GPU_RESTART_NDG) {
+            /* the restart merits (only): NDG of the current and the
+               averaged point against the last restart point */
+            double *g2 = (double *)malloc(sizeof(double) * ((size_t)n + 1));
+            double *a2 = (double *)malloc(sizeof(double) * ((size_t)m + 1));
+            orc_csr_mv(A, x, a2);
+            orc_csr_mv(&At, y, g2);
+            for (j = 0; j < n; ++j) g2[j] += c[j];
+            st->restart_merit[0] = orc_ndg(n, m, x, y, a2, g2, l, u, rl, ru, xr, yr, omega);
+            orc_csr_mv(A, xa, a2);
+            orc_csr_mv(&At, ya, g2);
+            for (j = 0; j < n; ++j) g2[j] += c[j];
+            st->restart_merit[1] = orc_ndg(n, m, xa, ya, a2, g2, l, u, rl, ru, xr, yr, omega);
+            free(g2); free(a2);
+        }
         if (best.primal <= P->eps_optimal && best.dual <= P->eps_optimal && best.gap <= P->eps_optimal) {
             if (average_better) {
                 memcpy(x, xa, (size_t)n * sizeof(double));
@@ -991,6 +1082,12 @@ int orc_pdlp_solve(const orc_csr *A, const double *c, const double *l, const dou
         }
         if (P->time_limit_seconds > 0.0 && wall_seconds() - t0 > P->time_limit_seconds) break;
         mu = kkt_worst(&best);
+        if (P->restart_rule == SK_GPU_RESTART_NDG) {
+            /* restart to the point of lower NDG, judged by its NDG */
+            average_better = st->restart_merit[1] < st->restart_merit[0];
+            best = average_better ? avg : cur;
+            mu = st->restart_merit[average_better ? 1 : 0];
+        }
         sufficient = mu <= 0.2 * mu_at_restart;
         necessary = mu <= 0.8 * mu_at_restart && mu > mu_previous;
         artificial = (double)its_since_restart >= 0.36 * (double)iteration;

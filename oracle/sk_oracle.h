@@ -17,6 +17,9 @@
  *   orc_first_order        src/lp/LpSolver.cpp:62-99 (run_first_order)
  *   orc_relative_kkt       src/cuda/GpuPdlp.cpp:171-176, 226-249 and
  *                          src/cuda/PdlpKernels.cu:101-209
+ *   orc_ndg                no reference counterpart: the normalized
+ *                          duality gap of Applegate et al. 2021, s.3.1,
+ *                          the restart merit of restart_rule NDG
  *
  * Pinning: see the header of sk_oracle.c.
  */
@@ -57,6 +60,12 @@ int orc_cuda_pdhg(const orc_csr *A, const orc_csr *Q, const double *qdiag,
 void   orc_ruiz(const orc_csr *a, int max_iters, double tol, double *R, double *C);
 void   orc_pock_chambolle(const orc_csr *a, double alpha, double *R, double *C);
 double orc_power_norm(const orc_csr *a, int iterations);
+/* Normalized duality gap of (x, y) against the restart point (xr, yr) for
+ * the LP min c'x, rl <= Ax <= ru, l <= x <= u (IEEE +-inf bounds); ax = A x,
+ * g = c + A'y given; omega the primal weight.  0 when z = z_r. */
+double orc_ndg(int n, int m, const double *x, const double *y, const double *ax, const double *g,
+               const double *l, const double *u, const double *rl, const double *ru, const double *xr,
+               const double *yr, double omega);
 
 /* PDLP on an already scaled LP (bounds: IEEE +-inf means absent). */
 int orc_pdlp_solve(const orc_csr *A, const double *c, const double *l,

@@ -64,6 +64,7 @@ def load():
         _decl(lib, "orc_first_order", C.c_int, [R, D, D, D, D, D, C.POINTER(sg.PdlpParams), D, D,
                                                 C.POINTER(sg.PdlpStats)])
         _decl(lib, "orc_relative_kkt", None, [R, R, D, D, D, D, D, D, D, D])
+        _decl(lib, "orc_ndg", C.c_double, [C.c_int, C.c_int] + [D] * 10 + [C.c_double])
         _orc = lib
     return _orc
 
@@ -286,3 +287,15 @@ def ref_postsolve(problem, reduced_x, max_passes=20):
     if lib.refps_postsolve(C.byref(s), max_passes, sg.dptr(rx) if rx.size else None, rx.size, sg.dptr(x)) != 0:
         raise RuntimeError("reference postsolve raised")
     return x[:problem.cols]
+
+
+def ndg(A, c, l, u, rl, ru, x, y, xr, yr, omega):
+    """orc_ndg for a dense or scipy matrix A (IEEE inf bounds)."""
+    import scipy.sparse as sp
+    A = sp.csr_matrix(A)
+    arr = [np.ascontiguousarray(a, np.float64) for a in (x, y)]
+    ax = np.asarray(A @ arr[0]).ravel()
+    g = np.asarray(c, np.float64) + np.asarray(A.T @ arr[1]).ravel()
+    vs = [arr[0], arr[1], ax, g] + [np.ascontiguousarray(a, np.float64) for a in (l, u, rl, ru, xr, yr)]
+    vs = [v if v.size else np.zeros(1) for v in vs]
+    return load().orc_ndg(A.shape[1], A.shape[0], *[sg.dptr(v) for v in vs], float(omega))

@@ -227,3 +227,71 @@ def test_validation_semantics():
     assert orc.orc_model_validate(bad.ptr()) == sg.SK_ERR_STRUCTURE
     crossed = _tiny_model([[1.0]], [1.0], [3.0], [2.0], [0.0], [1.0])
     assert orc.orc_model_validate(crossed.ptr()) == sg.SK_ERR_STRUCTURE
+
+
+# ---------------------------------------------------------------- NDG pins
+
+def test_ndg_closed_form_one_variable():
+    """min 2x, x in [0, 10], omega = 4: from x = 3 with restart point 5 the
+    ball of radius sqrt(4)*2 = 4 allows a step of 2 toward the cheaper side,
+    G = 2 * 2, merit 4 / 4; from x = 1 (restart 5, r = 8) the bound stops
+    the step at 0, G = 2 * 1, merit 2 / 8."""
+    import numpy as np
+    A = np.zeros((0, 1))
+    e = np.zeros(0)
+    v = so.ndg(A, [2.0], [0.0], [10.0], e, e, [3.0], e, [5.0], e, 4.0)
+    assert abs(v - 1.0) <= 1e-12, v
+    v = so.ndg(A, [2.0], [0.0], [10.0], e, e, [1.0], e, [5.0], e, 4.0)
+    assert abs(v - 0.25) <= 1e-12, v
+    # no movement since the restart point: 0
+    assert so.ndg(A, [2.0], [0.0], [10.0], e, e, [5.0], e, [5.0], e, 4.0) == 0.0
+
+
+def test_ndg_is_the_maximum_over_the_ball_brute_force():
+    """A 2 x 2 LP: the merit times r bounds the gap of every sampled point of
+    the box/ball intersection and is nearly attained by one of them."""
+    import numpy as np
+    A = np.array([[1.0, 2.0], [3.0, -1.0]])
+    c = np.array([1.0, -2.0])
+    l, u = np.array([0.0, -1.0]), np.array([4.0, 3.0])
+    rl, ru = np.array([-1.0, -np.inf]), np.array([2.0, 5.0])
+    x, y = np.array([1.0, 0.5]), np.array([0.3, -0.2])
+    xr, yr = np.array([1.6, 0.1]), np.array([-0.1, 0.2])
+    om = 1.7
+    v = so.ndg(A, c, l, u, rl, ru, x, y, xr, yr, om)
+    r = np.sqrt(om * np.sum((x - xr) ** 2) + np.sum((y - yr) ** 2) / om)
+
+    def supp(yv):
+        return np.where(yv > 0, np.where(np.isfinite(ru), yv * ru, 0.0),
+                        np.where(yv < 0, np.where(np.isfinite(rl), yv * rl, 0.0), 0.0)).sum(-1)
+
+    def gap(xh, yh):   # L(x, yh) - L(xh, y), L = c'x + y'Ax - p(y)
+        return (c @ x + yh @ (A @ x) - supp(yh)) - (xh @ c + (xh @ A.T) @ y - supp(y))
+
+    rng = np.random.default_rng(0)
+    d = rng.standard_normal((400000, 4))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d *= rng.uniform(0, 1, (len(d), 1)) ** 0.25 * r
+    xh = x + d[:, :2] / np.sqrt(om)
+    yh = y + d[:, 2:] * np.sqrt(om)
+    ok = np.all((xh >= l) & (xh <= u), axis=1)
+    g = gap(xh[ok], yh[ok])
+    assert g.max() <= v * r * (1 + 1e-12) + 1e-12
+    assert g.max() >= 0.97 * v * r, (g.max(), v * r)
+
+
+def test_ndg_vanishes_at_a_saddle_point():
+    """The textbook LP of tests/lp/test_pdlp.cpp:53-70 (max 3x + 5y, optimum
+    (2, 6)) with its optimal multipliers: no point of any ball has a positive
+    gap, so the merit is 0 whatever the restart point."""
+    import numpy as np
+    A = np.array([[1.0, 0.0], [0.0, 2.0], [3.0, 2.0]])
+    c = np.array([-3.0, -5.0])
+    l, u = np.zeros(2), np.full(2, np.inf)
+    rl, ru = np.full(3, -np.inf), np.array([4.0, 12.0, 18.0])
+    xs = np.array([2.0, 6.0])
+    ys = np.array([0.0, 1.5, 1.0])   # c + A'y = 0, y >= 0 on the active rows
+    assert np.allclose(c + A.T @ ys, 0)
+    for xr, yr in ((np.zeros(2), np.zeros(3)), (np.array([3.0, 1.0]), np.array([1.0, 0.0, 2.0]))):
+        v = so.ndg(A, c, l, u, rl, ru, xs, ys, xr, yr, 2.0)
+        assert abs(v) <= 1e-12, v

@@ -115,12 +115,13 @@ class PdlpStats(C.Structure):
                 ("kernel_launches", C.c_longlong), ("loop_seconds", C.c_double),
                 ("setup_seconds", C.c_double), ("primal_weight", C.c_double),
                 ("reanchors", C.c_int), ("iterations_to_merit", C.c_int * 4),
-                ("seconds_to_merit", C.c_double * 4)]
+                ("seconds_to_merit", C.c_double * 4), ("restart_merit", C.c_double * 2)]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
         d["iterations_to_merit"] = list(self.iterations_to_merit)
         d["seconds_to_merit"] = list(self.seconds_to_merit)
+        d["restart_merit"] = list(self.restart_merit)
         return d
 
 
