@@ -170,6 +170,13 @@ typedef struct skk_pdhg {
     double *hx, *hy;                         /* anchor, FP64                */
     double *hax, *hg, *hqx;                  /* A hx, c + A^T hy + Q hx, Q hx */
     float *ycf;                              /* FP32 copy of hy             */
+    /* Normalized-duality-gap restart merit (LP): each evaluation also
+     * writes the NDG of the evaluated point against the last restart
+     * point to eval_out[16 + which], in FP64. */
+    int ndg;
+    double *nx, *ng, *ny, *nax;              /* x, c + A^T y, y, A x (FP64)  */
+    double *ndg_s;                           /* search state, 8 doubles     */
+    double *ndg_part;                        /* 32 x part_capacity partials */
 } skk_pdhg;
 
 enum { SKK_F64 = 0, SKK_F32 = 1 };

@@ -900,6 +900,201 @@ __global__ __launch_bounds__(kBlock) void k_residuals_sel(Iter<T> I, int which,
     store_partials<8>(acc, part, stride);
 }
 
+// ------------------------------------------- normalized duality gap (LP)
+//
+// The restart merit of Applegate et al. 2021 (s.3.1) for the evaluated
+// point z = (x, y): G / r, G the largest gap L(x, yh) - L(xh, y) over the
+// box and the omega-ball of radius r = ||z - z_r||_omega around z (z_r the
+// last restart point).  The maximiser is one projected step of length t,
+// xh = proj_[l,u](x - t g / omega), g = c + A^T y, yh = prox of the support
+// function with step s = t omega, and the ball's distance grows with t:
+// t is found by three rounds of 32 candidates (ratio 2 from the lower
+// bound t0 = r / sqrt(|g|^2/omega + omega |Ax|^2), then 2^(1/31), then
+// 2^(1/961)), each one pass over the vectors, and G is evaluated at the
+// largest candidate inside the ball.  All in FP64.
+
+constexpr int kNdgCand = 32;
+enum { N_TBASE = 0, N_RATIO = 1, N_R2 = 2, N_R = 3 };
+
+// The evaluated point in FP64 (anchor added) and the two norms of t0.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ndg_prep(Iter<T> I, int which, const double *__restrict__ ax,
+                                                     const double *__restrict__ aty, Data<double> D, Anchor H,
+                                                     const skk_state *__restrict__ st, int n, int m, double *nx,
+                                                     double *ng, double *ny, double *nax, double *part, int stride)
+{
+    const int cur = st->cur;
+    const T *__restrict__ x = which ? I.xavg : I.x(cur);
+    const T *__restrict__ y = which ? I.yavg : I.y(cur);
+    double acc[2] = {0.0, 0.0};
+    const int gs = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gs) {
+        const double xd = x[j];
+        const double g = (H.x ? H.g[j] : D.c[j]) + aty[j];
+        nx[j] = H.x ? H.x[j] + xd : xd;
+        ng[j] = g;
+        acc[0] += g * g;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gs) {
+        const double yd = y[i];
+        const double a = H.x ? H.ax[i] + ax[i] : ax[i];
+        ny[i] = H.x ? H.y[i] + yd : yd;
+        nax[i] = a;
+        acc[1] += a * a;
+    }
+    store_partials<2>(acc, part, stride);
+}
+
+// One workgroup: r from the evaluation's distance sums (slots 6, 7 of
+// `sums`), t0, and the first search range.
+__global__ __launch_bounds__(kBlock) void k_ndg_init(const double *part, int stride, int count,
+                                                     const double *__restrict__ sums,
+                                                     const skk_state *__restrict__ st, double *s)
+{
+    const double gx = combine(part, count, false), gy = combine(part + stride, count, false);
+    if (threadIdx.x != 0) return;
+    const double om = st->omega;
+    const double r2 = om * fmax(0.0, sums[6]) + fmax(0.0, sums[7]) / om;
+    const double gn = gx / om + om * gy;
+    s[N_TBASE] = gn > 0.0 && r2 > 0.0 ? sqrt(r2 / gn) : 1.0;
+    s[N_RATIO] = 2.0;
+    s[N_R2] = r2;
+    s[N_R] = sqrt(r2);
+}
+
+__device__ __forceinline__ double ndg_dual(double yv, double a, double s, double rl, double ru)
+{
+    // prox of the support function of [rl, ru] at v = y + s a, step s
+    const double v = yv + s * a;
+    return fmin(fmax(0.0, v - s * ru), v - s * rl);
+}
+
+__device__ __forceinline__ double ndg_support(double v, double lo, double hi)
+{
+    if (v > 0.0) return isfinite(hi) ? v * hi : 0.0;
+    if (v < 0.0) return isfinite(lo) ? v * lo : 0.0;
+    return 0.0;
+}
+
+// The squared omega-distance of the step for kNdgCand lengths at once.
+__global__ __launch_bounds__(kBlock) void k_ndg_search(const double *__restrict__ nx, const double *__restrict__ ng,
+                                                       const double *__restrict__ ny, const double *__restrict__ nax,
+                                                       Data<double> D, const skk_state *__restrict__ st,
+                                                       const double *__restrict__ s, int n, int m, double *part,
+                                                       int stride)
+{
+    const double om = st->omega, t0 = s[N_TBASE], ratio = s[N_RATIO];
+    double t[kNdgCand], acc[kNdgCand];
+    t[0] = t0;
+#pragma unroll
+    for (int k = 1; k < kNdgCand; ++k) t[k] = t[k - 1] * ratio;
+#pragma unroll
+    for (int k = 0; k < kNdgCand; ++k) acc[k] = 0.0;
+    const int gs = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gs) {
+        const double xj = nx[j], gj = ng[j] / om, lo = D.l[j], hi = D.u[j];
+#pragma unroll
+        for (int k = 0; k < kNdgCand; ++k) {
+            const double d = fmin(fmax(xj - t[k] * gj, lo), hi) - xj;
+            acc[k] += om * d * d;
+        }
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gs) {
+        const double yi = ny[i], a = nax[i], rl = D.rl[i], ru = D.ru[i];
+#pragma unroll
+        for (int k = 0; k < kNdgCand; ++k) {
+            const double d = ndg_dual(yi, a, t[k] * om, rl, ru) - yi;
+            acc[k] += d * d / om;
+        }
+    }
+    store_partials<kNdgCand>(acc, part, stride);
+}
+
+// One workgroup: the largest candidate inside the ball starts the next,
+// finer range (ratio^(1/31)); when every candidate is inside, the range
+// moves on at the same ratio.
+__global__ __launch_bounds__(kBlock) void k_ndg_pick(const double *part, int stride, int count, double *s)
+{
+    __shared__ double d2[kNdgCand];
+    for (int k = 0; k < kNdgCand; ++k) {
+        const double v = combine(part + (size_t)k * stride, count, false);
+        if (threadIdx.x == 0) d2[k] = v;
+    }
+    if (threadIdx.x != 0) return;
+    const double r2 = s[N_R2], t0 = s[N_TBASE], ratio = s[N_RATIO];
+    int best = -1;
+    for (int k = 0; k < kNdgCand; ++k) if (d2[k] <= r2) best = k;
+    if (best < 0) {                                   // rounding at t0: step back
+        s[N_TBASE] = t0 / ratio;
+    } else if (best == kNdgCand - 1) {
+        s[N_TBASE] = t0 * pow(ratio, (double)(kNdgCand - 1));
+    } else {
+        s[N_TBASE] = t0 * pow(ratio, (double)best);
+        s[N_RATIO] = pow(ratio, 1.0 / (kNdgCand - 1));
+    }
+}
+
+// The gap at the chosen step length, as differences (no objective-sized
+// terms cancel): sum_j g_j (x_j - xh_j) + sum_i [(yh_i - y_i)(Ax)_i
+// - p_i(yh_i) + p_i(y_i)].
+__global__ __launch_bounds__(kBlock) void k_ndg_gap(const double *__restrict__ nx, const double *__restrict__ ng,
+                                                    const double *__restrict__ ny, const double *__restrict__ nax,
+                                                    Data<double> D, const skk_state *__restrict__ st,
+                                                    const double *__restrict__ s, int n, int m, double *part,
+                                                    int stride)
+{
+    const double om = st->omega, t = s[N_TBASE];
+    double acc[1] = {0.0};
+    const int gs = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gs) {
+        const double xj = nx[j], g = ng[j];
+        const double d = fmin(fmax(xj - t * g / om, D.l[j]), D.u[j]) - xj;
+        acc[0] -= g * d;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gs) {
+        const double yi = ny[i], a = nax[i], rl = D.rl[i], ru = D.ru[i];
+        const double yh = ndg_dual(yi, a, t * om, rl, ru);
+        acc[0] += (yh - yi) * a - ndg_support(yh, rl, ru) + ndg_support(yi, rl, ru);
+    }
+    store_partials<1>(acc, part, stride);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ndg_value(const double *part, int count, const double *__restrict__ s,
+                                                      double *out)
+{
+    const double G = combine(part, count, false);
+    if (threadIdx.x != 0) return;
+    const double r = s[N_R];
+    *out = r > 0.0 && isfinite(G) ? G / r : (r > 0.0 ? INFINITY : 0.0);
+}
+
+template <typename T> int ndg_eval(const skk_pdhg &P, const Iter<T> &I, int which)
+{
+    int rc;
+    const Data<double> D = data_of<double>(P);
+    const int g = P.elem_grid, stride = P.part_capacity;
+    hipLaunchKernelGGL(k_ndg_prep<T>, dim3(g), dim3(kBlock), 0, stream(), I, which, P.ax_d, P.aty_d, D,
+                       anchor_of(P, std::is_same<T, float>::value), P.st, P.n, P.m, P.nx, P.ng, P.ny, P.nax,
+                       P.ndg_part, stride);
+    if ((rc = launch_status())) return rc;
+    hipLaunchKernelGGL(k_ndg_init, dim3(1), dim3(kBlock), 0, stream(), P.ndg_part, stride, g,
+                       P.eval_out + 8 * which, P.st, P.ndg_s);
+    if ((rc = launch_status())) return rc;
+    for (int round = 0; round < 3; ++round) {
+        hipLaunchKernelGGL(k_ndg_search, dim3(g), dim3(kBlock), 0, stream(), P.nx, P.ng, P.ny, P.nax, D, P.st,
+                           P.ndg_s, P.n, P.m, P.ndg_part, stride);
+        if ((rc = launch_status())) return rc;
+        hipLaunchKernelGGL(k_ndg_pick, dim3(1), dim3(kBlock), 0, stream(), P.ndg_part, stride, g, P.ndg_s);
+        if ((rc = launch_status())) return rc;
+    }
+    hipLaunchKernelGGL(k_ndg_gap, dim3(g), dim3(kBlock), 0, stream(), P.nx, P.ng, P.ny, P.nax, D, P.st, P.ndg_s,
+                       P.n, P.m, P.ndg_part, stride);
+    if ((rc = launch_status())) return rc;
+    hipLaunchKernelGGL(k_ndg_value, dim3(1), dim3(kBlock), 0, stream(), P.ndg_part, g, P.ndg_s,
+                       P.eval_out + 16 + which);
+    return launch_status();
+}
+
 // AX[a0] <- A x from the FP64 product of the evaluation.
 template <typename T>
 __global__ void k_to_t_sel(Iter<T> I, const double *__restrict__ src, int m,
@@ -1494,6 +1689,7 @@ template <typename T> int eval(const skk_pdhg &P, int which, int adaptive, doubl
                        P.has_q ? P.qx_d : nullptr, anchor_of(P, prec_index<T>() == SKK_F32));
     if ((rc = launch_status())) return rc;
     if ((rc = finalize(rp, P.part_capacity, P.elem_grid, 8, 0, P.eval_out + 8 * which))) return rc;
+    if (P.ndg && (rc = ndg_eval<T>(P, I, which))) return rc;
     if (!which && adaptive) {
         // the live A x (and Q x) are refreshed from the FP64 products (GpuPdlp.cpp:367)
         hipLaunchKernelGGL(k_to_t_sel<T>, dim3(P.elem_grid), dim3(kBlock), 0, stream(), I, P.ax_d,

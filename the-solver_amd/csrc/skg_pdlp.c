@@ -407,14 +407,22 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
             (rc = dalloc(E, (void **)&Q->st, sizeof(skk_state))) ||
             (rc = dalloc(E, (void **)&Q->snap, sizeof(skk_state))) ||
             (rc = dalloc(E, (void **)&Q->part, sizeof(double) * 16 * (size_t)cap)) ||
-            (rc = dalloc(E, (void **)&Q->eval_out, sizeof(double) * 16)) ||
+            (rc = dalloc(E, (void **)&Q->eval_out, sizeof(double) * 24)) ||
             (rc = dalloc(E, (void **)&E->pv, 8 * nn)) || (rc = dalloc(E, (void **)&E->pw, 8 * mm)) ||
             (rc = dalloc(E, (void **)&E->pv2, 8 * nn)) || (rc = dalloc(E, (void **)&E->pnorm, 8)))
             goto done;
-        if ((rc = skk_host_alloc((void **)&E->h_eval, sizeof(double) * 16 + sizeof(skk_state)))) {
+        if (P->restart_rule == SK_GPU_RESTART_NDG) {
+            Q->ndg = 1;
+            if ((rc = dalloc(E, (void **)&Q->nx, 8 * nn)) || (rc = dalloc(E, (void **)&Q->ng, 8 * nn)) ||
+                (rc = dalloc(E, (void **)&Q->ny, 8 * mm)) || (rc = dalloc(E, (void **)&Q->nax, 8 * mm)) ||
+                (rc = dalloc(E, (void **)&Q->ndg_s, sizeof(double) * 8)) ||
+                (rc = dalloc(E, (void **)&Q->ndg_part, sizeof(double) * 32 * (size_t)cap)))
+                goto done;
+        }
+        if ((rc = skk_host_alloc((void **)&E->h_eval, sizeof(double) * 24 + sizeof(skk_state)))) {
             rc = skg_hip_fail("hipHostMalloc", rc); goto done;
         }
-        E->h_state = (skk_state *)(E->h_eval + 16);
+        E->h_state = (skk_state *)(E->h_eval + 24);
     }
     rc = 0;
 done:
@@ -495,7 +503,7 @@ static int run_window(engine *E, int prec, int period, int adaptive, double fixe
     } else {
         if ((rc = queue_window(E, prec, period, adaptive, fixed_terms))) return rc;
     }
-    if ((rc = skk_d2h_async(E->h_eval, E->P.eval_out, sizeof(double) * 16)) ||
+    if ((rc = skk_d2h_async(E->h_eval, E->P.eval_out, sizeof(double) * 18)) ||
         (rc = skk_d2h_async(E->h_state, E->P.st, sizeof(skk_state))) || (rc = skk_sync()))
         return skg_hip_fail("window read-back", rc);
     return 0;
@@ -543,7 +551,8 @@ static int check_params(const sk_gpu_pdlp_params *P)
 {
     if (P->eps_optimal <= 0.0 || P->max_iterations <= 0 || P->restart_period <= 0 ||
         P->power_iterations < 0 || P->eta_min_factor <= 0.0 || P->eta_max_factor < P->eta_min_factor ||
-        (P->precision != SK_GPU_PREC_FP64 && P->precision != SK_GPU_PREC_MIXED))
+        (P->precision != SK_GPU_PREC_FP64 && P->precision != SK_GPU_PREC_MIXED) ||
+        (P->restart_rule != SK_GPU_RESTART_KKT && P->restart_rule != SK_GPU_RESTART_NDG))
         return skg_fail("invalid PDLP parameters");
     return 0;
 }
@@ -583,6 +592,10 @@ int sk_gpu_pdlp_session_create_qp(int rows, int cols, int nnz, const int *row_of
     if (params) S->P = *params; else sk_gpu_pdlp_params_default(&S->P);
     if (check_params(&S->P) || skg_have_device()) { free(S); return -1; }
     if (has_q && !S->P.adaptive_step) { free(S); return skg_fail("a Hessian needs the adaptive step"); }
+    if (has_q && S->P.restart_rule == SK_GPU_RESTART_NDG) {
+        free(S);
+        return skg_fail("NDG restarts are for LPs (the trust-region step of a QP is not a projection)");
+    }
     S->E.mixed = S->P.precision == SK_GPU_PREC_MIXED;
     S->E.has_q = has_q;
     if (engine_build(&S->E, rows, cols, row_offsets, column_indices, values, q_row_offsets,
@@ -759,7 +772,7 @@ static int loop_run(sk_gpu_pdlp_session *S, int budget, double *x_out, double *y
     stats->cancelled = 0;
     while (L->iteration < stop_at) {
         const int period = P->restart_period < stop_at - L->iteration ? P->restart_period : stop_at - L->iteration;
-        kkt cur, avg;
+        kkt cur, avg, cand;
         int average_better, sufficient, necessary, artificial;
         double mu;
         if (period <= 0) break;
@@ -775,6 +788,8 @@ static int loop_run(sk_gpu_pdlp_session *S, int budget, double *x_out, double *y
         avg = kkt_from(E->h_eval + 8, E->cost_norm, E->bound_norm);
         average_better = kkt_worst(&avg) < kkt_worst(&cur);
         L->best = average_better ? avg : cur;
+        stats->restart_merit[0] = E->P.ndg ? E->h_eval[16] : kkt_worst(&cur);
+        stats->restart_merit[1] = E->P.ndg ? E->h_eval[17] : kkt_worst(&avg);
         if (P->verbosity > 1)
             fprintf(stderr, "it %d prec %s cur %.3e avg %.3e (p %.2e d %.2e g %.2e) eta %.3e omega %.3e\n",
                     L->iteration, L->prec == SKK_F32 ? "f32" : "f64", kkt_worst(&cur), kkt_worst(&avg),
@@ -798,6 +813,16 @@ static int loop_run(sk_gpu_pdlp_session *S, int budget, double *x_out, double *y
         /* GpuPdlp.cpp:430-434: cooperative cancel at the window sync */
         if (P->cancel && *P->cancel) { stats->cancelled = 1; break; }
         mu = kkt_worst(&L->best);
+        cand = L->best;
+        if (E->P.ndg) {
+            /* NDG restarts (Applegate et al. 2021, s.3.1): the restart
+               candidate is the point of lower normalized duality gap and
+               its NDG is the merit the criteria compare; termination stays
+               on the relative KKT */
+            average_better = stats->restart_merit[1] < stats->restart_merit[0];
+            mu = stats->restart_merit[average_better ? 1 : 0];
+            cand = average_better ? avg : cur;
+        }
         /* mixed precision: hand over to FP64 once FP32 is near its floor.
            Plain FP32 iterates: at tail_switch, or when within 100x of it
            no window has improved the best merit by 10% for
@@ -850,8 +875,8 @@ static int loop_run(sk_gpu_pdlp_session *S, int budget, double *x_out, double *y
         L->mu_previous = INFINITY;
         stats->restarts++;
         /* primal weight (GpuPdlp.cpp:485-493) */
-        if (L->best.pmove > 1e-14 && L->best.dmove > 1e-14) {
-            const double target = L->best.dmove / L->best.pmove;
+        if (cand.pmove > 1e-14 && cand.dmove > 1e-14) {
+            const double target = cand.dmove / cand.pmove;
             const double updated = exp(0.5 * log(target) + 0.5 * log(L->omega));
             if (isfinite(updated)) {
                 L->omega = fmin(1e6, fmax(1e-6, updated));
