@@ -440,6 +440,16 @@ int  sk_gpu_pdlp_session_iterate(sk_gpu_pdlp_session *session, int iterations,
  * (GpuPdlp::solve always starts over); -1 when no solve ran. */
 int  sk_gpu_pdlp_session_resume(sk_gpu_pdlp_session *session, double *x, double *y,
                                 sk_gpu_pdlp_stats *stats);
+/* A warm start with the cold start as its fallback (no reference
+ * counterpart: the reference always starts at zero, c/src/sk_solve.c:871).
+ * Solves from (x0, y0) -- a learned prediction, say -- for at most
+ * warm_iterations iterations; if that does not reach eps_optimal, solves
+ * again from zero under the session's own limits (time counted from the
+ * call).  *used_warm tells which start produced x, y; stats->iterations
+ * counts both attempts. */
+int  sk_gpu_pdlp_session_solve_fallback(sk_gpu_pdlp_session *session, const double *x0,
+                                        const double *y0, int warm_iterations, double *x,
+                                        double *y, sk_gpu_pdlp_stats *stats, int *used_warm);
 /* The stopping limits of later solves and resumes (GpuPdlp.hpp:19-37
  * max_iterations, time_limit_seconds, eps_optimal); the resident operator
  * and the loop state are kept. */

@@ -957,6 +957,39 @@ int sk_gpu_pdlp_session_solve(sk_gpu_pdlp_session *S, const double *x0, const do
     return rc;
 }
 
+int sk_gpu_pdlp_session_solve_fallback(sk_gpu_pdlp_session *S, const double *x0, const double *y0,
+                                       int warm_iterations, double *x_out, double *y_out,
+                                       sk_gpu_pdlp_stats *stats, int *used_warm)
+{
+    const int budget = S ? S->P.max_iterations : 0;
+    const double limit = S ? S->P.time_limit_seconds : 0.0;
+    const double t0 = skg_now();
+    int rc, warm_its = 0;
+    skg_clear_error();
+    if (!stats || !used_warm) return skg_fail("null stats");
+    *used_warm = 0;
+    if (!S || !S->ready) return skg_fail("invalid session");
+    if (warm_iterations <= 0) return skg_fail("invalid warm iteration budget");
+    if (x0 || y0) {
+        S->P.max_iterations = warm_iterations < budget ? warm_iterations : budget;
+        rc = sk_gpu_pdlp_session_solve(S, x0, y0, x_out, y_out, stats);
+        S->P.max_iterations = budget;
+        if (rc == -1) return -1;
+        if (rc == 0 && stats->converged) { *used_warm = 1; return 0; }
+        warm_its = stats->iterations;
+        /* the cold start is the fallback, within what is left */
+        if (limit > 0.0) {
+            S->P.time_limit_seconds = limit - (skg_now() - t0);
+            if (S->P.time_limit_seconds <= 0.0) { S->P.time_limit_seconds = limit; return rc; }
+        }
+    }
+    rc = sk_gpu_pdlp_session_solve(S, NULL, NULL, x_out, y_out, stats);
+    S->P.time_limit_seconds = limit;
+    stats->iterations += warm_its;
+    stats->seconds = skg_now() - t0;
+    return rc;
+}
+
 int sk_gpu_pdlp_session_resume(sk_gpu_pdlp_session *S, double *x_out, double *y_out, sk_gpu_pdlp_stats *stats)
 {
     const double t0 = skg_now();

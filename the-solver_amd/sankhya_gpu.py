@@ -282,6 +282,9 @@ def load_gpu():
         _declare(lib, "sk_gpu_pdlp_session_destroy", None, [vp])
         _declare(lib, "sk_gpu_pdlp_session_set_limits", C.c_int, [vp, C.c_int, C.c_double, C.c_double])
         _declare(lib, "sk_gpu_pdlp_session_resume", C.c_int, [vp, c_double_p, c_double_p, C.POINTER(PdlpStats)])
+        _declare(lib, "sk_gpu_pdlp_session_solve_fallback", C.c_int,
+                 [vp, c_double_p, c_double_p, C.c_int, c_double_p, c_double_p, C.POINTER(PdlpStats),
+                  C.POINTER(C.c_int)])
         _declare(lib, "sk_gpu_gather_rate", C.c_int, [C.c_int, C.c_int, C.c_int, c_double_p])
         _declare(lib, "sk_gpu_read_mps", C.c_int, [C.c_char_p, C.POINTER(SkModel)])
         _declare(lib, "sk_gpu_model_free", None, [C.POINTER(SkModel)])
