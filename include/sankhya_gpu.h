@@ -335,7 +335,17 @@ typedef struct sk_gpu_pdlp_params {  /* src/cuda/GpuPdlp.hpp:19-67 PdlpParams */
                                 relative-KKT merit, GpuPdlp.cpp:437-451) or
                                 SK_GPU_RESTART_NDG (1, normalized duality
                                 gap evaluated in FP64)                      */
-    int    pad;
+    int    autotune;         /* 1: at session creation, time a short run of
+                                the adaptive loop for each candidate of the
+                                products' row-tile entry cap {256 .. 2048},
+                                grid cap {1024, 4096} and streaming matrix
+                                loads {off, on}, and keep the fastest.  The
+                                choice changes the summation order, so
+                                results are reproducible for a fixed
+                                choice, not across tunings.  0 (default):
+                                the fixed rule (largest cap giving 1,024
+                                tiles, 4,096 workgroups, streaming above
+                                64 MB of FP64 entries).                     */
 } sk_gpu_pdlp_params;
 
 enum {
@@ -375,6 +385,11 @@ typedef struct sk_gpu_pdlp_stats {   /* src/cuda/GpuPdlp.hpp:69-92 PdlpStats */
                                     current and the averaged point: their
                                     relative KKT (restart_rule KKT) or their
                                     normalized duality gap (NDG)            */
+    int    tile_nnz;             /* row-tile entry cap of A and A^T         */
+    int    spmv_grid;            /* workgroup cap of the products           */
+    int    stream_loads;         /* non-temporal matrix loads (A and A^T)   */
+    int    tuned;                /* the autotuner chose the three above     */
+    double tune_seconds;         /* time the autotuner took                 */
 } sk_gpu_pdlp_stats;
 
 void sk_gpu_pdlp_params_default(sk_gpu_pdlp_params *p);

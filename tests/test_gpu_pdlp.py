@@ -240,3 +240,20 @@ def test_headline_size_first_iterations_match_the_oracle():
     assert abs(st.matrix_norm - sto.matrix_norm) <= 1e-10 * sto.matrix_norm
     assert np.linalg.norm(x - xo) <= 1e-8 * (1.0 + np.linalg.norm(xo))
     assert np.linalg.norm(y - yo) <= 1e-8 * (1.0 + np.linalg.norm(yo))
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_autotuned_session_solves_and_reports_its_choice(prec):
+    """autotune = 1 times each candidate tiling on the device and keeps the
+    fastest; whatever it keeps, the solve reaches the reference simplex
+    objective, and the stats name the choice."""
+    g = golden("small_lp_105.npz")
+    model = sg.Model.generated("skw_bounded_lp", int(g["m"]), int(g["n"]), 5, 105)
+    rc, x, y, st = gpu_pdlp(model, precision=prec, eps_optimal=1e-7, autotune=1)
+    assert rc == 0 and st.converged, sg.last_error()
+    assert st.tuned == 1 and st.tile_nnz in (256, 512, 1024, 2048) and st.spmv_grid in (1024, 4096)
+    assert st.stream_loads in (0, 1) and 0.0 < st.tune_seconds < 5.0
+    ref = float(g["simplex_objective"])
+    assert abs(float(model.vec("c") @ x) - ref) <= 1e-6 * (1 + abs(ref))
+    rc, x, y, st = gpu_pdlp(model, precision=prec, eps_optimal=1e-7)
+    assert st.tuned == 0 and st.spmv_grid == 4096

@@ -397,6 +397,30 @@ int skg_plan_tiles(const int *rp, int rows, int tile_nnz, int **tile_row)
     return nt;
 }
 
+/* Replace a device CSR's row tiles: entry cap tile_nnz, grid cap grid_cap
+ * (<= SKK_SPMV_GRID), non-temporal matrix loads when stream is set.  rp is
+ * the host copy of the row offsets the matrix was uploaded with. */
+int dcsr_retile(dcsr *d, const int *rp, int tile_nnz, int grid_cap, int stream)
+{
+    int rc, nt, *tiles = NULL, *dt = NULL;
+    if (grid_cap < 1 || grid_cap > SKK_SPMV_GRID) return skg_fail("invalid grid cap %d", grid_cap);
+    nt = skg_plan_tiles(rp, d->k.rows, tile_nnz, &tiles);
+    if (nt < 0) return skg_fail("invalid tile size %d or out of host memory", tile_nnz);
+    if ((rc = skk_malloc((void **)&dt, sizeof(int) * ((size_t)nt + 1))) ||
+        (rc = skk_h2d(dt, tiles, sizeof(int) * ((size_t)nt + 1)))) {
+        free(tiles); skk_free(dt);
+        return skg_hip_fail("tile upload", rc);
+    }
+    free(tiles);
+    skk_free(d->tiles);
+    d->tiles = dt;
+    d->k.tile_row = dt;
+    d->k.ntiles = nt;
+    d->k.grid = nt < grid_cap ? nt : grid_cap;
+    d->k.stream = stream;
+    return 0;
+}
+
 void dcsr_free(dcsr *d)
 {
     skk_free(d->rp); skk_free(d->ci); skk_free(d->vd); skk_free(d->vf); skk_free(d->tiles);

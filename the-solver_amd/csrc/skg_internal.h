@@ -53,6 +53,7 @@ typedef struct dcsr {
 int  skg_tile_nnz(long long nnz);
 int  skg_plan_tiles(const int *rp, int rows, int tile_nnz, int **tile_row);
 int  dcsr_upload(dcsr *d, const hcsr *a, int with_f32);
+int  dcsr_retile(dcsr *d, const int *rp, int tile_nnz, int grid_cap, int stream);
 void dcsr_free(dcsr *d);
 
 /* Ruiz equilibration (src/lp/Scaling.cpp:11-92) and Pock-Chambolle alpha=1

@@ -150,6 +150,10 @@ typedef struct engine {
     void *graph[2];
     int graph_period[2];
     long long launches;
+    int *rp_host[3];             /* host row offsets of A, A^T, Q (autotune) */
+    int tile_nnz, grid_cap, stream, tuned;   /* tiling of the products    */
+    double tune_seconds;
+    int unfused;
 } engine;
 
 static int dalloc(engine *E, void **p, size_t bytes)
@@ -169,6 +173,7 @@ static void engine_free(engine *E)
     dcsr_free(&E->A); dcsr_free(&E->AT); dcsr_free(&E->Q);
     skk_host_free(E->h_eval);
     free(E->perm_r); free(E->perm_c); free(E->R); free(E->C);
+    for (k = 0; k < 3; ++k) free(E->rp_host[k]);
     memset(E, 0, sizeof(*E));
 }
 
@@ -328,6 +333,17 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
     }
     ATp.p[n] = q;
     if ((rc = dcsr_upload(&E->A, &Ap, E->mixed)) || (rc = dcsr_upload(&E->AT, &ATp, E->mixed))) goto done;
+    E->tile_nnz = skg_tile_nnz(Ap.nnz);
+    E->grid_cap = SKK_SPMV_GRID;
+    E->stream = E->A.k.stream;
+    E->unfused = P->unfused;
+    if (P->autotune) {
+        E->rp_host[0] = (int *)malloc(sizeof(int) * ((size_t)m + 1));
+        E->rp_host[1] = (int *)malloc(sizeof(int) * ((size_t)n + 1));
+        if (!E->rp_host[0] || !E->rp_host[1]) { rc = skg_fail("out of host memory"); goto done; }
+        memcpy(E->rp_host[0], Ap.p, sizeof(int) * ((size_t)m + 1));
+        memcpy(E->rp_host[1], ATp.p, sizeof(int) * ((size_t)n + 1));
+    }
     if (E->has_q) {
         /* Q' = C Q C in slot order on both sides */
         hcsr Qp;
@@ -343,6 +359,11 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
         }
         Qp.p[n] = q;
         rc = dcsr_upload(&E->Q, &Qp, E->mixed);
+        if (!rc && P->autotune) {
+            E->rp_host[2] = (int *)malloc(sizeof(int) * ((size_t)n + 1));
+            if (!E->rp_host[2]) rc = skg_fail("out of host memory");
+            else memcpy(E->rp_host[2], Qp.p, sizeof(int) * ((size_t)n + 1));
+        }
         hcsr_free(&Qp);
         if (rc) goto done;
     }
@@ -399,6 +420,7 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
         if (E->AT.k.grid > cap) cap = E->AT.k.grid;
         if (E->has_q && E->Q.k.grid > cap) cap = E->Q.k.grid;
         if (Q->probe_grid > cap) cap = Q->probe_grid;
+        if (P->autotune && cap < SKK_SPMV_GRID) cap = SKK_SPMV_GRID;   /* any candidate grid fits */
         Q->part_capacity = cap;
         /* with few block partials every dual-prox workgroup can afford to
            reduce them itself: the step decision then rides in that launch */
@@ -557,6 +579,70 @@ static int check_params(const sk_gpu_pdlp_params *P)
     return 0;
 }
 
+/* Row tiles of every product with entry cap `tile`, grid cap `grid` and
+ * streaming matrix loads `stream` (A and A^T; a Hessian keeps its own
+ * streaming rule), and the launch shapes that follow from them. */
+static int apply_tiling(engine *E, int tile, int grid, int stream)
+{
+    skk_pdhg *Q = &E->P;
+    if (dcsr_retile(&E->A, E->rp_host[0], tile, grid, stream) ||
+        dcsr_retile(&E->AT, E->rp_host[1], tile, grid, stream))
+        return -1;
+    Q->A = E->A.k;
+    Q->AT = E->AT.k;
+    Q->probe_grid = E->A.k.grid;
+    if (E->has_q) {
+        if (dcsr_retile(&E->Q, E->rp_host[2], tile, grid, E->Q.k.stream)) return -1;
+        Q->Q = E->Q.k;
+        {
+            const long long both = (long long)E->A.k.ntiles + E->Q.k.ntiles;
+            Q->probe_grid = both < grid ? (int)both : grid;
+        }
+    }
+    Q->fuse_decision = !E->unfused && E->AT.k.grid + Q->probe_grid <= FUSE_MAX_PARTIALS;
+    E->tile_nnz = tile;
+    E->grid_cap = grid;
+    E->stream = stream;
+    return 0;
+}
+
+static int session_start(sk_gpu_pdlp_session *S, int prec, const double *x0, const double *y0,
+                         double *omega_out);
+
+/* The autotuner: for each candidate tiling, a cold start and a timed run
+ * of 32 adaptive iterations (after 8 untimed) on the library stream; the
+ * fastest is kept. */
+static int autotune(sk_gpu_pdlp_session *S)
+{
+    static const int tiles[4] = {256, 512, 1024, 2048};
+    static const int grids[2] = {1024, SKK_SPMV_GRID};
+    engine *E = &S->E;
+    const int prec = E->mixed ? SKK_F32 : SKK_F64;
+    const double t0 = skg_now();
+    double best_ms = INFINITY, omega;
+    int best[3] = {E->tile_nnz, E->grid_cap, E->stream}, t, g, st, rc;
+    if ((rc = skk_timeline_reserve(2))) return skg_hip_fail("event timeline", rc);
+    for (t = 0; t < 4; ++t)
+        for (g = 0; g < 2; ++g)
+            for (st = 0; st < 2; ++st) {
+                double ms;
+                if (apply_tiling(E, tiles[t], grids[g], st) || session_start(S, prec, NULL, NULL, &omega)) return -1;
+                if ((rc = S->P.adaptive_step ? skk_pdhg_adaptive_run(&E->P, prec, 8) : skk_pdhg_fixed_iter(&E->P, prec)) ||
+                    (rc = skk_timeline_mark(0)) ||
+                    (rc = S->P.adaptive_step ? skk_pdhg_adaptive_run(&E->P, prec, 32) : skk_pdhg_fixed_iter(&E->P, prec)) ||
+                    (rc = skk_timeline_mark(1)) || (rc = skk_sync()))
+                    return skg_hip_fail("autotune probe", rc);
+                ms = skk_timeline_ms(0, 1);
+                if (S->P.verbosity > 0)
+                    fprintf(stderr, "autotune tile %d grid %d stream %d: %.4f ms\n", tiles[t], grids[g], st, ms);
+                if (ms > 0.0 && ms < best_ms) { best_ms = ms; best[0] = tiles[t]; best[1] = grids[g]; best[2] = st; }
+            }
+    if (apply_tiling(E, best[0], best[1], best[2])) return -1;
+    E->tuned = 1;
+    E->tune_seconds = skg_now() - t0;
+    return 0;
+}
+
 int sk_gpu_pdlp_session_create(int rows, int cols, int nnz, const int *row_offsets,
                                const int *column_indices, const double *values, const double *c,
                                const double *row_lower, const double *row_upper,
@@ -612,6 +698,7 @@ int sk_gpu_pdlp_session_create_qp(int rows, int cols, int nnz, const int *row_of
     if (!(S->norm > 0.0) || !isfinite(S->norm)) S->norm = 1.0;
     S->eta_global = 0.9 / S->norm;
     S->ready = 1;
+    if (S->P.autotune && autotune(S)) { engine_free(&S->E); free(S); return -1; }
     *out = S;
     return 0;
 }
@@ -947,6 +1034,11 @@ int sk_gpu_pdlp_session_solve(sk_gpu_pdlp_session *S, const double *x0, const do
     S->L.mu_at_restart = S->L.mu_previous = S->L.best_mu_seen = INFINITY;
     E->launches = 0;
     S->stats.matrix_norm = S->norm;
+    S->stats.tile_nnz = E->tile_nnz;
+    S->stats.spmv_grid = E->grid_cap;
+    S->stats.stream_loads = E->stream;
+    S->stats.tuned = E->tuned;
+    S->stats.tune_seconds = E->tune_seconds;
     S->L.prec = E->mixed ? SKK_F32 : SKK_F64;
     if (session_start(S, S->L.prec, x0, y0, &S->L.omega)) return -1;
     if ((rc = skk_sync())) return skg_hip_fail("setup", rc);

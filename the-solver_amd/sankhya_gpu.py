@@ -100,7 +100,7 @@ class PdlpParams(C.Structure):
                 ("ruiz_iterations", C.c_int), ("ruiz_tol", C.c_double),
                 ("use_graph", C.c_int), ("verbosity", C.c_int), ("primal_weight", C.c_double),
                 ("unfused", C.c_int), ("refine", C.c_int), ("cancel", C.POINTER(C.c_int)),
-                ("restart_rule", C.c_int), ("pad", C.c_int)]
+                ("restart_rule", C.c_int), ("autotune", C.c_int)]
 
 
 class PdlpStats(C.Structure):
@@ -115,7 +115,9 @@ class PdlpStats(C.Structure):
                 ("kernel_launches", C.c_longlong), ("loop_seconds", C.c_double),
                 ("setup_seconds", C.c_double), ("primal_weight", C.c_double),
                 ("reanchors", C.c_int), ("iterations_to_merit", C.c_int * 4),
-                ("seconds_to_merit", C.c_double * 4), ("restart_merit", C.c_double * 2)]
+                ("seconds_to_merit", C.c_double * 4), ("restart_merit", C.c_double * 2),
+                ("tile_nnz", C.c_int), ("spmv_grid", C.c_int), ("stream_loads", C.c_int), ("tuned", C.c_int),
+                ("tune_seconds", C.c_double)]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
