@@ -183,6 +183,20 @@ def _solve_budget(S, iters, eps=1e-12):
     return wall, st
 
 
+def _tiling_params(args):
+    """Graph replay and the products' tiling of the headline sessions: the
+    autotuner by default; --tiling T,G,S fixes tile cap, grid cap and
+    streaming (0 rule, 1 off, 2 on) -- profiles use it to run the tiling the
+    tuner picked without timing candidates inside the profiled process."""
+    kw = dict(use_graph=0 if args.no_graph else 1)
+    if args.tiling:
+        t, g, st = (int(v) for v in args.tiling.split(","))
+        kw.update(autotune=0, tile_nnz=t, spmv_grid=g, stream_loads=st)
+    else:
+        kw.update(autotune=0 if args.no_tune else 1)
+    return kw
+
+
 def workload_lp1(args):
     """Headline: the 1M x 2M power-law LP.  `value` is the mixed engine's
     whole-solve rate: --steps iterations from a cold start through the real
@@ -201,14 +215,16 @@ def workload_lp1(args):
     results = {}
     for name, prec in (("mixed", sg.PREC_MIXED), ("fp64", sg.PREC_FP64)):
         t = time.perf_counter()
-        S = Session(lib, model, arrays, precision=prec, use_graph=1)
+        S = Session(lib, model, arrays, precision=prec, **_tiling_params(args))
         setup = time.perf_counter() - t
         _solve_budget(S, max(args.warmup, 1))                 # graph capture, caches
         wall, st = _solve_budget(S, args.steps)
         r = dict(setup_s=setup, iters_per_s=args.steps / wall, ms_per_iter=1e3 * wall / args.steps,
                  budget=dict(iterations=st.iterations, fp32_iterations=st.fp32_iterations,
                              restarts=st.restarts, reanchors=st.reanchors, host_syncs=st.host_syncs,
-                             kernel_launches=st.kernel_launches))
+                             kernel_launches=st.kernel_launches),
+                 tiling=dict(tuned=st.tuned, tile_nnz=st.tile_nnz, spmv_grid=st.spmv_grid,
+                             stream_loads=st.stream_loads, tune_s=round(st.tune_seconds, 3)))
         # the inner loop alone: raw iterations, no evaluation or restart
         S.iterate(args.warmup)
         t = time.perf_counter()
@@ -241,6 +257,24 @@ def workload_lp1(args):
                 f"kkt={kkt[:3]} to-merit {d['seconds_to_merit']}")
         S.close()
         results[name] = r
+
+    if not args.skip_solve and not args.skip_ndg:
+        # the same mixed engine with normalized-duality-gap restarts
+        S = Session(lib, model, arrays, precision=sg.PREC_MIXED, restart_rule=sg.RESTART_NDG, **_tiling_params(args))
+        S.set_limits(500000, args.solve_time_limit, 1e-6)
+        rc, xs, ys, st = S.solve()
+        d = st.as_dict()
+        kkt = np.zeros(5)
+        lib.sk_gpu_relative_kkt(m, n, nnz, sg.iptr(rp), sg.iptr(cj), sg.dptr(v), sg.dptr(vec["c"]),
+                                sg.dptr(vec["rlow"]), sg.dptr(vec["rupp"]), sg.dptr(vec["clow"]),
+                                sg.dptr(vec["cupp"]), sg.dptr(xs), sg.dptr(ys), sg.dptr(kkt))
+        results["mixed_ndg"] = dict(converged=bool(st.converged), iterations=st.iterations, loop_s=st.loop_seconds,
+                                    time_to_merit_s=dict(zip(("1e-3", "1e-4", "1e-5", "1e-6"),
+                                                             [v_ if v_ > 0 else None for v_ in d["seconds_to_merit"]])),
+                                    original_kkt=kkt[:3].tolist(), restarts=st.restarts)
+        log(f"mixed NDG solve: conv={st.converged} it={st.iterations} loop={st.loop_seconds:.2f}s kkt={kkt[:3]} "
+            f"to-merit {d['seconds_to_merit']}")
+        S.close()
 
     mix, f64 = results["mixed"], results["fp64"]
     k1, k2, k3 = kernel_bytes(m, n, nnz, 4, 4, 1.0)
@@ -281,6 +315,7 @@ def workload_lp1(args):
         "inner_loop_iters_per_s": {"mixed": round(mix["inner_iters_per_s"], 1),
                                    "fp64": round(f64["inner_iters_per_s"], 1)},
         "budget_solve": {"mixed": mix["budget"], "fp64": f64["budget"]},
+        "tiling": {"mixed": mix["tiling"], "fp64": f64["tiling"]},
         "setup_s": {"mixed": round(mix["setup_s"], 3), "fp64": round(f64["setup_s"], 3)},
         "spmv_gbs": {k: round(val, 1) for k, val in spmv_gbs.items()},
         "gather_roofline": gather,
@@ -299,6 +334,8 @@ def workload_lp1(args):
             a, b = mix["time_to_merit_s"][tol], f64["time_to_merit_s"][tol]
             if a and b:
                 line[f"speedup_time_to_kkt_{tol}_vs_fp64"] = round(b / a, 3)
+        if "mixed_ndg" in results:
+            line["solve"]["mixed_ndg_restarts"] = results["mixed_ndg"]
         # error of the mixed result against the FP64 engine's, both in the
         # original space, FP64 on the device
         line["mixed_vs_fp64"] = {"objective_rel_diff": abs(mix["objective"] - f64["objective"]) /
@@ -603,7 +640,10 @@ def main():
     ap.add_argument("--quick", action="store_true", help="small instance (smoke)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch restart windows directly instead of replaying hipGraphs (profilers)")
-    ap.add_argument("--solve-time-limit", type=float, default=60.0)
+    ap.add_argument("--solve-time-limit", type=float, default=45.0)
+    ap.add_argument("--skip-ndg", action="store_true", help="lp1: no NDG-restart solve")
+    ap.add_argument("--no-tune", action="store_true", help="lp1: fixed tiling instead of the autotuner")
+    ap.add_argument("--tiling", default="", help="lp1: fix the tiling as TILE,GRID,STREAM (no autotuner)")
     ap.add_argument("--train", type=int, default=16000, help="warm4: training instances")
     ap.add_argument("--test", type=int, default=4000, help="warm4: held-out instances")
     ap.add_argument("--compare", type=int, default=100, help="warm4: held-out instances solved cold and warm")

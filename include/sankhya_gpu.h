@@ -345,7 +345,14 @@ typedef struct sk_gpu_pdlp_params {  /* src/cuda/GpuPdlp.hpp:19-67 PdlpParams */
                                 choice, not across tunings.  0 (default):
                                 the fixed rule (largest cap giving 1,024
                                 tiles, 4,096 workgroups, streaming above
-                                64 MB of FP64 entries).                     */
+                                64 MB of FP64 entries), unless the three
+                                fields below fix a tiling.                  */
+    int    tile_nnz;         /* autotune 0: row-tile entry cap, 256..2048
+                                (0: the rule)                               */
+    int    spmv_grid;        /* autotune 0: workgroup cap of the products,
+                                1..4096 (0: 4096)                           */
+    int    stream_loads;     /* autotune 0: 0 the rule, 1 off, 2 on         */
+    int    pad;
 } sk_gpu_pdlp_params;
 
 enum {

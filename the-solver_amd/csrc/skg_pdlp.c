@@ -246,6 +246,33 @@ done:
     return rc;
 }
 
+/* Row tiles of every product with entry cap `tile`, grid cap `grid` and
+ * streaming matrix loads `stream` (A and A^T; a Hessian keeps its own
+ * streaming rule), and the launch shapes that follow from them. */
+static int apply_tiling(engine *E, int tile, int grid, int stream)
+{
+    skk_pdhg *Q = &E->P;
+    if (dcsr_retile(&E->A, E->rp_host[0], tile, grid, stream) ||
+        dcsr_retile(&E->AT, E->rp_host[1], tile, grid, stream))
+        return -1;
+    Q->A = E->A.k;
+    Q->AT = E->AT.k;
+    Q->probe_grid = E->A.k.grid;
+    if (E->has_q) {
+        if (dcsr_retile(&E->Q, E->rp_host[2], tile, grid, E->Q.k.stream)) return -1;
+        Q->Q = E->Q.k;
+        {
+            const long long both = (long long)E->A.k.ntiles + E->Q.k.ntiles;
+            Q->probe_grid = both < grid ? (int)both : grid;
+        }
+    }
+    Q->fuse_decision = !E->unfused && E->AT.k.grid + Q->probe_grid <= FUSE_MAX_PARTIALS;
+    E->tile_nnz = tile;
+    E->grid_cap = grid;
+    E->stream = stream;
+    return 0;
+}
+
 /* Build the scaled, permuted, binned problem on the device. */
 static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, const double *v,
                         const int *qrp, const int *qci, const double *qv,
@@ -337,7 +364,7 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
     E->grid_cap = SKK_SPMV_GRID;
     E->stream = E->A.k.stream;
     E->unfused = P->unfused;
-    if (P->autotune) {
+    if (P->autotune || P->tile_nnz || P->spmv_grid || P->stream_loads) {
         E->rp_host[0] = (int *)malloc(sizeof(int) * ((size_t)m + 1));
         E->rp_host[1] = (int *)malloc(sizeof(int) * ((size_t)n + 1));
         if (!E->rp_host[0] || !E->rp_host[1]) { rc = skg_fail("out of host memory"); goto done; }
@@ -359,7 +386,7 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
         }
         Qp.p[n] = q;
         rc = dcsr_upload(&E->Q, &Qp, E->mixed);
-        if (!rc && P->autotune) {
+        if (!rc && E->rp_host[0]) {
             E->rp_host[2] = (int *)malloc(sizeof(int) * ((size_t)n + 1));
             if (!E->rp_host[2]) rc = skg_fail("out of host memory");
             else memcpy(E->rp_host[2], Qp.p, sizeof(int) * ((size_t)n + 1));
@@ -420,7 +447,7 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
         if (E->AT.k.grid > cap) cap = E->AT.k.grid;
         if (E->has_q && E->Q.k.grid > cap) cap = E->Q.k.grid;
         if (Q->probe_grid > cap) cap = Q->probe_grid;
-        if (P->autotune && cap < SKK_SPMV_GRID) cap = SKK_SPMV_GRID;   /* any candidate grid fits */
+        if (E->rp_host[0] && cap < SKK_SPMV_GRID) cap = SKK_SPMV_GRID;   /* any later tiling fits */
         Q->part_capacity = cap;
         /* with few block partials every dual-prox workgroup can afford to
            reduce them itself: the step decision then rides in that launch */
@@ -445,6 +472,19 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
             rc = skg_hip_fail("hipHostMalloc", rc); goto done;
         }
         E->h_state = (skk_state *)(E->h_eval + 24);
+    }
+    if (!P->autotune && E->rp_host[0]) {
+        /* a tiling fixed by the caller (a tuned choice reused, say) */
+        const int tile = P->tile_nnz ? P->tile_nnz : E->tile_nnz;
+        const int grid = P->spmv_grid ? P->spmv_grid : SKK_SPMV_GRID;
+        const int stream = P->stream_loads ? P->stream_loads == 2 : E->stream;
+        if (tile < SKK_TILE_MIN_NNZ || tile > SKK_TILE_NNZ || grid < 1 || grid > SKK_SPMV_GRID ||
+            P->stream_loads < 0 || P->stream_loads > 2) {
+            rc = skg_fail("invalid tiling: tile_nnz %d, spmv_grid %d, stream_loads %d", P->tile_nnz,
+                          P->spmv_grid, P->stream_loads);
+            goto done;
+        }
+        if ((rc = apply_tiling(E, tile, grid, stream))) goto done;
     }
     rc = 0;
 done:
@@ -576,33 +616,6 @@ static int check_params(const sk_gpu_pdlp_params *P)
         (P->precision != SK_GPU_PREC_FP64 && P->precision != SK_GPU_PREC_MIXED) ||
         (P->restart_rule != SK_GPU_RESTART_KKT && P->restart_rule != SK_GPU_RESTART_NDG))
         return skg_fail("invalid PDLP parameters");
-    return 0;
-}
-
-/* Row tiles of every product with entry cap `tile`, grid cap `grid` and
- * streaming matrix loads `stream` (A and A^T; a Hessian keeps its own
- * streaming rule), and the launch shapes that follow from them. */
-static int apply_tiling(engine *E, int tile, int grid, int stream)
-{
-    skk_pdhg *Q = &E->P;
-    if (dcsr_retile(&E->A, E->rp_host[0], tile, grid, stream) ||
-        dcsr_retile(&E->AT, E->rp_host[1], tile, grid, stream))
-        return -1;
-    Q->A = E->A.k;
-    Q->AT = E->AT.k;
-    Q->probe_grid = E->A.k.grid;
-    if (E->has_q) {
-        if (dcsr_retile(&E->Q, E->rp_host[2], tile, grid, E->Q.k.stream)) return -1;
-        Q->Q = E->Q.k;
-        {
-            const long long both = (long long)E->A.k.ntiles + E->Q.k.ntiles;
-            Q->probe_grid = both < grid ? (int)both : grid;
-        }
-    }
-    Q->fuse_decision = !E->unfused && E->AT.k.grid + Q->probe_grid <= FUSE_MAX_PARTIALS;
-    E->tile_nnz = tile;
-    E->grid_cap = grid;
-    E->stream = stream;
     return 0;
 }
 

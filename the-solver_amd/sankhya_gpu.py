@@ -100,7 +100,8 @@ class PdlpParams(C.Structure):
                 ("ruiz_iterations", C.c_int), ("ruiz_tol", C.c_double),
                 ("use_graph", C.c_int), ("verbosity", C.c_int), ("primal_weight", C.c_double),
                 ("unfused", C.c_int), ("refine", C.c_int), ("cancel", C.POINTER(C.c_int)),
-                ("restart_rule", C.c_int), ("autotune", C.c_int)]
+                ("restart_rule", C.c_int), ("autotune", C.c_int), ("tile_nnz", C.c_int),
+                ("spmv_grid", C.c_int), ("stream_loads", C.c_int), ("pad", C.c_int)]
 
 
 class PdlpStats(C.Structure):
