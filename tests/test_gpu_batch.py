@@ -68,17 +68,23 @@ def test_iteration_limit_reported():
     assert all(r.iterations == 128 and r.status == 1 for r in res)
 
 
-def test_fp64_tail_and_fp64_mode_agree():
-    """Mixed precision hands the instances that reach tail_switch to the
-    FP64 launch; FP64 iterates throughout reach the same optimum."""
+@pytest.mark.parametrize("refine", [0, 1])
+def test_fp64_tail_and_fp64_mode_agree(refine):
+    """Plain FP32 iterates (refine 0) hand the instances that reach
+    tail_switch to the FP64 launch; anchored FP32 iterates (refine 1, the
+    default) solve every instance to 1e-8 with no FP64 tail.  FP64 iterates
+    throughout reach the same optimum."""
     models = _models(16, 20, 100, 21)
     rc_m, xm, ym, rm, sm = sg.batch_solve(models, eps_optimal=1e-8, max_iterations=200000,
-                                          precision=sg.PREC_MIXED)
+                                          precision=sg.PREC_MIXED, refine=refine)
     rc_d, xd, yd, rd, sd = sg.batch_solve(models, eps_optimal=1e-8, max_iterations=200000,
                                           precision=sg.PREC_FP64)
     assert rc_m == 0 and rc_d == 0, sg.last_error()
     assert sm.converged == len(models) and sd.converged == len(models)
-    assert sm.handoffs > 0 and sm.fp32_iterations > 0
+    if refine:
+        assert sm.handoffs == 0 and sm.fp32_iterations == sm.total_iterations
+    else:
+        assert sm.handoffs > 0 and sm.fp32_iterations > 0
     assert sd.fp32_iterations == 0 and sd.handoffs == 0
     for k, md in enumerate(models):
         c = md.vec("c")

@@ -16,7 +16,9 @@ typedef struct skb_params {
     double tail_switch;    /* FP32 -> FP64 handoff merit (mixed precision)  */
     int max_iterations, restart_period, power_iterations;
     int mixed;             /* 1: FP32 phase then FP64 tail; 0: FP64 only    */
-    int tail_stall_windows, pad;
+    int tail_stall_windows;
+    int anchored;          /* FP32 phase holds corrections to an FP64 anchor
+                              kept in x_out / y_out (no FP64 tail)          */
 } skb_params;
 
 /* One instance: offsets into the packed arrays. */

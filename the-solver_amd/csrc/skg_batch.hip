@@ -127,18 +127,27 @@ __device__ Kkt make_kkt(const double *s, double cost_norm, double bound_norm)
 
 // Eight residual sums of PdlpKernels.cu:101-209 in FP64 for the point
 // (x * xs, y * ys) -- xs = ys = 1 for the iterate, 1/terms for the average.
+// With an anchor (hx, hy: FP64, global) the vectors are corrections: the
+// point is (hx + x * xs, hy + y * ys), and the restart point is the anchor.
 template <int BT, typename T>
 __device__ void residual_sums(const Inst &I, const T *x, const T *y, const T *xr, const T *yr,
-                              double xs, double ys, double *sh, double (&out)[8])
+                              double xs, double ys, const double *hx, const double *hy, double *sh,
+                              double (&out)[8])
 {
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int j = threadIdx.x; j < I.n; j += BT) {
         double aty = 0.0;
         const int p1 = I.trp[j + 1];
+        if (hy) {
 #pragma unroll SKB_UNROLL
-        for (int p = I.trp[j]; p < p1; ++p) aty += I.tvd[p] * (ys * (double)y[I.tci[p]]);
-        const double xj = xs * (double)x[j], lo = I.ld[j], hi = I.ud[j], r = I.cd[j] + aty;
-        const double dx = xj - (double)xr[j];
+            for (int p = I.trp[j]; p < p1; ++p) aty += I.tvd[p] * (hy[I.tci[p]] + ys * (double)y[I.tci[p]]);
+        } else {
+#pragma unroll SKB_UNROLL
+            for (int p = I.trp[j]; p < p1; ++p) aty += I.tvd[p] * (ys * (double)y[I.tci[p]]);
+        }
+        const double xd = xs * (double)x[j];
+        const double xj = hx ? hx[j] + xd : xd, lo = I.ld[j], hi = I.ud[j], r = I.cd[j] + aty;
+        const double dx = hx ? xd : xj - (double)xr[j];
         acc[2] += I.cd[j] * xj;
         acc[4] += xj * xj;
         acc[6] += dx * dx;
@@ -154,11 +163,17 @@ __device__ void residual_sums(const Inst &I, const T *x, const T *y, const T *xr
     for (int i = threadIdx.x; i < I.m; i += BT) {
         double ax = 0.0;
         const int p1 = I.arp[i + 1];
+        if (hx) {
 #pragma unroll SKB_UNROLL
-        for (int p = I.arp[i]; p < p1; ++p) ax += I.avd[p] * (xs * (double)x[I.aci[p]]);
+            for (int p = I.arp[i]; p < p1; ++p) ax += I.avd[p] * (hx[I.aci[p]] + xs * (double)x[I.aci[p]]);
+        } else {
+#pragma unroll SKB_UNROLL
+            for (int p = I.arp[i]; p < p1; ++p) ax += I.avd[p] * (xs * (double)x[I.aci[p]]);
+        }
         const double rl = I.rld[i], ru = I.rud[i];
         const double pr = ax - fmin(fmax(ax, rl), ru);
-        const double yi = ys * (double)y[i], dy = yi - (double)yr[i];
+        const double yd = ys * (double)y[i];
+        const double yi = hy ? hy[i] + yd : yd, dy = hy ? yd : yi - (double)yr[i];
         acc[0] += pr * pr;
         acc[5] += yi * yi;
         acc[7] += dy * dy;
@@ -222,6 +237,13 @@ __device__ void solve_instance(const skb_batch &B, int k, T *lds, double *sh)
     double anorm, eta, omega = 1.0, updates = 0.0;
     int iteration = 0, restarts = 0, fp32_iterations = 0;
     const skb_inst &d = B.inst[k];
+    // Anchored FP32 phase: the iterates are corrections to an FP64 anchor
+    // kept in the instance's output vectors, which absorbs the restart point
+    // at every restart; xr is then always zero and yr holds the anchor's y
+    // in FP32 as the dual prox centre.
+    const bool anch = kFp32 && P.anchored;
+    double *hx = anch ? B.x_out + d.col : nullptr;
+    double *hy = anch ? B.y_out + d.row : nullptr;
     if (resume) {
         anorm = o.matrix_norm; eta = o.final_step; omega = o.omega; updates = o.updates;
         iteration = o.iterations; restarts = o.restarts; fp32_iterations = o.fp32_iterations;
@@ -278,6 +300,10 @@ __device__ void solve_instance(const skb_batch &B, int k, T *lds, double *sh)
         for (int i = threadIdx.x; i < m; i += BT) {
             Y0[i] = 0; Y1[i] = 0; AX0[i] = 0; AX1[i] = 0; AX2[i] = 0; ysum[i] = 0; yr[i] = 0;
         }
+        if (anch) {
+            for (int j = threadIdx.x; j < n; j += BT) hx[j] = 0.0;
+            for (int i = threadIdx.x; i < m; i += BT) hy[i] = 0.0;
+        }
     }
     __syncthreads();
     T *x = X0, *xc = X1, *y = Y0, *yc = Y1, *ax = AX0, *axp = AX1, *axc = AX2;
@@ -307,7 +333,9 @@ __device__ void solve_instance(const skb_batch &B, int k, T *lds, double *sh)
                 const T yi = y[i];
                 if (pending) ysum[i] += yi;
                 const T v = yi + sigma * ((T)2 * ax[i] - axp[i]);
-                yc[i] = v - sigma * clampv<T>(v / sigma, rll[i], rul[i]);
+                // anchored: the prox written for the correction (centre -yr)
+                yc[i] = anch ? fmin(fmax(-yr[i], v - sigma * rul[i]), v - sigma * rll[i])
+                             : v - sigma * clampv<T>(v / sigma, rll[i], rul[i]);
             }
             __syncthreads();
             double acc[3] = {0.0, 0.0, 0.0};
@@ -363,10 +391,10 @@ __device__ void solve_instance(const skb_batch &B, int k, T *lds, double *sh)
         if (kFp32) fp32_iterations += period;
         // FP64 evaluation of the iterate and of the running average
         double sc[8], sa[8];
-        residual_sums<BT, T>(I, x, y, xr, yr, 1.0, 1.0, sh, sc);
+        residual_sums<BT, T>(I, x, y, xr, yr, 1.0, 1.0, hx, hy, sh, sc);
         double terms = accepted - accepted_at_restart;
         if (terms < 1.0) terms = 1.0;
-        residual_sums<BT, T>(I, xsum, ysum, xr, yr, 1.0 / terms, 1.0 / terms, sh, sa);
+        residual_sums<BT, T>(I, xsum, ysum, xr, yr, 1.0 / terms, 1.0 / terms, hx, hy, sh, sa);
         const Kkt cur = make_kkt(sc, I.cost_norm, I.bound_norm);
         const Kkt avg = make_kkt(sa, I.cost_norm, I.bound_norm);
         const bool avg_better = avg.worst < cur.worst;
@@ -374,7 +402,7 @@ __device__ void solve_instance(const skb_batch &B, int k, T *lds, double *sh)
         final_avg = avg_better;
         if (best.primal <= P.eps && best.dual <= P.eps && best.gap <= P.eps) { converged = 1; break; }
         const double mu = best.worst;
-        if (kFp32 && P.mixed) {
+        if (kFp32 && P.mixed && !anch) {
             // hand off to FP64 iterates once the merit reaches tail_switch,
             // or when within 100x of it no window has improved the best
             // merit by 10% for tail_stall_windows windows
@@ -398,11 +426,45 @@ __device__ void solve_instance(const skb_batch &B, int k, T *lds, double *sh)
             for (int i = threadIdx.x; i < m; i += BT) y[i] = ysum[i] * inv;
             __syncthreads();
         }
-        spmv_rows<BT, T>(m, I.arp, I.aci, av, x, ax);
-        __syncthreads();
-        for (int i = threadIdx.x; i < m; i += BT) { axp[i] = ax[i]; ysum[i] = 0; yr[i] = y[i]; }
-        for (int j = threadIdx.x; j < n; j += BT) { xsum[j] = 0; xr[j] = x[j]; }
-        __syncthreads();
+        if (anch) {
+            // the restart point becomes the anchor (x kept inside [l, u]);
+            // the corrections return to zero and the correction problem's
+            // FP32 data are formed from FP64 products at the new anchor
+            for (int j = threadIdx.x; j < n; j += BT) {
+                hx[j] = fmin(fmax(hx[j] + (double)x[j], I.ld[j]), I.ud[j]);
+                x[j] = 0; xsum[j] = 0;
+            }
+            for (int i = threadIdx.x; i < m; i += BT) {
+                hy[i] += (double)y[i];
+                y[i] = 0; ysum[i] = 0; ax[i] = 0; axp[i] = 0;
+            }
+            __syncthreads();
+            for (int j = threadIdx.x; j < n; j += BT) {
+                double g = I.cd[j];
+                const int p1 = I.trp[j + 1];
+#pragma unroll SKB_UNROLL
+                for (int p = I.trp[j]; p < p1; ++p) g += I.tvd[p] * hy[I.tci[p]];
+                cl[j] = (T)g;
+                ll[j] = __double2float_ru(I.ld[j] - hx[j]);
+                ul[j] = __double2float_rd(I.ud[j] - hx[j]);
+            }
+            for (int i = threadIdx.x; i < m; i += BT) {
+                double a = 0.0;
+                const int p1 = I.arp[i + 1];
+#pragma unroll SKB_UNROLL
+                for (int p = I.arp[i]; p < p1; ++p) a += I.avd[p] * hx[I.aci[p]];
+                rll[i] = (T)(I.rld[i] - a);
+                rul[i] = (T)(I.rud[i] - a);
+                yr[i] = (T)hy[i];
+            }
+            __syncthreads();
+        } else {
+            spmv_rows<BT, T>(m, I.arp, I.aci, av, x, ax);
+            __syncthreads();
+            for (int i = threadIdx.x; i < m; i += BT) { axp[i] = ax[i]; ysum[i] = 0; yr[i] = y[i]; }
+            for (int j = threadIdx.x; j < n; j += BT) { xsum[j] = 0; xr[j] = x[j]; }
+            __syncthreads();
+        }
         accepted_at_restart = accepted;
         since_restart = 0;
         mu_at_restart = mu;
@@ -416,8 +478,14 @@ __device__ void solve_instance(const skb_batch &B, int k, T *lds, double *sh)
     }
     // scaled-space point in FP64 (the host unscales; the tail resumes here)
     const double inv = 1.0 / fmax(1.0, accepted - accepted_at_restart);
-    for (int j = threadIdx.x; j < n; j += BT) B.x_out[d.col + j] = final_avg ? (double)xsum[j] * inv : (double)x[j];
-    for (int i = threadIdx.x; i < m; i += BT) B.y_out[d.row + i] = final_avg ? (double)ysum[i] * inv : (double)y[i];
+    if (anch) {
+        for (int j = threadIdx.x; j < n; j += BT)
+            hx[j] = fmin(fmax(hx[j] + (final_avg ? (double)xsum[j] * inv : (double)x[j]), I.ld[j]), I.ud[j]);
+        for (int i = threadIdx.x; i < m; i += BT) hy[i] += final_avg ? (double)ysum[i] * inv : (double)y[i];
+    } else {
+        for (int j = threadIdx.x; j < n; j += BT) B.x_out[d.col + j] = final_avg ? (double)xsum[j] * inv : (double)x[j];
+        for (int i = threadIdx.x; i < m; i += BT) B.y_out[d.row + i] = final_avg ? (double)ysum[i] * inv : (double)y[i];
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         o.iterations = iteration;

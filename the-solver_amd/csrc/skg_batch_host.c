@@ -311,10 +311,14 @@ int sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_opt
     B.params.mixed = mixed;
     B.params.tail_switch = o.pdlp.tail_switch;
     B.params.tail_stall_windows = o.pdlp.tail_stall_windows;
+    B.params.anchored = mixed && o.pdlp.refine;
     memset(&ph, 0, sizeof(ph));
     ph.run32 = mixed;
     ph.run64 = 1;
-    if (mixed && (long long)lds64 > (long long)skb_max_lds()) {
+    if (B.params.anchored) {
+        /* anchored FP32 iterates: no FP64 tail */
+        ph.run64 = 0;
+    } else if (mixed && (long long)lds64 > (long long)skb_max_lds()) {
         /* FP64 iterates of the largest instance do not fit in LDS: FP32 only */
         ph.run64 = 0;
         B.params.tail_switch = -1.0;
