@@ -562,7 +562,7 @@ def workload_warm4(args):
         return c0 + P @ theta, b0 - half, b0 + half
 
     arrays = _arrays(model)
-    eps_train = 1e-4
+    eps_train = args.train_eps
     S = _session_qp(lib, model, arrays, None, precision=sg.PREC_MIXED, use_graph=1, eps_optimal=eps_train)
     Y = np.zeros((args.train, n + m), np.float32)
     t0 = time.perf_counter()
@@ -601,11 +601,17 @@ def workload_warm4(args):
         rc, xc, yc, stc = S.solve()
         x0 = np.clip(Yhat[k, :n].astype(np.float64), l, u)
         y0 = Yhat[k, n:].astype(np.float64)
-        rc, xw, yw, stw = S.solve(x0, y0)
-        if not stw.converged and stc.converged:
-            fallbacks += 1               # the cold start is the fallback
+        # the learned start with the cold start as its fallback, given the
+        # cold solve's iteration count
+        xw, yw, stw = np.zeros(n), np.zeros(m), sg.PdlpStats()
+        used = C.c_int(0)
+        if lib.sk_gpu_pdlp_session_solve_fallback(S.h, sg.dptr(x0), sg.dptr(y0), max(stc.iterations, 64),
+                                                  sg.dptr(xw), sg.dptr(yw), C.byref(stw), C.byref(used)) != 0:
+            raise RuntimeError(sg.last_error())
+        if not used.value:
+            fallbacks += 1
         cold_it.append(stc.iterations); warm_it.append(stw.iterations)
-        cold_s.append(stc.loop_seconds); warm_s.append(stw.loop_seconds)
+        cold_s.append(stc.loop_seconds); warm_s.append(stw.seconds)
         conv_c += stc.converged; conv_w += stw.converged
         sol = np.concatenate([xc, yc])
         errs.append(float(np.linalg.norm(Yhat[k] - sol) / max(np.linalg.norm(sol), 1e-12)))
@@ -615,6 +621,7 @@ def workload_warm4(args):
              "training_data": {"seconds": gen_s, "eps": eps_train, "median_iterations": float(np.median(gen_iters))},
              "mlp": {"params": net.n_params, "epochs": args.epochs, "train_s": train_s,
                      "loss_first": float(hist[0]), "loss_last": float(hist[-1]), "inference_s_all_test": infer_s,
+                     "loss_curve_every_10_epochs": [round(float(v), 4) for v in hist[::10]],
                      "median_rel_error_vs_cold_solution": float(np.median(errs))},
              "cold": {"converged": conv_c, "median_iterations": float(np.median(cold_it)),
                       "mean_seconds": float(np.mean(cold_s))},
@@ -644,10 +651,11 @@ def main():
     ap.add_argument("--skip-ndg", action="store_true", help="lp1: no NDG-restart solve")
     ap.add_argument("--no-tune", action="store_true", help="lp1: fixed tiling instead of the autotuner")
     ap.add_argument("--tiling", default="", help="lp1: fix the tiling as TILE,GRID,STREAM (no autotuner)")
-    ap.add_argument("--train", type=int, default=16000, help="warm4: training instances")
-    ap.add_argument("--test", type=int, default=4000, help="warm4: held-out instances")
-    ap.add_argument("--compare", type=int, default=100, help="warm4: held-out instances solved cold and warm")
-    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--train", type=int, default=1000, help="warm4: training instances")
+    ap.add_argument("--test", type=int, default=100, help="warm4: held-out instances")
+    ap.add_argument("--compare", type=int, default=20, help="warm4: held-out instances solved cold and warm")
+    ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--train-eps", type=float, default=1e-6, help="warm4: tolerance of the training solves")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="copy this many MiB device-to-device 3 times first (counter calibration)")
     args = ap.parse_args()
