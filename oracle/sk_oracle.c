@@ -20,7 +20,14 @@
  *     tests/lp/test_scaling.cpp (row/column inf-norms within tolerance,
  *     zero matrix -> identity) and the closed form on 2x2 matrices;
  *   - orc_power_norm by diagonal and rank-one matrices (closed form) and a
- *     dense SVD on small seeds.
+ *     dense SVD on small seeds;
+ *   - orc_pdlp_solve_qp / orc_first_order_qp by: an all-zero Hessian, which
+ *     must give the LP path's iterates bit for bit (every Q term vanishes);
+ *     closed forms (a separable box QP, x = clamp(-c/d); an equality-
+ *     constrained two-variable QP solved by its Lagrange system); and the
+ *     reference C core's own first-order QP solve (tests/golden/qp_small.npz,
+ *     objective within 1e-7 relative, and the same point within the
+ *     solve tolerance).
  */
 #include "sk_oracle.h"
 
@@ -845,6 +852,29 @@ static kkt_t residuals(int n, int m, const double *x, const double *y, const dou
     return k;
 }
 
+/* residuals() of a QP: the dual residual and the dual terms use c + Qx, and
+ * both objectives carry x'Qx/2 (primal c'x + x'Qx/2, dual ... - x'Qx/2);
+ * qx = Q x (NULL: an LP, residuals() itself).  `cq` is n doubles of
+ * scratch. */
+static kkt_t residuals_q(int n, int m, const double *x, const double *y, const double *ax,
+                         const double *aty, const double *qx, const double *c, const double *l,
+                         const double *u, const double *rl, const double *ru, const double *xr,
+                         const double *yr, double cost_norm, double bound_norm, double *cq)
+{
+    kkt_t k;
+    double xqx = 0.0;
+    int j;
+    if (!qx) return residuals(n, m, x, y, ax, aty, c, l, u, rl, ru, xr, yr, cost_norm, bound_norm);
+    for (j = 0; j < n; ++j) { cq[j] = c[j] + qx[j]; xqx += x[j] * qx[j]; }
+    k = residuals(n, m, x, y, ax, aty, cq, l, u, rl, ru, xr, yr, cost_norm, bound_norm);
+    k.pobj -= 0.5 * xqx;
+    k.dobj -= 0.5 * xqx;
+    k.gap = fabs(k.pobj - k.dobj) / (1.0 + fabs(k.pobj) + fabs(k.dobj));
+    if (!isfinite(k.primal) || !isfinite(k.dual) || !isfinite(k.gap))
+        k.primal = k.dual = k.gap = INFINITY;
+    return k;
+}
+
 static double finite_norm(const double *v, int n)   /* GpuPdlp.cpp:84-90 */
 {
     double acc = 0.0;
@@ -929,10 +959,10 @@ double orc_ndg(int n, int m, const double *x, const double *y, const double *ax,
 
 /* GpuPdlp::solve, src/cuda/GpuPdlp.cpp:252-534, with the kernels of
  * src/cuda/PdlpKernels.cu:71-97 (fixed step) and 360-518 (adaptive step). */
-int orc_pdlp_solve(const orc_csr *A, const double *c, const double *l, const double *u,
-                   const double *rl, const double *ru, const sk_gpu_pdlp_params *P,
-                   const double *x0, const double *y0, double *x_out, double *y_out,
-                   sk_gpu_pdlp_stats *st)
+static int pdlp_solve(const orc_csr *A, const orc_csr *Q, const double *c, const double *l,
+                      const double *u, const double *rl, const double *ru,
+                      const sk_gpu_pdlp_params *P, const double *x0, const double *y0,
+                      double *x_out, double *y_out, sk_gpu_pdlp_stats *st)
 {
     const int n = A->cols, m = A->rows;
     const double t0 = wall_seconds();
@@ -942,13 +972,18 @@ int orc_pdlp_solve(const orc_csr *A, const double *c, const double *l, const dou
     double eta, accepted_total = 0.0, accepted_at_restart = 0.0, accept_count = 0.0, updates = 0.0;
     double mu_at_restart = INFINITY, mu_previous = INFINITY;
     int iteration = 0, windows_since_restart = 0, its_since_restart = 0, i, j, k;
+    double *qx = NULL, *qxc = NULL, *qxa = NULL, *cq = NULL;
     kkt_t best = {1, 1, 1, 0, 0, 0, 0};
     memset(st, 0, sizeof(*st));
+    /* the Hessian enters the adaptive-step engine only, and not the NDG
+       merit (its trust-region maximiser is not a projection for a QP) */
+    if (Q && (!P->adaptive_step || P->restart_rule == SK_GPU_RESTART_NDG)) return -1;
     if (orc_csr_transpose(A, &At)) return -1;
 #define ALLOCN(p) p = (double *)calloc((size_t)n + 1, sizeof(double))
 #define ALLOCM(p) p = (double *)calloc((size_t)m + 1, sizeof(double))
     ALLOCN(x); ALLOCN(xb); ALLOCN(xs); ALLOCN(xr); ALLOCN(xa); ALLOCN(aty); ALLOCN(xc);
     ALLOCM(y); ALLOCM(ys); ALLOCM(yr); ALLOCM(ya); ALLOCM(axb); ALLOCM(ax); ALLOCM(yc); ALLOCM(axc); ALLOCM(axp);
+    if (Q) { ALLOCN(qx); ALLOCN(qxc); ALLOCN(qxa); ALLOCN(cq); }
 #undef ALLOCN
 #undef ALLOCM
     cost_norm = finite_norm(c, n);
@@ -966,6 +1001,7 @@ int orc_pdlp_solve(const orc_csr *A, const double *c, const double *l, const dou
     if (x0) for (j = 0; j < n; ++j) x[j] = xb[j] = xr[j] = x0[j];
     if (y0) for (i = 0; i < m; ++i) y[i] = yr[i] = y0[i];
     if (x0) { orc_csr_mv(A, x, ax); memcpy(axp, ax, (size_t)m * sizeof(double)); }
+    if (x0 && Q) orc_csr_mv(Q, x, qx);
     eta = eta_global;
 
     while (iteration < P->max_iterations) {
@@ -978,7 +1014,8 @@ int orc_pdlp_solve(const orc_csr *A, const double *c, const double *l, const dou
         for (k = 0; k < period; ++k) {
             if (P->adaptive_step) {
                 const double sigma = eta * omega, tau = eta / omega;
-                double dx2 = 0.0, dy2 = 0.0, inter = 0.0, movement, eta_bar, kk, shrink, grow, eta_next;
+                double dx2 = 0.0, dy2 = 0.0, inter = 0.0, dq = 0.0, movement, eta_bar, kk, shrink, grow;
+                double eta_next;
                 int accept;
                 for (i = 0; i < m; ++i) {
                     const double axbar = 2.0 * ax[i] - axp[i];
@@ -986,14 +1023,22 @@ int orc_pdlp_solve(const orc_csr *A, const double *c, const double *l, const dou
                     yc[i] = v - sigma * clamp_to(v / sigma, rl[i], ru[i]);
                 }
                 orc_csr_mv(&At, yc, aty);
-                for (j = 0; j < n; ++j) xc[j] = clamp_to(x[j] - tau * (c[j] + aty[j]), l[j], u[j]);
+                /* QP: the gradient at x carries Qx */
+                for (j = 0; j < n; ++j)
+                    xc[j] = clamp_to(x[j] - tau * (c[j] + aty[j] + (Q ? qx[j] : 0.0)), l[j], u[j]);
                 orc_csr_mv(A, xc, axc);
-                for (j = 0; j < n; ++j) { const double d = xc[j] - x[j]; dx2 += d * d; }
+                if (Q) orc_csr_mv(Q, xc, qxc);
+                for (j = 0; j < n; ++j) {
+                    const double d = xc[j] - x[j];
+                    dx2 += d * d;
+                    if (Q) dq += d * (qxc[j] - qx[j]);
+                }
                 for (i = 0; i < m; ++i) {
                     const double dy = yc[i] - y[i], adx = axc[i] - ax[i];
                     dy2 += dy * dy; inter += dy * adx;
                 }
-                inter = fabs(inter);
+                /* the local bound also covers dx'Q dx/2 (>= 0 for convex Q) */
+                inter = fabs(inter) + 0.5 * fmax(dq, 0.0);
                 movement = 0.5 * (omega * dx2 + dy2 / omega);
                 eta_bar = (inter > 0.0 && movement > 0.0) ? movement / inter : eta_max;
                 accept = eta <= eta_bar;
@@ -1009,6 +1054,7 @@ int orc_pdlp_solve(const orc_csr *A, const double *c, const double *l, const dou
                     accept_count += 1.0;
                     for (j = 0; j < n; ++j) { x[j] = xc[j]; xs[j] += xc[j]; }
                     for (i = 0; i < m; ++i) { y[i] = yc[i]; ys[i] += yc[i]; axp[i] = ax[i]; ax[i] = axc[i]; }
+                    if (Q) memcpy(qx, qxc, (size_t)n * sizeof(double));
                 }
             } else {
                 const double tau = eta_global / omega, sigma = eta_global * omega;
@@ -1035,7 +1081,8 @@ int orc_pdlp_solve(const orc_csr *A, const double *c, const double *l, const dou
 
         orc_csr_mv(A, x, ax);
         orc_csr_mv(&At, y, aty);
-        cur = residuals(n, m, x, y, ax, aty, c, l, u, rl, ru, xr, yr, cost_norm, bound_norm);
+        if (Q) orc_csr_mv(Q, x, qx);
+        cur = residuals_q(n, m, x, y, ax, aty, qx, c, l, u, rl, ru, xr, yr, cost_norm, bound_norm, cq);
         if (P->adaptive_step) {
             terms = accept_count - accepted_at_restart;
             accepted_total = accept_count;
@@ -1050,7 +1097,8 @@ int orc_pdlp_solve(const orc_csr *A, const double *c, const double *l, const dou
             double *scratch = P->adaptive_step ? axb : ax;
             orc_csr_mv(A, xa, scratch);
             orc_csr_mv(&At, ya, aty);
-            avg = residuals(n, m, xa, ya, scratch, aty, c, l, u, rl, ru, xr, yr, cost_norm, bound_norm);
+            if (Q) orc_csr_mv(Q, xa, qxa);
+            avg = residuals_q(n, m, xa, ya, scratch, aty, qxa, c, l, u, rl, ru, xr, yr, cost_norm, bound_norm, cq);
         }
         st->host_syncs++;
         average_better = kkt_worst(&avg) < kkt_worst(&cur);
@@ -1103,6 +1151,7 @@ int orc_pdlp_solve(const orc_csr *A, const double *c, const double *l, const dou
             memcpy(y, ya, (size_t)m * sizeof(double));
             memcpy(xb, xa, (size_t)n * sizeof(double));
             if (P->adaptive_step) memcpy(ax, axb, (size_t)m * sizeof(double));
+            if (Q) memcpy(qx, qxa, (size_t)n * sizeof(double));
         } else {
             memcpy(xb, x, (size_t)n * sizeof(double));
         }
@@ -1146,8 +1195,25 @@ int orc_pdlp_solve(const orc_csr *A, const double *c, const double *l, const dou
     st->seconds = wall_seconds() - t0;
     free(x); free(xb); free(xs); free(xr); free(xa); free(aty); free(xc);
     free(y); free(ys); free(yr); free(ya); free(axb); free(ax); free(yc); free(axc); free(axp);
+    free(qx); free(qxc); free(qxa); free(cq);
     orc_csr_free(&At);
     return 0;
+}
+
+int orc_pdlp_solve(const orc_csr *A, const double *c, const double *l, const double *u,
+                   const double *rl, const double *ru, const sk_gpu_pdlp_params *P,
+                   const double *x0, const double *y0, double *x_out, double *y_out,
+                   sk_gpu_pdlp_stats *st)
+{
+    return pdlp_solve(A, NULL, c, l, u, rl, ru, P, x0, y0, x_out, y_out, st);
+}
+
+int orc_pdlp_solve_qp(const orc_csr *A, const orc_csr *Q, const double *c, const double *l,
+                      const double *u, const double *rl, const double *ru,
+                      const sk_gpu_pdlp_params *P, const double *x0, const double *y0,
+                      double *x_out, double *y_out, sk_gpu_pdlp_stats *st)
+{
+    return pdlp_solve(A, Q, c, l, u, rl, ru, P, x0, y0, x_out, y_out, st);
 }
 
 static double to_ieee(double b)
@@ -1159,9 +1225,9 @@ static double to_ieee(double b)
 
 /* run_first_order, src/lp/LpSolver.cpp:62-99: Ruiz, then Pock-Chambolle on
  * the Ruiz-scaled matrix, composed and applied once; x = C x', y = R y'. */
-int orc_first_order(const orc_csr *A, const double *c, const double *l, const double *u,
-                    const double *rl, const double *ru, const sk_gpu_pdlp_params *P,
-                    double *x, double *y, sk_gpu_pdlp_stats *st)
+static int first_order(const orc_csr *A, const orc_csr *Q, const double *c, const double *l,
+                       const double *u, const double *rl, const double *ru,
+                       const sk_gpu_pdlp_params *P, double *x, double *y, sk_gpu_pdlp_stats *st)
 {
     const int m = A->rows, n = A->cols;
     double *R1 = (double *)malloc((size_t)(m + 1) * sizeof(double));
@@ -1196,11 +1262,37 @@ int orc_first_order(const orc_csr *A, const double *c, const double *l, const do
         us[j] = to_ieee(u[j]) / C1[j];
     }
     for (i = 0; i < m; ++i) { rls[i] = R1[i] * to_ieee(rl[i]); rus[i] = R1[i] * to_ieee(ru[i]); }
-    rc = orc_pdlp_solve(&S, cs, ls, us, rls, rus, P, NULL, NULL, x, y, st);
+    if (Q) {
+        /* the scaled Hessian C Q C (the row/column scaling comes from A alone) */
+        orc_csr Qs;
+        Qs.rows = Q->rows; Qs.cols = Q->cols; Qs.nnz = Q->nnz; Qs.p = Q->p; Qs.j = Q->j;
+        Qs.v = (double *)malloc((size_t)(Q->nnz + 1) * sizeof(double));
+        for (i = 0; i < n; ++i)
+            for (k = Q->p[i]; k < Q->p[i + 1]; ++k) Qs.v[k] = Q->v[k] * C1[i] * C1[Q->j[k]];
+        rc = pdlp_solve(&S, &Qs, cs, ls, us, rls, rus, P, NULL, NULL, x, y, st);
+        free(Qs.v);
+    } else {
+        rc = pdlp_solve(&S, NULL, cs, ls, us, rls, rus, P, NULL, NULL, x, y, st);
+    }
     for (j = 0; j < n; ++j) x[j] *= C1[j];
     for (i = 0; i < m; ++i) y[i] *= R1[i];
     free(R1); free(C1); free(R2); free(C2); free(cs); free(ls); free(us); free(rls); free(rus); free(S.v);
     return rc;
+}
+
+int orc_first_order(const orc_csr *A, const double *c, const double *l, const double *u,
+                    const double *rl, const double *ru, const sk_gpu_pdlp_params *P,
+                    double *x, double *y, sk_gpu_pdlp_stats *st)
+{
+    return first_order(A, NULL, c, l, u, rl, ru, P, x, y, st);
+}
+
+int orc_first_order_qp(const orc_csr *A, const orc_csr *Q, const double *c, const double *l,
+                       const double *u, const double *rl, const double *ru,
+                       const sk_gpu_pdlp_params *P, double *x, double *y, sk_gpu_pdlp_stats *st)
+{
+    if (!Q || Q->rows != A->cols || Q->cols != A->cols) return -1;
+    return first_order(A, Q, c, l, u, rl, ru, P, x, y, st);
 }
 
 /* Relative KKT triple of (x, y) in the space the data is given in; bounds

@@ -20,6 +20,9 @@
  *   orc_ndg                no reference counterpart: the normalized
  *                          duality gap of Applegate et al. 2021, s.3.1,
  *                          the restart merit of restart_rule NDG
+ *   orc_pdlp_solve_qp,     no reference counterpart (GpuPdlp is LP-only):
+ *   orc_first_order_qp     the two above with a convex Hessian, as the
+ *                          device engine extends them
  *
  * Pinning: see the header of sk_oracle.c.
  */
@@ -79,6 +82,21 @@ int orc_first_order(const orc_csr *A, const double *c, const double *l,
                     const double *u, const double *rl, const double *ru,
                     const sk_gpu_pdlp_params *params, double *x, double *y,
                     sk_gpu_pdlp_stats *stats);
+/* The same two with a convex Hessian Q (CSR, n x n, symmetric): the
+ * primal gradient is c + Qx + A'y at the current x, the step bound covers
+ * |dy'A dx| + dx'Q dx / 2, the KKT terms use c + Qx and both objectives
+ * carry x'Qx/2; scaling comes from A alone and Q is scaled to C Q C.  No
+ * reference counterpart (GpuPdlp is LP-only); it restates the device
+ * engine's QP rule.  Adaptive steps and KKT restarts only (else -1). */
+int orc_pdlp_solve_qp(const orc_csr *A, const orc_csr *Q, const double *c,
+                      const double *l, const double *u, const double *rl,
+                      const double *ru, const sk_gpu_pdlp_params *params,
+                      const double *x0, const double *y0, double *x, double *y,
+                      sk_gpu_pdlp_stats *stats);
+int orc_first_order_qp(const orc_csr *A, const orc_csr *Q, const double *c,
+                       const double *l, const double *u, const double *rl,
+                       const double *ru, const sk_gpu_pdlp_params *params,
+                       double *x, double *y, sk_gpu_pdlp_stats *stats);
 /* out: rel primal, rel dual, rel gap, primal objective, dual objective. */
 void orc_relative_kkt(const orc_csr *A, const orc_csr *Q, const double *c,
                       const double *rl, const double *ru, const double *l,

@@ -63,6 +63,10 @@ def load():
                                                C.POINTER(sg.PdlpStats)])
         _decl(lib, "orc_first_order", C.c_int, [R, D, D, D, D, D, C.POINTER(sg.PdlpParams), D, D,
                                                 C.POINTER(sg.PdlpStats)])
+        _decl(lib, "orc_pdlp_solve_qp", C.c_int, [R, R, D, D, D, D, D, C.POINTER(sg.PdlpParams), D, D, D, D,
+                                                  C.POINTER(sg.PdlpStats)])
+        _decl(lib, "orc_first_order_qp", C.c_int, [R, R, D, D, D, D, D, C.POINTER(sg.PdlpParams), D, D,
+                                                   C.POINTER(sg.PdlpStats)])
         _decl(lib, "orc_relative_kkt", None, [R, R, D, D, D, D, D, D, D, D])
         _decl(lib, "orc_ndg", C.c_double, [C.c_int, C.c_int] + [D] * 10 + [C.c_double])
         _orc = lib
@@ -223,15 +227,21 @@ def relative_kkt(model, x, y):
 
 
 def first_order_pdlp(model, params):
-    """Scaling + PDLP + unscaling on the CPU, FP64 (orc_first_order)."""
+    """Scaling + PDLP + unscaling on the CPU, FP64 (orc_first_order, or
+    orc_first_order_qp when the model has a Hessian)."""
     orc = load()
     A = Csr.from_model(model)
+    Q = Csr.from_q(model)
     c, rl, ru, l, u = (model.vec(k) for k in ("c", "rlow", "rupp", "clow", "cupp"))
     x = np.zeros(max(model.ncol, 1))
     y = np.zeros(max(model.nrow, 1))
     st = sg.PdlpStats()
-    rc = orc.orc_first_order(A.ref(), sg.dptr(c), sg.dptr(l), sg.dptr(u), sg.dptr(rl), sg.dptr(ru),
-                             C.byref(params), sg.dptr(x), sg.dptr(y), C.byref(st))
+    args = (sg.dptr(c), sg.dptr(l), sg.dptr(u), sg.dptr(rl), sg.dptr(ru), C.byref(params), sg.dptr(x),
+            sg.dptr(y), C.byref(st))
+    if Q is None:
+        rc = orc.orc_first_order(A.ref(), *args)
+    else:
+        rc = orc.orc_first_order_qp(A.ref(), Q.ref(), *args)
     return rc, x[:model.ncol], y[:model.nrow], st
 
 

@@ -13,6 +13,8 @@
   oracle's FP64 relative KKT and agreeing in objective.
 * The mixed engine's first restart window on the headline LP against the
   oracle's FP64 PDLP.
+* Both engines' first iterations and first restart window on the config-2
+  QP against the oracle's FP64 QP PDLP.
 """
 import ctypes as C
 
@@ -163,6 +165,49 @@ def test_config2_solve_certified_by_the_oracle(prec):
 
 
 test_config2_solve_certified_by_the_oracle.objs = {}
+
+
+def _gpu_pdlp_qp(model, **params):
+    """sk_gpu_pdlp_solve with the model's Hessian."""
+    p, i, v = model.csc()
+    rp, cj, vv = sg.csc_to_csr(model.nrow, model.ncol, p, i, v)
+    vec = {k: model.vec(k) for k in ("c", "rlow", "rupp", "clow", "cupp")}
+    q = model.q_csc()
+    P = sg.default_pdlp_params(**params)
+    x = np.zeros(model.ncol)
+    y = np.zeros(model.nrow)
+    st = sg.PdlpStats()
+    rc = sg.load_gpu().sk_gpu_pdlp_solve(model.nrow, model.ncol, int(rp[-1]), sg.iptr(rp), sg.iptr(cj),
+                                         sg.dptr(vv), sg.iptr(q[0]), sg.iptr(q[1]), sg.dptr(q[2]), int(q[0][-1]),
+                                         sg.dptr(vec["c"]), sg.dptr(vec["rlow"]), sg.dptr(vec["rupp"]),
+                                         sg.dptr(vec["clow"]), sg.dptr(vec["cupp"]), None, None, C.byref(P),
+                                         sg.dptr(x), sg.dptr(y), C.byref(st))
+    assert rc == 0, sg.last_error()
+    return x, y, st
+
+
+@pytest.mark.parametrize("prec,its,tol", [(sg.PREC_FP64, 8, 1e-8), (sg.PREC_FP64, 64, 1e-8),
+                                          (sg.PREC_MIXED, 8, 1e-6), (sg.PREC_MIXED, 64, 1e-4)])
+def test_config2_first_iterations_match_the_oracle(prec, its, tol):
+    """The config-2 QP at its real shape (n = 500,000, m = 250,000, nnz(A)
+    3M, Q = D + B'B) against the oracle's FP64 QP PDLP stopped at the same
+    iteration: the same ||A||, the same accept/reject count and the same
+    iterate -- FP64 up to the summation order, the mixed engine (FP32
+    corrections anchored at the FP64 start) within its FP32 drift.
+    Measured: FP64 5e-15 (8) and 6e-14 (64); mixed 7e-8 (8) and 1.4e-5
+    (64)."""
+    model = _config2()
+    P = dict(max_iterations=its, use_graph=0)
+    rc_o, xo, yo, sto = so.first_order_pdlp(model, sg.default_pdlp_params(precision=sg.PREC_FP64, **P))
+    assert rc_o == 0 and sto.iterations == its
+    x, y, st = _gpu_pdlp_qp(model, precision=prec, **P)
+    assert st.iterations == its
+    assert abs(st.matrix_norm - sto.matrix_norm) <= 1e-10 * sto.matrix_norm
+    assert st.rejected_steps == sto.rejected_steps
+    ex = np.linalg.norm(x - xo) / (1.0 + np.linalg.norm(xo))
+    ey = np.linalg.norm(y - yo) / (1.0 + np.linalg.norm(yo))
+    print(f"config-2 prec={prec} its={its}: ex={ex:.3e} ey={ey:.3e}")
+    assert ex <= tol and ey <= tol, (ex, ey)
 
 
 def test_headline_mixed_engine_first_window_matches_the_oracle():

@@ -51,7 +51,9 @@ __device__ __forceinline__ void wg_sum(double (&v)[S], double *sh /* S * BT/64 *
         if (lane == 0) sh[s * W + wave] = t;
     }
     __syncthreads();
-#pragma unroll
+    // one sum at a time: unrolled over all S * W partials the compiler
+    // issues every LDS read first and spills to hold them
+#pragma unroll 1
     for (int s = 0; s < S; ++s) {
         double t = sh[s * W];
 #pragma unroll
@@ -543,6 +545,7 @@ template <typename T> void launch_phase(const skb_batch *B, int block_threads, i
     case 128: hipLaunchKernelGGL((k_batch_pdlp<128, T>), dim3(grid), dim3(128), lds, bstream(), *B); break;
     case 256: hipLaunchKernelGGL((k_batch_pdlp<256, T>), dim3(grid), dim3(256), lds, bstream(), *B); break;
     case 512: hipLaunchKernelGGL((k_batch_pdlp<512, T>), dim3(grid), dim3(512), lds, bstream(), *B); break;
+    case 1024: hipLaunchKernelGGL((k_batch_pdlp<1024, T>), dim3(grid), dim3(1024), lds, bstream(), *B); break;
     default: break;
     }
 }
@@ -558,6 +561,7 @@ template <typename T> int resident(int block_threads, size_t lds)
     case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_batch_pdlp<128, T>, 128, lds); break;
     case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_batch_pdlp<256, T>, 256, lds); break;
     case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_batch_pdlp<512, T>, 512, lds); break;
+    case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_batch_pdlp<1024, T>, 1024, lds); break;
     default: break;
     }
     return e == hipSuccess ? v : -1;
@@ -575,7 +579,9 @@ size_t skb_lds_bytes(int max_m, int max_n, int word_bytes)
 
 int skb_launch(const skb_batch *B, int block_threads, int grid, int fp64)
 {
-    if (block_threads != 64 && block_threads != 128 && block_threads != 256 && block_threads != 512) return -1;
+    if (block_threads != 64 && block_threads != 128 && block_threads != 256 && block_threads != 512 &&
+        block_threads != 1024)
+        return -1;
     const size_t lds = skb_lds_bytes(B->max_m, B->max_n, fp64 ? 8 : 4);
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, bstream(), B->queue);
     hipError_t e = hipGetLastError();

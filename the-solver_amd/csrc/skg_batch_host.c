@@ -182,12 +182,29 @@ typedef struct phases {
     int run32, run64;          /* which phases run */
 } phases;
 
+/* The return code of the last launch_phases that failed (0: none). */
+static _Thread_local int g_phase_rc;
+
 static int launch_phases(const skb_batch *B, const phases *ph)
 {
     int rc = 0;
-    if (ph->run32 && (rc = skb_launch(B, ph->bt, ph->grid32, 0))) return rc;
-    if (ph->run64 && (rc = skb_launch(B, ph->bt, ph->grid64, 1))) return rc;
+    if (ph->run32 && (rc = skb_launch(B, ph->bt, ph->grid32, 0))) return g_phase_rc = rc;
+    if (ph->run64 && (rc = skb_launch(B, ph->bt, ph->grid64, 1))) return g_phase_rc = rc;
     return 0;
+}
+
+/* The error of a failed timed launch, never 0 (a launch refused before it
+ * reached the runtime leaves no HIP error behind: -1). */
+static int launch_failure(const char *what)
+{
+    int e = g_phase_rc > 0 ? g_phase_rc : skk_last_launch_error();
+    g_phase_rc = 0;
+    if (e <= 0) {
+        skg_fail("%s refused (block size or grid)", what);
+        return -1;
+    }
+    skg_hip_fail(what, e);
+    return e;
 }
 
 /* Time the phases (events on the library stream); milliseconds or < 0. */
@@ -352,7 +369,7 @@ int sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_opt
             tp.grid32 = grid_for(sample, cand[k], lds32, 0);
             tp.grid64 = grid_for(sample, cand[k], lds64, 1);
             ms = timed_launch(&T, &tp, &none, 0);
-            if (ms < 0.0) { rc = skk_last_launch_error(); goto fail; }
+            if (ms < 0.0) { rc = launch_failure("autotune launch"); goto fail; }
             if (mixed && ph.run64) {
                 /* four windows hand nothing to the FP64 tail, which is most
                    of a batch's time and whose LDS and occupancy differ:
@@ -364,7 +381,7 @@ int sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_opt
                 t64.run32 = 0;
                 t64.run64 = 1;
                 ms64 = timed_launch(&T64, &t64, &none, 0);
-                if (ms64 < 0.0) { rc = skk_last_launch_error(); goto fail; }
+                if (ms64 < 0.0) { rc = launch_failure("autotune launch"); goto fail; }
                 ms += ms64;
             }
             stats->tune_ms[k] = ms;
@@ -379,13 +396,15 @@ int sk_gpu_batch_solve(int count, const sk_model *models, const sk_gpu_batch_opt
         for (k = 0; k < 3; ++k)
             if (stats->tune_ms[k] <= 1.10 * best) bt = cand[k];
     }
-    if (bt != 64 && bt != 128 && bt != 256 && bt != 512) { skg_fail("block_threads must be 64..512"); rc = -1; goto fail; }
+    if (bt != 64 && bt != 128 && bt != 256 && bt != 512 && bt != 1024) {
+        skg_fail("block_threads must be 64, 128, 256, 512 or 1024"); rc = -1; goto fail;
+    }
     ph.bt = bt;
     ph.grid32 = grid_for(count, bt, lds32, 0);
     ph.grid64 = grid_for(count, bt, lds64, 1);
     {
         const double ms = timed_launch(&B, &ph, &graph, o.use_graph);
-        if (ms < 0.0) { rc = skk_last_launch_error(); skg_hip_fail("batch launch", rc ? rc : -1); goto fail; }
+        if (ms < 0.0) { rc = launch_failure("batch launch"); goto fail; }
         stats->device_seconds = ms * 1e-3;
     }
     hout = (skb_out *)malloc(sizeof(skb_out) * (size_t)count);
