@@ -157,10 +157,12 @@ sk_status sk_gpu_read_mps(const char *path, sk_model *m);
 
 /* ------------------------------------------------------------------------
  * Device-resident sparse matrix.
- * Replaces SankhyaCudaCSR (cuda/include/sankhya_cuda.h:14-21).  The first six
- * fields keep the reference's meaning; `values` are FP64.  The handle also
- * keeps CSR(A^T) on the device so that A^T*y is a gather, not an atomic
- * scatter, and an opaque row-bin plan for the binned SpMV kernels.
+ * Replaces SankhyaCudaCSR (cuda/include/sankhya_cuda.h:14-21) with the same
+ * six fields in the same order (a SankhyaCudaCSR can be passed as is);
+ * `values` are FP64.  Behind the handle the library also keeps CSR(A^T) on
+ * the device, so that A^T*y is a gather, not an atomic scatter, and the
+ * row-tile plan of the SpMV kernels; it finds them through
+ * device_row_offsets, which identifies the handle.
  * ---------------------------------------------------------------------- */
 typedef struct sk_gpu_csr {
     int   rows;
@@ -169,16 +171,18 @@ typedef struct sk_gpu_csr {
     void *device_row_offsets;       /* int32[rows+1]                      */
     void *device_column_indices;    /* int32[nnz]                         */
     void *device_values;            /* double[nnz]                        */
-    void *plan;                     /* opaque: CSR(A^T), bins, FP32 copy  */
 } sk_gpu_csr;
 
 /* cuda/include/sankhya_cuda.h:25-32 sankhya_cuda_csr_create.  Validates
  * (canonical offsets, in-range columns, finite values) before any device
- * allocation; frees previous content; 0 on success, -1 on failure. */
+ * allocation; frees previous content this library created (other field
+ * values are just overwritten); 0 on success, -1 on failure. */
 int  sk_gpu_csr_create(sk_gpu_csr *matrix, int rows, int cols, int nnz,
                        const int *row_offsets, const int *column_indices,
                        const double *values);
-/* cuda/include/sankhya_cuda.h:33 sankhya_cuda_csr_destroy.  Idempotent. */
+/* cuda/include/sankhya_cuda.h:33 sankhya_cuda_csr_destroy.  Idempotent;
+ * a handle the library did not create (or already destroyed) is only
+ * cleared. */
 void sk_gpu_csr_destroy(sk_gpu_csr *matrix);
 
 /* cuda/include/sankhya_cuda.h:38-42.  Device pointers; enqueued on the

@@ -41,6 +41,34 @@ def test_csr_create_validates_before_touching_the_device():
     assert "column index" in sg.last_error()
 
 
+def test_device_csr_handle_has_the_reference_layout(tmp_path):
+    """sk_gpu_csr is SankhyaCudaCSR field for field (three ints, then the
+    three device pointers), so a caller's SankhyaCudaCSR passes as is."""
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stddef.h>\n#include "sankhya_gpu.h"\n'
+        "struct six { int rows, cols, nnz; void *ro, *ci, *v; };\n"
+        "_Static_assert(sizeof(sk_gpu_csr) == sizeof(struct six), \"size\");\n"
+        "_Static_assert(offsetof(sk_gpu_csr, nnz) == offsetof(struct six, nnz), \"nnz\");\n"
+        "_Static_assert(offsetof(sk_gpu_csr, device_row_offsets) == offsetof(struct six, ro), \"ro\");\n"
+        "_Static_assert(offsetof(sk_gpu_csr, device_values) == offsetof(struct six, v), \"v\");\n")
+    import subprocess
+    r = subprocess.run(["gcc", "-std=c11", "-fsyntax-only", "-I", str(ROOT / "include"), str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert C.sizeof(sg.GpuCsr) == 40
+
+
+def test_destroy_of_a_foreign_handle_only_clears_it():
+    """A handle the library did not create (here: made-up pointers) is
+    cleared, and nothing is freed -- no device call, so this runs anywhere."""
+    lib = sg.load_gpu()
+    h = sg.GpuCsr(3, 4, 5, 0x1000, 0x2000, 0x3000)
+    lib.sk_gpu_csr_destroy(C.byref(h))
+    assert (h.rows, h.cols, h.nnz) == (0, 0, 0)
+    assert h.device_row_offsets is None and h.device_values is None
+
+
 def test_params_defaults_and_names():
     lib = sg.load_gpu()
     p = sg.PdlpParams()

@@ -43,7 +43,7 @@ def test_rejections_before_allocation():
     assert lib.sk_gpu_csr_create(C.byref(h), 2, 3, 4, sg.iptr(OFF), sg.iptr(bad_col), sg.dptr(VAL)) != 0
     bad_val = np.array([1.0, np.nan, 3.0, 4.0])
     assert lib.sk_gpu_csr_create(C.byref(h), 2, 3, 4, sg.iptr(OFF), sg.iptr(COL), sg.dptr(bad_val)) != 0
-    assert h.plan is None
+    assert h.device_row_offsets is None
 
 
 def test_device_chain_known_answers():
@@ -64,7 +64,7 @@ def test_device_chain_known_answers():
     assert np.array_equal(dy.numpy(), [424.0, 1203.0])
     lib.sk_gpu_csr_destroy(C.byref(h))
     lib.sk_gpu_csr_destroy(C.byref(h))       # idempotent
-    assert h.rows == 0 and h.plan is None
+    assert h.rows == 0 and h.device_row_offsets is None
 
 
 def _random_csr(rng, m, n, lengths):

@@ -6,6 +6,7 @@
 #include "skg_internal.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -16,15 +17,67 @@ typedef struct csr_plan {
     dcsr A, AT;      /* caller row order, bins addressed through row lists */
 } csr_plan;
 
-void sk_gpu_csr_destroy(sk_gpu_csr *matrix)     /* cuda/src/sankhya_cuda.cu:154-163 */
+/* The handle keeps the reference's six fields (SankhyaCudaCSR,
+ * cuda/include/sankhya_cuda.h:14-21), so the plan behind it -- CSR(A^T),
+ * the row tiles -- is found through its row-offset allocation, which the
+ * plan owns and no other handle shares.  A few live handles at a time: a
+ * locked list, searched linearly. */
+typedef struct plan_slot {
+    const void *key;
+    csr_plan *plan;
+} plan_slot;
+
+static pthread_mutex_t g_plan_lock = PTHREAD_MUTEX_INITIALIZER;
+static plan_slot *g_plans;
+static int g_plan_count, g_plan_cap;
+
+static int plan_register(csr_plan *plan)
+{
+    int rc = 0;
+    pthread_mutex_lock(&g_plan_lock);
+    if (g_plan_count == g_plan_cap) {
+        const int cap = g_plan_cap ? 2 * g_plan_cap : 16;
+        plan_slot *grown = (plan_slot *)realloc(g_plans, sizeof(plan_slot) * (size_t)cap);
+        if (grown) { g_plans = grown; g_plan_cap = cap; }
+        else rc = -1;
+    }
+    if (!rc) {
+        g_plans[g_plan_count].key = plan->A.rp;
+        g_plans[g_plan_count].plan = plan;
+        ++g_plan_count;
+    }
+    pthread_mutex_unlock(&g_plan_lock);
+    return rc;
+}
+
+/* The plan registered for `key`, removed from the list when `take`. */
+static csr_plan *plan_find(const void *key, int take)
+{
+    csr_plan *plan = NULL;
+    int k;
+    if (!key) return NULL;
+    pthread_mutex_lock(&g_plan_lock);
+    for (k = 0; k < g_plan_count; ++k)
+        if (g_plans[k].key == key) {
+            plan = g_plans[k].plan;
+            if (take) g_plans[k] = g_plans[--g_plan_count];
+            break;
+        }
+    pthread_mutex_unlock(&g_plan_lock);
+    return plan;
+}
+
+/* cuda/src/sankhya_cuda.cu:154-163.  Frees only what this library
+ * created: a handle it does not know (zeroed, copied after a destroy, or
+ * never created) is just cleared. */
+void sk_gpu_csr_destroy(sk_gpu_csr *matrix)
 {
     csr_plan *plan;
     if (!matrix) return;
-    plan = (csr_plan *)matrix->plan;
+    plan = plan_find(matrix->device_row_offsets, 1);
     if (plan) { dcsr_free(&plan->A); dcsr_free(&plan->AT); free(plan); }
     matrix->rows = matrix->cols = matrix->nnz = 0;
     matrix->device_row_offsets = matrix->device_column_indices = matrix->device_values = NULL;
-    matrix->plan = NULL;
 }
 
 static int plan_upload(dcsr *d, const hcsr *a)
@@ -53,17 +106,20 @@ int sk_gpu_csr_create(sk_gpu_csr *matrix, int rows, int cols, int nnz, const int
         return -1;
     }
     hcsr_free(&at);
+    if (plan_register(plan)) {
+        dcsr_free(&plan->A); dcsr_free(&plan->AT); free(plan);
+        return skg_fail("out of host memory");
+    }
     matrix->rows = rows; matrix->cols = cols; matrix->nnz = nnz;
     matrix->device_row_offsets = plan->A.rp;
     matrix->device_column_indices = plan->A.ci;
     matrix->device_values = plan->A.vd;
-    matrix->plan = plan;
     return 0;
 }
 
 static const csr_plan *plan_of(const sk_gpu_csr *m)
 {
-    return m ? (const csr_plan *)m->plan : NULL;
+    return m ? plan_find(m->device_row_offsets, 0) : NULL;
 }
 
 int sk_gpu_spmv_device_f64(const sk_gpu_csr *matrix, const double *dx, double *dy)

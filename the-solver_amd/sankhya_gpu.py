@@ -74,7 +74,7 @@ class SkOptions(C.Structure):
 class GpuCsr(C.Structure):
     _fields_ = [("rows", C.c_int), ("cols", C.c_int), ("nnz", C.c_int),
                 ("device_row_offsets", C.c_void_p), ("device_column_indices", C.c_void_p),
-                ("device_values", C.c_void_p), ("plan", C.c_void_p)]
+                ("device_values", C.c_void_p)]
 
 
 class LpSettings(C.Structure):
