@@ -542,6 +542,10 @@ def workload_warm4(args):
     """Learned warm start on the parametric LP family (m=20,000, n=50,000)."""
     lib = sg.load_gpu()
     w = sg.load_workloads()
+    if args.train is None:
+        args.train = 1000
+    if args.epochs is None:
+        args.epochs = 300
     m, n, dim = (20000, 50000, 16) if not args.quick else (2000, 5000, 16)
     seed = 20260829
     model = sg.Model()
@@ -651,10 +655,11 @@ def main():
     ap.add_argument("--skip-ndg", action="store_true", help="lp1: no NDG-restart solve")
     ap.add_argument("--no-tune", action="store_true", help="lp1: fixed tiling instead of the autotuner")
     ap.add_argument("--tiling", default="", help="lp1: fix the tiling as TILE,GRID,STREAM (no autotuner)")
-    ap.add_argument("--train", type=int, default=1000, help="warm4: training instances")
+    ap.add_argument("--train", type=int, default=None,
+                    help="warm4: training instances (default 1000); mlp4: rows (default 4096)")
     ap.add_argument("--test", type=int, default=100, help="warm4: held-out instances")
     ap.add_argument("--compare", type=int, default=20, help="warm4: held-out instances solved cold and warm")
-    ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--epochs", type=int, default=None, help="warm4: default 300; mlp4: default 2")
     ap.add_argument("--train-eps", type=float, default=1e-6, help="warm4: tolerance of the training solves")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="copy this many MiB device-to-device 3 times first (counter calibration)")
@@ -684,14 +689,14 @@ def workload_mlp4(args):
     (16 -> 512 -> 512 -> 70,000, batch 256) on random standardised data:
     training and inference throughput against the BF16 MFMA peak."""
     n_in, hid, out = 16, 512, 70000
-    rows = max(args.train if args.train != 16000 else 4096, 512)
+    rows = max(args.train if args.train is not None else 4096, 512)
     rng = np.random.default_rng(0)
     X = rng.standard_normal((rows, n_in)).astype(np.float32)
     Y = rng.standard_normal((rows, out)).astype(np.float32)
     net = sg.Mlp(n_in, hid, out, seed=1)
     net.set_scaling(X.mean(0), X.std(0), Y.mean(0), Y.std(0))
     net.train(X[:512], Y[:512], epochs=1, batch=256, lr=1e-3)      # warm-up
-    epochs = max(args.epochs if args.epochs != 40 else 2, 1)
+    epochs = max(args.epochs if args.epochs is not None else 2, 1)
     t = time.perf_counter()
     net.train(X, Y, epochs=epochs, batch=256, lr=1e-3)
     train_s = time.perf_counter() - t

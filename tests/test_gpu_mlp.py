@@ -58,3 +58,36 @@ def test_training_reduces_loss_on_a_smooth_map():
     net.set_scaling(X.mean(0), X.std(0), Y.mean(0), Y.std(0))
     hist = net.train(X, Y, epochs=30, batch=128, lr=2e-3)
     assert hist[-1] < 0.2 * hist[0], hist
+
+
+def test_adam_step_and_both_weight_copies():
+    """One training step (batch = the whole set): the parameters move by
+    Adam's first step, p - lr * g / (|g| + eps) elementwise (m/(1-b1) = g,
+    v/(1-b2) = g^2), and the BF16 weight copies the kernels read -- row-major
+    for the backward products (refreshed by Adam), transposed for the
+    forward ones -- are those of the new parameters: prediction and
+    gradients afterwards match the oracle at the new point."""
+    rng = np.random.default_rng(6)
+    n_in, hid, n_out, B = 16, 192, 700, 96
+    net = sg.Mlp(n_in, hid, n_out, seed=11)
+    X = rng.standard_normal((B, n_in)).astype(np.float32)
+    T = rng.standard_normal((B, n_out)).astype(np.float32)
+    net.set_scaling(X.mean(0), X.std(0) + 0.1, T.mean(0), T.std(0) + 0.1)
+    p0 = net.params().astype(np.float64)
+    _, g = net.gradients(X, T)
+    lr = 1e-3
+    net.train(X, T, epochs=1, batch=B, lr=lr)
+    p1 = net.params().astype(np.float64)
+    g = g.astype(np.float64)
+    step = p0 - lr * g / (np.abs(g) + 1e-8)
+    big = np.abs(g) > 1e-5          # the batch order changes g's last bits
+    assert big.mean() > 0.5
+    assert np.max(np.abs(p1 - step)[big]) <= 1e-3 * lr
+    assert np.max(np.abs(p1 - p0)) <= 1.01 * lr
+    # the copies: forward (transposed copies) and backward (row-major copies)
+    im, isd, om, osd = (a.astype(np.float64) for a in (X.mean(0), X.std(0) + 0.1, T.mean(0), T.std(0) + 0.1))
+    Y = net.predict(X)
+    _, ref_g, ref_y = mo.forward_backward(net.params(), X, T, n_in, hid, n_out, im, isd, om, osd, bf16=True)
+    assert np.max(np.abs(Y - ref_y)) <= 1e-3 * (np.max(np.abs(ref_y)) + 1)
+    _, g1 = net.gradients(X, T)
+    assert np.linalg.norm(g1 - ref_g) <= 2e-2 * np.linalg.norm(ref_g)
