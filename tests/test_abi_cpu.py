@@ -6,6 +6,7 @@ import re
 from pathlib import Path
 
 import numpy as np
+import pytest
 
 import sankhya_gpu as sg
 
@@ -125,3 +126,41 @@ def test_batch_validates_before_touching_the_device():
     small = sg.Model.generated("skw_bounded_lp", 30, 60, 5, 4)
     rc, _, _, _, _ = sg.batch_solve([small, big])
     assert rc == -1 and "65536" in sg.last_error()
+
+
+class _Hcsr(C.Structure):
+    _fields_ = [("rows", C.c_int), ("cols", C.c_int), ("nnz", C.c_int), ("p", C.POINTER(C.c_int)),
+                ("j", C.POINTER(C.c_int)), ("v", C.POINTER(C.c_double))]
+
+
+@pytest.mark.parametrize("m,n,nnz", [(300000, 200000, 1_500_000), (1000, 5, 1_200_000)])
+def test_host_transpose_is_the_stable_counting_sort(m, n, nnz):
+    """hcsr_transpose above 2^20 entries runs on host threads (each owns a
+    range of output rows); its output must be the serial counting sort's:
+    each output row in ascending source position, duplicates included, on
+    a matrix with a few very long columns."""
+    lib = sg.load_gpu()
+    rng = np.random.default_rng(3)
+    rows = np.sort(rng.integers(0, m, nnz))
+    cols = rng.integers(0, n, nnz).astype(np.int32)
+    vals = rng.standard_normal(nnz)
+    rp = np.zeros(m + 1, np.int64)
+    np.add.at(rp, rows + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    A = _Hcsr(m, n, nnz, sg.iptr(rp), sg.iptr(cols), sg.dptr(vals))
+    T = _Hcsr()
+    lib.hcsr_transpose.restype = C.c_int
+    lib.hcsr_transpose.argtypes = [C.POINTER(_Hcsr), C.POINTER(_Hcsr)]
+    lib.hcsr_free.restype = None
+    lib.hcsr_free.argtypes = [C.POINTER(_Hcsr)]
+    assert lib.hcsr_transpose(C.byref(A), C.byref(T)) == 0
+    try:
+        p = np.ctypeslib.as_array(T.p, shape=(n + 1,)).copy()
+        j = np.ctypeslib.as_array(T.j, shape=(nnz,)).copy()
+        v = np.ctypeslib.as_array(T.v, shape=(nnz,)).copy()
+    finally:
+        lib.hcsr_free(C.byref(T))
+    order = np.argsort(cols, kind="stable")
+    expect_p = np.concatenate([[0], np.cumsum(np.bincount(cols, minlength=n))])
+    assert np.array_equal(p, expect_p)
+    assert np.array_equal(j, rows[order]) and np.array_equal(v, vals[order])

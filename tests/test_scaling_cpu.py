@@ -37,9 +37,9 @@ def test_product_scaling_matches_oracle(seed, m, n):
     lib.skg_ruiz.restype = None
     lib.skg_ruiz.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, sg.c_double_p, sg.c_double_p]
     lib.skg_pock_chambolle_into.restype = None
-    lib.skg_pock_chambolle_into.argtypes = [C.c_void_p, sg.c_double_p, sg.c_double_p]
+    lib.skg_pock_chambolle_into.argtypes = [C.c_void_p, C.c_void_p, sg.c_double_p, sg.c_double_p]
     lib.skg_ruiz(C.addressof(A.s), C.addressof(AT.s), 100, 1e-2, sg.dptr(R), sg.dptr(Cs))
-    lib.skg_pock_chambolle_into(C.addressof(A.s), sg.dptr(R), sg.dptr(Cs))
+    lib.skg_pock_chambolle_into(C.addressof(A.s), C.addressof(AT.s), sg.dptr(R), sg.dptr(Cs))
     # oracle: Ruiz, scale, Pock-Chambolle on the scaled matrix, compose
     R1, C1 = np.zeros(m), np.zeros(n)
     orc.orc_ruiz(A.ref(), 100, 1e-2, sg.dptr(R1), sg.dptr(C1))

@@ -105,7 +105,7 @@ static int pack(hbatch *h, int count, const sk_model *models, const sk_gpu_pdlp_
         if (hcsr_transpose(&A, &AT)) { hcsr_free(&A); return skg_fail("out of host memory"); }
         if (P->scaling) {
             skg_ruiz(&A, &AT, P->ruiz_iterations, P->ruiz_tol, R, Cs);
-            skg_pock_chambolle_into(&A, R, Cs);
+            skg_pock_chambolle_into(&A, &AT, R, Cs);
         } else {
             for (i = 0; i < d->m; ++i) R[i] = 1.0;
             for (j = 0; j < d->n; ++j) Cs[j] = 1.0;

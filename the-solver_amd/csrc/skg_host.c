@@ -11,6 +11,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+
+#include <omp.h>
 #include <time.h>
 
 /* ----------------------------------------------------------- errors */
@@ -209,17 +211,55 @@ int hcsr_alloc(hcsr *a, int rows, int cols, long long nnz)
     return 0;
 }
 
+/* Transpose by host threads with the serial counting sort's exact output:
+ * each thread owns a contiguous range of output rows (source columns),
+ * scans the whole source in order and places only the entries of its
+ * range -- sequential reads, no atomics, and every output row filled in
+ * ascending source position.  `cur` holds cols + 1 ints of cursors; the
+ * threads use disjoint ranges of it. */
+static void transpose_threaded(const hcsr *a, hcsr *t, int *cur)
+{
+    const int rows = a->rows, cols = a->cols;
+#pragma omp parallel
+    {
+        const int nt = omp_get_num_threads(), id = omp_get_thread_num();
+        const int c0 = (int)((long long)cols * id / nt), c1 = (int)((long long)cols * (id + 1) / nt);
+        int i, k;
+        for (k = 0; k < a->nnz; ++k) {
+            const int c = a->j[k];
+            if (c >= c0 && c < c1) t->p[c + 1]++;
+        }
+#pragma omp barrier
+#pragma omp single
+        for (i = 0; i < cols; ++i) t->p[i + 1] += t->p[i];
+        for (i = c0; i < c1; ++i) cur[i] = t->p[i];
+        for (i = 0; i < rows; ++i)
+            for (k = a->p[i]; k < a->p[i + 1]; ++k) {
+                const int c = a->j[k];
+                if (c >= c0 && c < c1) {
+                    const int q = cur[c]++;
+                    t->j[q] = i; t->v[q] = a->v[k];
+                }
+            }
+    }
+}
+
 /* Transpose by counting sort; entries of each output row come out in
- * ascending source-row order. */
+ * ascending source-row order (source order within a row). */
 int hcsr_transpose(const hcsr *a, hcsr *t)
 {
     int i, k;
     int *next;
     if (hcsr_alloc(t, a->cols, a->rows, a->nnz)) return -1;
-    for (k = 0; k < a->nnz; ++k) t->p[a->j[k] + 1]++;
-    for (i = 0; i < a->cols; ++i) t->p[i + 1] += t->p[i];
     next = (int *)malloc(((size_t)a->cols + 1) * sizeof(int));
     if (!next) { hcsr_free(t); return -1; }
+    if (a->nnz >= (1 << 20) && omp_get_max_threads() > 1) {
+        transpose_threaded(a, t, next);
+        free(next);
+        return 0;
+    }
+    for (k = 0; k < a->nnz; ++k) t->p[a->j[k] + 1]++;
+    for (i = 0; i < a->cols; ++i) t->p[i + 1] += t->p[i];
     memcpy(next, t->p, ((size_t)a->cols + 1) * sizeof(int));
     for (i = 0; i < a->rows; ++i)
         for (k = a->p[i]; k < a->p[i + 1]; ++k) {
@@ -452,6 +492,7 @@ int dcsr_upload(dcsr *d, const hcsr *a, int with_f32)
         float *tmp = (float *)malloc(sizeof(float) * (size_t)(a->nnz ? a->nnz : 1));
         int k;
         if (!tmp) { rc = -1; goto fail; }
+#pragma omp parallel for schedule(static)
         for (k = 0; k < a->nnz; ++k) tmp[k] = (float)a->v[k];
         if ((rc = skk_malloc((void **)&d->vf, sizeof(float) * (size_t)(a->nnz ? a->nnz : 1))) ||
             (rc = skk_h2d(d->vf, tmp, sizeof(float) * (size_t)a->nnz))) { free(tmp); goto fail; }

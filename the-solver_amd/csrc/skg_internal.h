@@ -59,7 +59,7 @@ void dcsr_free(dcsr *d);
 /* Ruiz equilibration (src/lp/Scaling.cpp:11-92) and Pock-Chambolle alpha=1
  * composed into (R, C) (Scaling.cpp:111-152); at = CSR(A^T). */
 void skg_ruiz(const hcsr *a, const hcsr *at, int max_iters, double tol, double *R, double *C);
-void skg_pock_chambolle_into(const hcsr *a, double *R, double *C);
+void skg_pock_chambolle_into(const hcsr *a, const hcsr *at, double *R, double *C);
 
 /* Process-wide tuning (set by sk_gpu_set_tuning, read at plan time). */
 extern skg_tuning g_skg_tuning;

@@ -66,11 +66,14 @@ static double finite_norm(const double *v, int n)       /* GpuPdlp.cpp:84-90 */
 
 /* Ruiz equilibration, src/lp/Scaling.cpp:11-92: simultaneous row/column
  * division by sqrt of the current inf-norms until every nonzero norm is
- * within tol of 1; identity if that does not happen within max_iters. */
+ * within tol of 1; identity if that does not happen within max_iters.
+ * Rows and columns are swept by host threads; every value is a maximum or
+ * an elementwise update, so the result does not depend on the thread
+ * count. */
 void skg_ruiz(const hcsr *a, const hcsr *at, int max_iters, double tol, double *R, double *C)
 {
     const int m = a->rows, n = a->cols;
-    int i, j, k, iter, converged = 0;
+    int i, j, iter, converged = 0;
     double *rn, *cn;
     for (i = 0; i < m; ++i) R[i] = 1.0;
     for (j = 0; j < n; ++j) C[j] = 1.0;
@@ -80,26 +83,32 @@ void skg_ruiz(const hcsr *a, const hcsr *at, int max_iters, double tol, double *
     if (!rn || !cn) { free(rn); free(cn); return; }
     for (iter = 0; iter < max_iters; ++iter) {
         double dev = 0.0;
+#pragma omp parallel for schedule(dynamic, 8192) reduction(max : dev)
         for (i = 0; i < m; ++i) {
             double best = 0.0;
+            int k;
             for (k = a->p[i]; k < a->p[i + 1]; ++k) {
                 const double v = fabs(a->v[k]) * C[a->j[k]];
                 if (v > best) best = v;
             }
             rn[i] = R[i] * best;
+            if (rn[i] > 0.0 && fabs(rn[i] - 1.0) > dev) dev = fabs(rn[i] - 1.0);
         }
+#pragma omp parallel for schedule(dynamic, 8192) reduction(max : dev)
         for (j = 0; j < n; ++j) {
             double best = 0.0;
+            int k;
             for (k = at->p[j]; k < at->p[j + 1]; ++k) {
                 const double v = fabs(at->v[k]) * R[at->j[k]];
                 if (v > best) best = v;
             }
             cn[j] = C[j] * best;
+            if (cn[j] > 0.0 && fabs(cn[j] - 1.0) > dev) dev = fabs(cn[j] - 1.0);
         }
-        for (i = 0; i < m; ++i) if (rn[i] > 0.0 && fabs(rn[i] - 1.0) > dev) dev = fabs(rn[i] - 1.0);
-        for (j = 0; j < n; ++j) if (cn[j] > 0.0 && fabs(cn[j] - 1.0) > dev) dev = fabs(cn[j] - 1.0);
         if (dev < tol) { converged = 1; break; }
+#pragma omp parallel for schedule(static)
         for (i = 0; i < m; ++i) if (rn[i] > 0.0) R[i] /= sqrt(rn[i]);
+#pragma omp parallel for schedule(static)
         for (j = 0; j < n; ++j) if (cn[j] > 0.0) C[j] /= sqrt(cn[j]);
     }
     if (!converged) {
@@ -112,21 +121,33 @@ void skg_ruiz(const hcsr *a, const hcsr *at, int max_iters, double tol, double *
 /* Pock-Chambolle with alpha = 1 on the matrix a scaled by (R0, C0),
  * src/lp/Scaling.cpp:111-145: R_i = 1/sqrt(sum_j |a'_ij|),
  * C_j = 1/sqrt(sum_i |a'_ij|); empty rows/columns keep 1.  Multiplies the
- * result into R0, C0 (compose_scaling, Scaling.cpp:147-152). */
-void skg_pock_chambolle_into(const hcsr *a, double *R, double *C)
+ * result into R0, C0 (compose_scaling, Scaling.cpp:147-152).  Column sums
+ * run over at = a^T, whose rows list each column's entries in increasing
+ * row order -- the order of a row-major sweep -- so threads change no bit. */
+void skg_pock_chambolle_into(const hcsr *a, const hcsr *at, double *R, double *C)
 {
     const int m = a->rows, n = a->cols;
-    int i, j, k;
+    int i, j;
     double *ra = (double *)calloc((size_t)m + 1, sizeof(double));
     double *ca = (double *)calloc((size_t)n + 1, sizeof(double));
     if (!ra || !ca) { free(ra); free(ca); return; }
-    for (i = 0; i < m; ++i)
+#pragma omp parallel for schedule(dynamic, 8192)
+    for (i = 0; i < m; ++i) {
+        int k;
         for (k = a->p[i]; k < a->p[i + 1]; ++k) {
             const double v = fabs(a->v[k] * R[i] * C[a->j[k]]);
-            if (v == 0.0) continue;
-            ra[i] += pow(v, 1.0);
-            ca[a->j[k]] += pow(v, 1.0);
+            if (v != 0.0) ra[i] += v;
         }
+    }
+#pragma omp parallel for schedule(dynamic, 8192)
+    for (j = 0; j < n; ++j) {
+        int k;
+        for (k = at->p[j]; k < at->p[j + 1]; ++k) {
+            const int r = at->j[k];
+            const double v = fabs(at->v[k] * R[r] * C[j]);
+            if (v != 0.0) ca[j] += v;
+        }
+    }
     for (i = 0; i < m; ++i) if (ra[i] > 0.0) R[i] *= 1.0 / sqrt(ra[i]);
     for (j = 0; j < n; ++j) if (ca[j] > 0.0) C[j] *= 1.0 / sqrt(ca[j]);
     free(ra); free(ca);
@@ -281,11 +302,14 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
     hcsr A0, AT0, Ap, ATp;
     int *inv_r = NULL, *inv_c = NULL, i, j, k, q, rc = -1;
     skk_bins bA, bAT;
+    double tp[6];    /* phase clock: transpose, scaling, order, copies, upload */
+    tp[0] = skg_now();
     memset(&Ap, 0, sizeof(Ap)); memset(&ATp, 0, sizeof(ATp)); memset(&AT0, 0, sizeof(AT0));
     E->m = m; E->n = n;
     A0.rows = m; A0.cols = n; A0.nnz = rp[m];
     A0.p = (int *)rp; A0.j = (int *)ci; A0.v = (double *)v;
     if (hcsr_transpose(&A0, &AT0)) return skg_fail("out of host memory");
+    tp[1] = skg_now();
     E->R = (double *)malloc(sizeof(double) * (size_t)(m + 1));
     E->C = (double *)malloc(sizeof(double) * (size_t)(n + 1));
     E->perm_r = (int *)malloc(sizeof(int) * (size_t)(m + 1));
@@ -297,11 +321,12 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
     }
     if (P->scaling) {
         skg_ruiz(&A0, &AT0, P->ruiz_iterations, P->ruiz_tol, E->R, E->C);
-        skg_pock_chambolle_into(&A0, E->R, E->C);
+        skg_pock_chambolle_into(&A0, &AT0, E->R, E->C);
     } else {
         for (i = 0; i < m; ++i) E->R[i] = 1.0;
         for (j = 0; j < n; ++j) E->C[j] = 1.0;
     }
+    tp[2] = skg_now();
     /* Bin classes decide the permutations.  Inside a class, rows are grouped
        by the longest column they hold (ties: the lower index), so that the
        rows of each long column of a power-law matrix sit together and A^T y
@@ -314,6 +339,7 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
         int *first = (int *)malloc(sizeof(int) * (size_t)(n + 1));
         int *rowkey = (int *)malloc(sizeof(int) * (size_t)(m + 1));
         if (!first || !rowkey) { free(first); free(rowkey); skg_fail("out of host memory"); goto done; }
+#pragma omp parallel for schedule(static)
         for (i = 0; i < m; ++i) {
             int best = n, bdeg = -1, q2;
             for (q2 = A0.p[i]; q2 < A0.p[i + 1]; ++q2) {
@@ -325,6 +351,7 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
         rc = skg_plan_bins_keyed(A0.p, m, &g_skg_tuning, rowkey, n, E->perm_r, &bA);
         if (!rc) {
             for (k = 0; k < m; ++k) inv_r[E->perm_r[k]] = k;
+#pragma omp parallel for schedule(static)
             for (j = 0; j < n; ++j) {
                 int mn = m, q2;
                 for (q2 = AT0.p[j]; q2 < AT0.p[j + 1]; ++q2) if (inv_r[AT0.j[q2]] < mn) mn = inv_r[AT0.j[q2]];
@@ -338,28 +365,48 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
     }
     for (k = 0; k < m; ++k) inv_r[E->perm_r[k]] = k;
     for (k = 0; k < n; ++k) inv_c[E->perm_c[k]] = k;
+    tp[3] = skg_now();
     if (hcsr_alloc(&Ap, m, n, A0.nnz) || hcsr_alloc(&ATp, n, m, A0.nnz)) { skg_fail("out of host memory"); goto done; }
+    /* the scaled matrix and its transpose in slot order: offsets first,
+       then every row filled independently */
     q = 0;
     for (k = 0; k < m; ++k) {
         const int r = E->perm_r[k];
         Ap.p[k] = q;
-        for (i = A0.p[r]; i < A0.p[r + 1]; ++i, ++q) {
-            Ap.j[q] = inv_c[A0.j[i]];
-            Ap.v[q] = A0.v[i] * E->R[r] * E->C[A0.j[i]];
-        }
+        q += A0.p[r + 1] - A0.p[r];
     }
     Ap.p[m] = q;
+#pragma omp parallel for schedule(dynamic, 4096) private(i)
+    for (k = 0; k < m; ++k) {
+        const int r = E->perm_r[k];
+        int o = Ap.p[k];
+        for (i = A0.p[r]; i < A0.p[r + 1]; ++i, ++o) {
+            Ap.j[o] = inv_c[A0.j[i]];
+            Ap.v[o] = A0.v[i] * E->R[r] * E->C[A0.j[i]];
+        }
+    }
     q = 0;
     for (k = 0; k < n; ++k) {
         const int col = E->perm_c[k];
         ATp.p[k] = q;
-        for (i = AT0.p[col]; i < AT0.p[col + 1]; ++i, ++q) {
-            ATp.j[q] = inv_r[AT0.j[i]];
-            ATp.v[q] = AT0.v[i] * E->R[AT0.j[i]] * E->C[col];
-        }
+        q += AT0.p[col + 1] - AT0.p[col];
     }
     ATp.p[n] = q;
+#pragma omp parallel for schedule(dynamic, 4096) private(i)
+    for (k = 0; k < n; ++k) {
+        const int col = E->perm_c[k];
+        int o = ATp.p[k];
+        for (i = AT0.p[col]; i < AT0.p[col + 1]; ++i, ++o) {
+            ATp.j[o] = inv_r[AT0.j[i]];
+            ATp.v[o] = AT0.v[i] * E->R[AT0.j[i]] * E->C[col];
+        }
+    }
+    tp[4] = skg_now();
     if ((rc = dcsr_upload(&E->A, &Ap, E->mixed)) || (rc = dcsr_upload(&E->AT, &ATp, E->mixed))) goto done;
+    tp[5] = skg_now();
+    if (P->verbosity > 0)
+        fprintf(stderr, "set-up: transpose %.3f s, scaling %.3f s, ordering %.3f s, permuted copies %.3f s, "
+                "upload %.3f s\n", tp[1] - tp[0], tp[2] - tp[1], tp[3] - tp[2], tp[4] - tp[3], tp[5] - tp[4]);
     E->tile_nnz = skg_tile_nnz(Ap.nnz);
     E->grid_cap = SKK_SPMV_GRID;
     E->stream = E->A.k.stream;
