@@ -95,45 +95,52 @@ static int pack(hbatch *h, int count, const sk_model *models, const sk_gpu_pdlp_
     AL(h->cd, double, cols); AL(h->ld, double, cols); AL(h->ud, double, cols);
     AL(h->rld, double, rows); AL(h->rud, double, rows); AL(h->R, double, rows); AL(h->C, double, cols);
 #undef AL
-    for (k = 0; k < count; ++k) {
-        const sk_model *md = &models[k];
-        skb_inst *d = &h->inst[k];
-        hcsr A, AT;
-        double *R = h->R + d->row, *Cs = h->C + d->col;
-        int i, j, p;
-        if (hcsr_from_csc(&md->A, &A)) return skg_fail("out of host memory");
-        if (hcsr_transpose(&A, &AT)) { hcsr_free(&A); return skg_fail("out of host memory"); }
-        if (P->scaling) {
-            skg_ruiz(&A, &AT, P->ruiz_iterations, P->ruiz_tol, R, Cs);
-            skg_pock_chambolle_into(&A, &AT, R, Cs);
-        } else {
-            for (i = 0; i < d->m; ++i) R[i] = 1.0;
-            for (j = 0; j < d->n; ++j) Cs[j] = 1.0;
-        }
-        for (i = 0; i <= d->m; ++i) h->a_rp[d->a_rp + i] = A.p[i];
-        for (j = 0; j <= d->n; ++j) h->t_rp[d->t_rp + j] = AT.p[j];
-        for (i = 0; i < d->m; ++i)
-            for (p = A.p[i]; p < A.p[i + 1]; ++p) {
-                const double v = A.v[p] * R[i] * Cs[A.j[p]];
-                h->a_ci[d->nz + p] = (unsigned short)A.j[p]; h->a_vd[d->nz + p] = v; h->a_v[d->nz + p] = (float)v;
+    /* instances on host threads: each writes its own slices */
+    {
+        int failed = 0;
+#pragma omp parallel for schedule(dynamic, 8) reduction(|| : failed)
+        for (k = 0; k < count; ++k) {
+            const sk_model *md = &models[k];
+            skb_inst *d = &h->inst[k];
+            hcsr A, AT;
+            double *R = h->R + d->row, *Cs = h->C + d->col;
+            int i, j, p;
+            if (failed) continue;
+            if (hcsr_from_csc(&md->A, &A)) { failed = 1; continue; }
+            if (hcsr_transpose(&A, &AT)) { hcsr_free(&A); failed = 1; continue; }
+            if (P->scaling) {
+                skg_ruiz(&A, &AT, P->ruiz_iterations, P->ruiz_tol, R, Cs);
+                skg_pock_chambolle_into(&A, &AT, R, Cs);
+            } else {
+                for (i = 0; i < d->m; ++i) R[i] = 1.0;
+                for (j = 0; j < d->n; ++j) Cs[j] = 1.0;
             }
-        for (j = 0; j < d->n; ++j)
-            for (p = AT.p[j]; p < AT.p[j + 1]; ++p) {
-                const double v = AT.v[p] * R[AT.j[p]] * Cs[j];
-                h->t_ci[d->nz + p] = (unsigned short)AT.j[p]; h->t_vd[d->nz + p] = v; h->t_v[d->nz + p] = (float)v;
+            for (i = 0; i <= d->m; ++i) h->a_rp[d->a_rp + i] = A.p[i];
+            for (j = 0; j <= d->n; ++j) h->t_rp[d->t_rp + j] = AT.p[j];
+            for (i = 0; i < d->m; ++i)
+                for (p = A.p[i]; p < A.p[i + 1]; ++p) {
+                    const double v = A.v[p] * R[i] * Cs[A.j[p]];
+                    h->a_ci[d->nz + p] = (unsigned short)A.j[p]; h->a_vd[d->nz + p] = v; h->a_v[d->nz + p] = (float)v;
+                }
+            for (j = 0; j < d->n; ++j)
+                for (p = AT.p[j]; p < AT.p[j + 1]; ++p) {
+                    const double v = AT.v[p] * R[AT.j[p]] * Cs[j];
+                    h->t_ci[d->nz + p] = (unsigned short)AT.j[p]; h->t_vd[d->nz + p] = v; h->t_v[d->nz + p] = (float)v;
+                }
+            for (j = 0; j < d->n; ++j) {
+                h->cd[d->col + j] = md->c[j] * Cs[j];
+                h->ld[d->col + j] = to_ieee(md->clow[j]) / Cs[j];
+                h->ud[d->col + j] = to_ieee(md->cupp[j]) / Cs[j];
             }
-        for (j = 0; j < d->n; ++j) {
-            h->cd[d->col + j] = md->c[j] * Cs[j];
-            h->ld[d->col + j] = to_ieee(md->clow[j]) / Cs[j];
-            h->ud[d->col + j] = to_ieee(md->cupp[j]) / Cs[j];
+            for (i = 0; i < d->m; ++i) {
+                h->rld[d->row + i] = R[i] * to_ieee(md->rlow[i]);
+                h->rud[d->row + i] = R[i] * to_ieee(md->rupp[i]);
+            }
+            d->cost_norm = sqrt(finite_sq(h->cd + d->col, d->n));
+            d->bound_norm = sqrt(finite_sq(h->rld + d->row, d->m) + finite_sq(h->rud + d->row, d->m));
+            hcsr_free(&A); hcsr_free(&AT);
         }
-        for (i = 0; i < d->m; ++i) {
-            h->rld[d->row + i] = R[i] * to_ieee(md->rlow[i]);
-            h->rud[d->row + i] = R[i] * to_ieee(md->rupp[i]);
-        }
-        d->cost_norm = sqrt(finite_sq(h->cd + d->col, d->n));
-        d->bound_norm = sqrt(finite_sq(h->rld + d->row, d->m) + finite_sq(h->rud + d->row, d->m));
-        hcsr_free(&A); hcsr_free(&AT);
+        if (failed) return skg_fail("out of host memory");
     }
     g_sort_inst = h->inst;
     qsort(h->order, (size_t)count, sizeof(int), by_size_desc);
