@@ -9,10 +9,15 @@
  * (Jayaragul/The-solver, paths relative to that repository).
  *
  * Threading: the library queues all device work on one stream per device
- * (default priority), starts no threads and takes no locks; a CSR handle or
- * a PDLP session must not be used from two host threads at once.  The
- * caller selects the device (hipSetDevice) as with the reference's
- * cudaSetDevice (apps/sankhya_pdhg.cu:79).
+ * (default priority).  The host set-up of large problems (transposes,
+ * scaling sweeps, permuted copies, batch packing) runs on OpenMP threads
+ * (OMP_NUM_THREADS), each output written by one thread in the serial order,
+ * so results do not depend on the thread count; the registry behind the
+ * CSR handles takes a lock.  A CSR handle or a PDLP session must not be
+ * used from two host threads at once, and calls that run on one device
+ * share its stream and event timeline: serialize them.  The caller selects
+ * the device (hipSetDevice) as with the reference's cudaSetDevice
+ * (apps/sankhya_pdhg.cu:79).
  */
 #ifndef SANKHYA_GPU_H
 #define SANKHYA_GPU_H

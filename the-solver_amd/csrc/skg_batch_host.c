@@ -57,7 +57,7 @@ static void hbatch_free(hbatch *h)
     memset(h, 0, sizeof(*h));
 }
 
-static const skb_inst *g_sort_inst;
+static _Thread_local const skb_inst *g_sort_inst;   /* the comparator's batch */
 static int by_size_desc(const void *a, const void *b)
 {
     const skb_inst *x = &g_sort_inst[*(const int *)a], *y = &g_sort_inst[*(const int *)b];
