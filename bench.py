@@ -439,7 +439,7 @@ def _engines(lib, model, arrays, q, args, use_graph=0, repeats=3):
 def _line(metric_value, unit, args, workload, extra):
     line = {"metric": METRIC, "value": metric_value, "unit": unit, "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "fp32 iterates + fp64 checks/tail", "data": "synthetic (seeded generator, in-process)",
+            "dtype": "fp32 iterates anchored in fp64, fp64 checks", "data": "synthetic (seeded generator, in-process)",
             "config": {"workload": workload}}
     line.update(extra)
     return line
