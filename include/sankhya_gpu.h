@@ -225,11 +225,12 @@ int sk_gpu_spmv_f32(int rows, int cols, int nnz, const int *row_offsets,
 const char *sk_gpu_last_error(void);
 
 /* Device diagnostic, no reference counterpart: the random-gather issue
- * rate of the current device -- `count` 4-byte gathers at uniformly random
- * positions of a table of `table_elems` floats, streamed like a CSR
- * product's (value, index) pairs.  It bounds every sparse product here,
- * whose gathered vectors are indexed at random.  *gathers_per_second is
- * the mean over `reps` timed launches.  0 or -1. */
+ * rate of the current device -- `count` (at most 2^30) 4-byte gathers at
+ * uniformly random positions of a table of `table_elems` floats, streamed
+ * like a CSR product's (value, index) pairs, eight gathers in flight per
+ * lane.  It bounds every sparse product here, whose gathered vectors are
+ * indexed at random.  *gathers_per_second is the mean over `reps` timed
+ * launches.  0 or -1. */
 int sk_gpu_gather_rate(int table_elems, int count, int reps, double *gathers_per_second);
 
 /* ------------------------------------------------------------------------

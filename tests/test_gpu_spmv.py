@@ -139,6 +139,7 @@ def test_gather_rate_probe():
     assert lib.sk_gpu_gather_rate(1 << 20, 1 << 22, 3, C.byref(rate)) == 0, sg.last_error()
     assert 1e9 < rate.value < 1e13
     assert lib.sk_gpu_gather_rate(0, 10, 1, C.byref(rate)) == -1
+    assert lib.sk_gpu_gather_rate(10, (1 << 30) + 1, 1, C.byref(rate)) == -1
 
 
 def test_headline_size_products_match_scipy():

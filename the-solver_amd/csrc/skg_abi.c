@@ -826,7 +826,7 @@ int sk_gpu_gather_rate(int table_elems, int count, int reps, double *gathers_per
 {
     int rc;
     skg_clear_error();
-    if (table_elems <= 0 || count <= 0 || reps <= 0 || !gathers_per_second)
+    if (table_elems <= 0 || count <= 0 || count > (1 << 30) || reps <= 0 || !gathers_per_second)
         return skg_fail("invalid gather probe arguments");
     if (skg_have_device()) return -1;
     rc = skk_gather_probe(table_elems, count, reps, gathers_per_second);

@@ -217,20 +217,70 @@ int finalize(const double *part, int stride, int count, int nsum, int nmax, doub
 
 // ----------------------------------------------------------- binned SpMV
 
-// Lane-strided partial dot product of one row; two accumulators.
+// Lane-strided partial dot product of one row: the entries p, p + L,
+// p + 2L, ... below e, the even ones of that sequence summed into s0 and
+// the odd ones into s1, each in order.  Eight entries a round: their index
+// and value loads, then their eight gathers, are issued back to back with
+// no branch between them (a position past the end reads the row's last
+// entry again and its product is not added), so a lane has eight gathers
+// in flight -- a per-entry branch would make each gather wait for the
+// previous one.
 template <typename VT, typename XT, typename AT, bool NT = true>
 __device__ __forceinline__ AT row_part(const int *__restrict__ ci, const VT *__restrict__ val,
                                        const XT *__restrict__ x, int p, int e, int L)
 {
     AT s0 = 0, s1 = 0;
-    for (; p + L < e; p += 2 * L) {
-        const int c0 = mload<NT>(ci + p), c1 = mload<NT>(ci + p + L);
-        const VT v0 = mload<NT>(val + p), v1 = mload<NT>(val + p + L);
-        s0 += static_cast<AT>(v0) * static_cast<AT>(x[c0]);
-        s1 += static_cast<AT>(v1) * static_cast<AT>(x[c1]);
+    const int last = e - 1;
+    for (; p < e; p += 8 * L) {
+        int c[8];
+        VT v[8];
+        XT g[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int q = min(p + u * L, last);
+            c[u] = mload<NT>(ci + q);
+            v[u] = mload<NT>(val + q);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) g[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (p + u * L < e) {
+                const AT t = static_cast<AT>(v[u]) * static_cast<AT>(g[u]);
+                if (u & 1) s1 += t;
+                else s0 += t;
+            }
+        }
     }
-    if (p < e) s0 += static_cast<AT>(mload<NT>(val + p)) * static_cast<AT>(x[mload<NT>(ci + p)]);
     return s0 + s1;
+}
+
+// The products of a tile's entries [p0, p1) (p1 > p0) into prod, K entries
+// per lane at positions p0 + k * kBlock + tid: all K index and value loads,
+// then all K gathers, with no branch between them (positions past the end
+// read the tile's last entry and are not stored), so the gathers are in
+// flight together.
+template <int K, typename VT, typename XT, typename AT, bool NT>
+__device__ __forceinline__ void tile_products(const int *__restrict__ ci, const VT *__restrict__ val,
+                                              const XT *__restrict__ x, int p0, int p1, AT *prod)
+{
+    const int tid = threadIdx.x, last = p1 - 1;
+    int c[K];
+    VT v[K];
+    XT g[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int q = min(p0 + k * kBlock + tid, last);
+        c[k] = mload<NT>(ci + q);
+        v[k] = mload<NT>(val + q);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) g[k] = x[c[k]];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int p = p0 + k * kBlock + tid;
+        if (p < p1) prod[p - p0] = static_cast<AT>(v[k]) * static_cast<AT>(g[k]);
+    }
 }
 
 // CSR-stream product over row tiles (the CSR-Adaptive scheme of Greathouse
@@ -271,11 +321,15 @@ __device__ __forceinline__ void spmv_tile(const int *__restrict__ rp, const int 
         __syncthreads();
         return;
     }
-    // products of the tile's entries, kPer independent loads per lane
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        const int p = p0 + k * kBlock + tid;
-        if (p < p1) prod[p - p0] = static_cast<AT>(mload<NT>(val + p)) * static_cast<AT>(x[mload<NT>(ci + p)]);
+    // products of the tile's entries, at most kPer per lane: 1, 2, 4 or 8
+    // (the tile's entry count is the same for every lane)
+    static_assert(kPer >= 1 && kPer <= 8, "tile_products covers up to 8 entries per lane");
+    if (p1 > p0) {
+        const int per = (p1 - p0 + kBlock - 1) / kBlock;
+        if (per > 4) tile_products<8, VT, XT, AT, NT>(ci, val, x, p0, p1, prod);
+        else if (per > 2) tile_products<4, VT, XT, AT, NT>(ci, val, x, p0, p1, prod);
+        else if (per > 1) tile_products<2, VT, XT, AT, NT>(ci, val, x, p0, p1, prod);
+        else tile_products<1, VT, XT, AT, NT>(ci, val, x, p0, p1, prod);
     }
     __syncthreads();
     if (nr * 2 <= kBlock) {
@@ -1813,19 +1867,27 @@ __global__ void k_probe_fill(int *idx, float *val, int count, int table, float *
     }
 }
 
-// The access pattern of a CSR product without its row bookkeeping: each
-// lane streams 8 (value, index) pairs with coalesced non-temporal loads and
-// gathers table[index] for each.
+// The access pattern of a CSR product without its row bookkeeping: lanes
+// stride over the (value, index) stream with coalesced loads, eight pairs
+// and their eight gathers table[index] in flight per lane, over a grid of
+// at most SKK_SPMV_GRID workgroups (the products' own grid cap).
 __global__ __launch_bounds__(kBlock) void k_probe_gather(const int *__restrict__ idx, const float *__restrict__ val,
                                                          const float *__restrict__ tab, float *out, int count)
 {
-    const int base = blockIdx.x * kBlock * 8 + threadIdx.x;
+    const int stride = gridDim.x * kBlock;
+    int k = blockIdx.x * kBlock + threadIdx.x;
     float s = 0.0f;
+    for (; k + 7 * stride < count; k += 8 * stride) {
+        int j[8];
+        float v[8], g[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int p = base + k * kBlock;
-        if (p < count) s += ntload(val + p) * tab[ntload(idx + p)];
+        for (int u = 0; u < 8; ++u) { j[u] = idx[k + u * stride]; v[u] = val[k + u * stride]; }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) g[u] = tab[j[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u] * g[u];
     }
+    for (; k < count; k += stride) s += val[k] * tab[idx[k]];
     out[blockIdx.x * kBlock + threadIdx.x] = s;
 }
 
@@ -2286,10 +2348,11 @@ int skk_relkkt_noalloc(const skk_csr *A, const skk_csr *AT, const double *c, con
  * launches after one warm-up.  Returns gathers per second in *rate. */
 int skk_gather_probe(int table, int count, int reps, double *rate)
 {
-    if (table <= 0 || count <= 0 || reps <= 0 || !rate) return -1;
+    if (table <= 0 || count <= 0 || count > (1 << 30) || reps <= 0 || !rate) return -1;
     int *idx = nullptr;
     float *val = nullptr, *tab = nullptr, *out = nullptr;
-    const int grid = (count + kBlock * 8 - 1) / (kBlock * 8);
+    const int grid = (int)(((long long)count + kBlock - 1) / kBlock < SKK_SPMV_GRID
+                           ? ((long long)count + kBlock - 1) / kBlock : SKK_SPMV_GRID);
     int rc = skk_malloc((void **)&idx, sizeof(int) * (size_t)count);
     if (!rc) rc = skk_malloc((void **)&val, sizeof(float) * (size_t)count);
     if (!rc) rc = skk_malloc((void **)&tab, sizeof(float) * (size_t)table);
