@@ -217,6 +217,35 @@ int finalize(const double *part, int stride, int count, int nsum, int nmax, doub
 
 // ----------------------------------------------------------- binned SpMV
 
+// Row epilogues.  Every epilogue has row(r, sum, partials).  One with
+// kPrefetch, load(r) -- the row's own inputs, which do not depend on the
+// sum -- and finish(r, sum, inputs, partials) is driven in two halves: its
+// loads are issued right after the tile's gathers, so they are in flight
+// while the products go to LDS and the rows are summed.  finish(r, s,
+// load(r), p) computes exactly what row(r, s, p) does.
+struct NoIn {};
+template <class E, std::enable_if_t<E::kPrefetch, int> = 0>
+__device__ __forceinline__ auto epi_load(const E &e, int r, int) -> decltype(e.load(r))
+{
+    return e.load(r);
+}
+template <class E> __device__ __forceinline__ NoIn epi_load(const E &, int, long) { return NoIn{}; }
+template <class E, typename AT, class In>
+__device__ __forceinline__ auto epi_finish(const E &e, int r, AT s, const In &in, double *p, int)
+    -> decltype(e.finish(r, s, in, p))
+{
+    e.finish(r, s, in, p);
+}
+template <class E, typename AT>
+__device__ __forceinline__ void epi_finish(const E &e, int r, AT s, const NoIn &, double *p, long)
+{
+    e.row(r, s, p);
+}
+
+// Rows per lane when a tile has more rows than half the workgroup (a tile
+// holds at most SKK_TILE_ROWS rows).
+constexpr int kRowsPerLane = (SKK_TILE_ROWS + kBlock - 1) / kBlock;
+
 // Lane-strided partial dot product of one row: the entries p, p + L,
 // p + 2L, ... below e, the even ones of that sequence summed into s0 and
 // the odd ones into s1, each in order.  Eight entries a round: their index
@@ -259,10 +288,12 @@ __device__ __forceinline__ AT row_part(const int *__restrict__ ci, const VT *__r
 // per lane at positions p0 + k * kBlock + tid: all K index and value loads,
 // then all K gathers, with no branch between them (positions past the end
 // read the tile's last entry and are not stored), so the gathers are in
-// flight together.
-template <int K, typename VT, typename XT, typename AT, bool NT>
+// flight together.  mid() runs after the gathers are issued and before
+// their results are stored.
+template <int K, typename VT, typename XT, typename AT, bool NT, class Mid>
 __device__ __forceinline__ void tile_products(const int *__restrict__ ci, const VT *__restrict__ val,
-                                              const XT *__restrict__ x, int p0, int p1, AT *prod)
+                                              const XT *__restrict__ x, int p0, int p1, AT *prod,
+                                              const Mid &mid)
 {
     const int tid = threadIdx.x, last = p1 - 1;
     int c[K];
@@ -276,6 +307,15 @@ __device__ __forceinline__ void tile_products(const int *__restrict__ ci, const 
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) g[k] = x[c[k]];
+    // mid()'s loads must stay behind the gathers, so that waiting for the
+    // gathers below does not wait for them (vector-memory counters retire
+    // in order).  Their addresses take an opaque zero produced here, which
+    // the compiler cannot move its loads above, and nothing is scheduled
+    // across this point.
+    int zero;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(zero)::"memory");
+    __builtin_amdgcn_sched_barrier(0);
+    mid(zero);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int p = p0 + k * kBlock + tid;
@@ -303,6 +343,7 @@ __device__ __forceinline__ void spmv_tile(const int *__restrict__ rp, const int 
                                           const VT *__restrict__ val, const XT *__restrict__ x, int r0,
                                           int r1, const Epi &e, double *part, AT *prod)
 {
+    using In = decltype(epi_load(e, 0, 0));
     constexpr int kPer = SKK_TILE_NNZ / kBlock;
     const int tid = threadIdx.x;
     const int p0 = rp[r0], p1 = rp[r1];
@@ -321,28 +362,52 @@ __device__ __forceinline__ void spmv_tile(const int *__restrict__ rp, const int 
         __syncthreads();
         return;
     }
+    if (nr == 0) return;   // an empty matrix's one empty tile: no rows to finish
+    // Reduction mapping: with nr * 2 <= kBlock, 2^lg lanes per row (lg =
+    // floor(log2(kBlock / nr)) capped at 6) and lane 0 of the group
+    // finishes it; otherwise one lane per row, rows tid, tid + kBlock, ...
+    const bool grouped = nr * 2 <= kBlock;
+    int lg = 0;
+    if (grouped)
+        while (lg < 6 && (nr << (lg + 1)) <= kBlock) ++lg;
+    const int L = 1 << lg, g = tid >> lg, lane = tid & (L - 1);
+    // The rows' offsets and epilogue inputs, loaded after the gathers (row
+    // indices clamped to the tile, so the loads need no branch).
+    int qa[kRowsPerLane], qb[kRowsPerLane];
+    In in[kRowsPerLane];
+    auto fetch = [&](int zero) {
+        if (grouped) {
+            const int row = r0 + min(g, nr - 1) + zero;
+            qa[0] = rp[row] - p0;
+            qb[0] = rp[row + 1] - p0;
+            in[0] = epi_load(e, row, 0);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kRowsPerLane; ++k) {
+                const int row = min(r0 + tid + k * kBlock, r1 - 1) + zero;
+                qa[k] = rp[row] - p0;
+                qb[k] = rp[row + 1] - p0;
+                in[k] = epi_load(e, row, 0);
+            }
+        }
+    };
     // products of the tile's entries, at most kPer per lane: 1, 2, 4 or 8
     // (the tile's entry count is the same for every lane)
     static_assert(kPer >= 1 && kPer <= 8, "tile_products covers up to 8 entries per lane");
     if (p1 > p0) {
         const int per = (p1 - p0 + kBlock - 1) / kBlock;
-        if (per > 4) tile_products<8, VT, XT, AT, NT>(ci, val, x, p0, p1, prod);
-        else if (per > 2) tile_products<4, VT, XT, AT, NT>(ci, val, x, p0, p1, prod);
-        else if (per > 1) tile_products<2, VT, XT, AT, NT>(ci, val, x, p0, p1, prod);
-        else tile_products<1, VT, XT, AT, NT>(ci, val, x, p0, p1, prod);
+        if (per > 4) tile_products<8, VT, XT, AT, NT>(ci, val, x, p0, p1, prod, fetch);
+        else if (per > 2) tile_products<4, VT, XT, AT, NT>(ci, val, x, p0, p1, prod, fetch);
+        else if (per > 1) tile_products<2, VT, XT, AT, NT>(ci, val, x, p0, p1, prod, fetch);
+        else tile_products<1, VT, XT, AT, NT>(ci, val, x, p0, p1, prod, fetch);
+    } else {
+        fetch(0);
     }
     __syncthreads();
-    if (nr * 2 <= kBlock) {
-        // 2^lg lanes per row, lg = floor(log2(kBlock / nr)) capped at 6
-        int lg = 0;
-        while (lg < 6 && (nr << (lg + 1)) <= kBlock) ++lg;
-        const int L = 1 << lg, g = tid >> lg, lane = tid & (L - 1);
+    if (grouped) {
         AT sum = 0;
-        int row = r0 + g;
-        if (g < nr) {
-            const int q1 = rp[row + 1] - p0;
-            for (int q = rp[row] - p0 + lane; q < q1; q += L) sum += prod[q];
-        }
+        if (g < nr)
+            for (int q = qa[0] + lane; q < qb[0]; q += L) sum += prod[q];
         // the L lanes of a row combine through DPP swaps up to 16 lanes
         // (L is uniform; all lanes take part), then LDS permutes
         if (L >= 2) sum += dpp<0xB1>(sum);     // lane ^ 1
@@ -350,14 +415,16 @@ __device__ __forceinline__ void spmv_tile(const int *__restrict__ rp, const int 
         if (L >= 8) sum += dpp<0x141>(sum);    // the other four of the eight
         if (L >= 16) sum += dpp<0x140>(sum);   // the other eight of the row
         for (int off = 16; off < L; off <<= 1) sum += shfl_xor_d(sum, off);
-        if (g < nr && lane == 0) e.row(row, sum, part);
+        if (g < nr && lane == 0) epi_finish(e, r0 + g, sum, in[0], part, 0);
     } else {
-        for (int g = tid; g < nr; g += kBlock) {
-            const int row = r0 + g;
-            const int q1 = rp[row + 1] - p0;
-            AT sum = 0;
-            for (int q = rp[row] - p0; q < q1; ++q) sum += prod[q];
-            e.row(row, sum, part);
+#pragma unroll
+        for (int k = 0; k < kRowsPerLane; ++k) {
+            const int row = r0 + tid + k * kBlock;
+            if (row < r1) {
+                AT sum = 0;
+                for (int q = qa[k]; q < qb[k]; ++q) sum += prod[q];
+                epi_finish(e, row, sum, in[k], part, 0);
+            }
         }
     }
     __syncthreads();   // prod is reused by the next tile
@@ -625,16 +692,21 @@ template <typename T, typename AT> struct EpiPrimalAdaptive {
         tau = static_cast<T>(st->eta / st->omega);
         pending = st->pending;
     }
-    __device__ void row(int j, AT s, double *p) const
+    static constexpr bool kPrefetch = sizeof(T) == 4;
+    struct In {
+        T x, c, l, u, q;
+    };
+    __device__ In load(int j) const { return In{x[j], D.c[j], D.l[j], D.u[j], qx ? qx[j] : T(0)}; }
+    __device__ void finish(int j, AT s, const In &v, double *p) const
     {
-        const T xj = x[j];
-        const T g = D.c[j] + static_cast<T>(s) + (qx ? qx[j] : T(0));
-        const T xn = clampv(xj - tau * g, D.l[j], D.u[j]);
+        const T g = v.c + static_cast<T>(s) + v.q;
+        const T xn = clampv(v.x - tau * g, v.l, v.u);
         xc[j] = xn;
-        if (pending) I.xsum.add(j, xj);
-        const double d = static_cast<double>(xn) - static_cast<double>(xj);
+        if (pending) I.xsum.add(j, v.x);
+        const double d = static_cast<double>(xn) - static_cast<double>(v.x);
         p[0] += d * d;
     }
+    __device__ void row(int j, AT s, double *p) const { finish(j, s, load(j), p); }
     __device__ void end(double *p) const
     {
         double v[1] = {p[0]};
@@ -660,14 +732,20 @@ template <typename T, typename AT> struct EpiProbe {
         const int cur = st->cur;
         y = k.I.y(cur); yc = k.I.y(cur ^ 1); ax = k.I.ax(st->a0); axc = k.I.ax(st->a2);
     }
-    __device__ void row(int i, AT s, double *p) const
+    static constexpr bool kPrefetch = sizeof(T) == 4;
+    struct In {
+        T yc, y, ax;
+    };
+    __device__ In load(int i) const { return In{yc[i], y[i], ax[i]}; }
+    __device__ void finish(int i, AT s, const In &v, double *p) const
     {
         const T a = static_cast<T>(s);
         axc[i] = a;
-        const double dy = static_cast<double>(yc[i]) - static_cast<double>(y[i]);
+        const double dy = static_cast<double>(v.yc) - static_cast<double>(v.y);
         p[0] += dy * dy;
-        p[1] += dy * (static_cast<double>(a) - static_cast<double>(ax[i]));
+        p[1] += dy * (static_cast<double>(a) - static_cast<double>(v.ax));
     }
+    __device__ void row(int i, AT s, double *p) const { finish(i, s, load(i), p); }
     __device__ void end(double *p) const
     {
         double v[2] = {p[0], p[1]};
@@ -803,13 +881,19 @@ template <typename T, typename AT> struct EpiQProbe {
         const int cur = st->cur;
         x = I.x(cur); xc = I.x(cur ^ 1); qx = I.qx(cur); qxc = I.qx(cur ^ 1);
     }
-    __device__ void row(int j, AT s, double *p) const
+    static constexpr bool kPrefetch = sizeof(T) == 4;
+    struct In {
+        T xc, x, qx;
+    };
+    __device__ In load(int j) const { return In{xc[j], x[j], qx[j]}; }
+    __device__ void finish(int j, AT s, const In &v, double *p) const
     {
         const T q = static_cast<T>(s);
         qxc[j] = q;
-        p[0] += (static_cast<double>(xc[j]) - static_cast<double>(x[j])) *
-                (static_cast<double>(q) - static_cast<double>(qx[j]));
+        p[0] += (static_cast<double>(v.xc) - static_cast<double>(v.x)) *
+                (static_cast<double>(q) - static_cast<double>(v.qx));
     }
+    __device__ void row(int j, AT s, double *p) const { finish(j, s, load(j), p); }
     __device__ void end(double *p) const
     {
         double v[1] = {p[0]};
