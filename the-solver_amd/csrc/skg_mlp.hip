@@ -60,12 +60,14 @@ __device__ __forceinline__ void epilogue(const Epi &e, int r, int c, float v)
 // the MFMA operand layout).  Source element (r, k) lives at
 //   src[r * ld + k]  when KC (K-contiguous source), else src[k * ld + r].
 // Loads are 16-byte vectors of 8 elements where the whole vector is in range
-// and aligned (vec), otherwise element by element with zero fill.  The
-// values travel through registers so the next tile's loads overlap the
-// current tile's MFMAs.
+// and aligned (vec), otherwise element by element with zero fill.  A vector
+// stays packed in its registers until store(), which runs after the
+// current tile's MFMAs: nothing reads it earlier, so the load is in flight
+// while they run (unpacking it at load time would make every load wait at
+// once).
 template <bool KC> struct TileLoader {
     static constexpr int kVec = (BM * BK) / (8 * THREADS);   // vectors per thread (2)
-    uint16_t v[kVec][8];
+    uint4 w[kVec];
     __device__ void load(const uint16_t *__restrict__ src, int ld, int r0, int rows, int k0, int kend, bool vec)
     {
         const int t = threadIdx.x;
@@ -76,18 +78,16 @@ template <bool KC> struct TileLoader {
             const int k = KC ? k0 + ((t & 3) << 3) : k0 + (t >> 4) + q * (THREADS / 16);
             const bool full = KC ? (r < rows && k + 8 <= kend) : (k < kend && r + 8 <= rows);
             if (vec && full) {
-                const uint4 w = *reinterpret_cast<const uint4 *>(KC ? src + (size_t)r * ld + k
-                                                                     : src + (size_t)k * ld + r);
-                const uint16_t *h = reinterpret_cast<const uint16_t *>(&w);
-#pragma unroll
-                for (int e = 0; e < 8; ++e) v[q][e] = h[e];
+                w[q] = *reinterpret_cast<const uint4 *>(KC ? src + (size_t)r * ld + k : src + (size_t)k * ld + r);
             } else {
+                uint32_t h[8];
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
                     const int rr = KC ? r : r + e, kk = KC ? k + e : k;
-                    v[q][e] = (rr < rows && kk < kend) ? src[KC ? (size_t)rr * ld + kk : (size_t)kk * ld + rr]
-                                                       : (uint16_t)0;
+                    h[e] = (rr < rows && kk < kend) ? src[KC ? (size_t)rr * ld + kk : (size_t)kk * ld + rr] : 0u;
                 }
+                w[q] = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16),
+                                  h[6] | (h[7] << 16));
             }
         }
     }
@@ -98,15 +98,12 @@ template <bool KC> struct TileLoader {
         for (int q = 0; q < kVec; ++q) {
             if (KC) {
                 const int r = (t >> 2) + q * (THREADS / 4), k = (t & 3) << 3;
-                uint4 w;
-                uint16_t *h = reinterpret_cast<uint16_t *>(&w);
-#pragma unroll
-                for (int e = 0; e < 8; ++e) h[e] = v[q][e];
-                *reinterpret_cast<uint4 *>(&lds[r][k]) = w;
+                *reinterpret_cast<uint4 *>(&lds[r][k]) = w[q];
             } else {
                 const int r = (t & 15) << 3, k = (t >> 4) + q * (THREADS / 16);
+                const uint32_t p[4] = {w[q].x, w[q].y, w[q].z, w[q].w};
 #pragma unroll
-                for (int e = 0; e < 8; ++e) lds[r + e][k] = v[q][e];
+                for (int e = 0; e < 8; ++e) lds[r + e][k] = static_cast<uint16_t>(p[e >> 1] >> (16 * (e & 1)));
             }
         }
     }
