@@ -938,9 +938,10 @@ static int loop_run(sk_gpu_pdlp_session *S, int budget, double *x_out, double *y
         stats->restart_merit[0] = E->P.ndg ? E->h_eval[16] : kkt_worst(&cur);
         stats->restart_merit[1] = E->P.ndg ? E->h_eval[17] : kkt_worst(&avg);
         if (P->verbosity > 1)
-            fprintf(stderr, "it %d prec %s cur %.3e avg %.3e (p %.2e d %.2e g %.2e) eta %.3e omega %.3e\n",
-                    L->iteration, L->prec == SKK_F32 ? "f32" : "f64", kkt_worst(&cur), kkt_worst(&avg),
-                    L->best.primal, L->best.dual, L->best.gap, E->h_state->eta, E->h_state->omega);
+            fprintf(stderr, "it %d prec %s cur %.3e avg %.3e (p %.2e d %.2e g %.2e) eta %.3e omega %.3e "
+                    "merits %.3e %.3e\n", L->iteration, L->prec == SKK_F32 ? "f32" : "f64", kkt_worst(&cur),
+                    kkt_worst(&avg), L->best.primal, L->best.dual, L->best.gap, E->h_state->eta,
+                    E->h_state->omega, stats->restart_merit[0], stats->restart_merit[1]);
         if (!isfinite(kkt_worst(&cur)) && !isfinite(kkt_worst(&avg))) { diverged = 1; break; }
         {
             static const double marks[4] = {1e-3, 1e-4, 1e-5, 1e-6};
@@ -1006,6 +1007,9 @@ static int loop_run(sk_gpu_pdlp_session *S, int budget, double *x_out, double *y
         artificial = (double)L->its_since_restart >= 0.36 * (double)L->iteration;
         L->mu_previous = mu;
         if (!(sufficient || necessary || artificial)) continue;
+        if (P->verbosity > 1)
+            fprintf(stderr, "restart at %d: sufficient %d necessary %d artificial %d to %s, merit %.3e\n",
+                    L->iteration, sufficient, necessary, artificial, average_better ? "average" : "current", mu);
         if ((rc = skk_pdhg_restart(&E->P, L->prec, average_better, P->adaptive_step)))
             return skg_hip_fail("restart launch", rc);
         E->launches += 1;
@@ -1025,6 +1029,7 @@ static int loop_run(sk_gpu_pdlp_session *S, int budget, double *x_out, double *y
         if (cand.pmove > 1e-14 && cand.dmove > 1e-14) {
             const double target = cand.dmove / cand.pmove;
             const double updated = exp(0.5 * log(target) + 0.5 * log(L->omega));
+            if (P->verbosity > 1) fprintf(stderr, "weight %.4e -> %.4e (target %.4e)\n", L->omega, updated, target);
             if (isfinite(updated)) {
                 L->omega = fmin(1e6, fmax(1e-6, updated));
                 if ((rc = skk_pdhg_set_omega(&E->P, L->omega))) return skg_hip_fail("primal weight update", rc);
