@@ -323,7 +323,9 @@ typedef struct sk_gpu_pdlp_params {  /* src/cuda/GpuPdlp.hpp:19-67 PdlpParams */
     int    ruiz_iterations;  /* 100                                         */
     double ruiz_tol;         /* 1e-2                                        */
     int    use_graph;        /* capture each restart window in a hipGraph   */
-    int    verbosity;
+    int    verbosity;        /* stderr: 1 set-up and autotuner timings; 2 also
+                                every window's merits, restart (criterion,
+                                target point, merit) and weight update     */
     double primal_weight;    /* initial primal weight; 0: ||c||/||b|| of the
                                 scaled problem (no reference counterpart:
                                 warm starts pass the family's learned value) */
@@ -478,7 +480,10 @@ int  sk_gpu_pdlp_session_resume(sk_gpu_pdlp_session *session, double *x, double 
  * warm_iterations iterations; if that does not reach eps_optimal, solves
  * again from zero under the session's own limits (time counted from the
  * call).  *used_warm tells which start produced x, y; stats->iterations
- * counts both attempts. */
+ * counts both attempts.  The budget should carry slack over a cold solve
+ * (twice the family's typical cold iteration count, say): a warm start
+ * that is slightly slower than a cold one then still finishes, while one
+ * that cannot converge costs a bounded detour. */
 int  sk_gpu_pdlp_session_solve_fallback(sk_gpu_pdlp_session *session, const double *x0,
                                         const double *y0, int warm_iterations, double *x,
                                         double *y, sk_gpu_pdlp_stats *stats, int *used_warm);
