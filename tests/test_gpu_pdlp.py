@@ -257,3 +257,36 @@ def test_autotuned_session_solves_and_reports_its_choice(prec):
     assert abs(float(model.vec("c") @ x) - ref) <= 1e-6 * (1 + abs(ref))
     rc, x, y, st = gpu_pdlp(model, precision=prec, eps_optimal=1e-7)
     assert st.tuned == 0 and st.spmv_grid == 4096
+
+
+@pytest.mark.parametrize("tile", [256, 512, 1024, 2048])
+def test_every_tiling_matches_the_oracle_on_a_power_law_lp(tile):
+    """The products' tile paths at every entry cap the autotuner can pick:
+    a 20,000 x 40,000 instance of the power-law generator (column degrees up
+    to 2,000, so A^T has rows longer than every cap -- the workgroup-wide
+    long-row path -- next to tiles of one to hundreds of rows, and 1, 2, 4
+    or 8 entries per lane), both grid caps.  The FP64 engine after 8 adaptive
+    iterations against the oracle's FP64 PDLP within summation-order
+    rounding; the mixed engine (FP32 products) within the FP32 drift of 8
+    iterations."""
+    import ctypes as C
+    w = sg.load_workloads()
+    model = sg.Model()
+    q = C.c_double()
+    assert w.skw_zipf_lp(C.byref(model.m), 20_000, 40_000, 1.2, 8.0, 2000, 0.3, 20260826, C.byref(q)) == 0
+    model._generated = True
+    P = dict(max_iterations=8, use_graph=0)
+    rc_o, xo, yo, sto = so.first_order_pdlp(model, sg.default_pdlp_params(precision=sg.PREC_FP64, **P))
+    assert rc_o == 0 and sto.iterations == 8
+    for grid in (1024, 4096):
+        T = dict(autotune=0, tile_nnz=tile, spmv_grid=grid, stream_loads=1 + (grid == 4096))
+        rc, x, y, st = gpu_pdlp(model, precision=sg.PREC_FP64, **P, **T)
+        assert rc == 0 and st.iterations == 8, sg.last_error()
+        assert st.tile_nnz == tile and st.spmv_grid == grid
+        assert abs(st.matrix_norm - sto.matrix_norm) <= 1e-10 * sto.matrix_norm
+        assert np.linalg.norm(x - xo) <= 1e-9 * (1.0 + np.linalg.norm(xo))
+        assert np.linalg.norm(y - yo) <= 1e-9 * (1.0 + np.linalg.norm(yo))
+        rc, xm, ym, st = gpu_pdlp(model, precision=sg.PREC_MIXED, **P, **T)
+        assert rc == 0 and st.iterations == 8 and st.fp32_iterations == 8, sg.last_error()
+        assert np.linalg.norm(xm - xo) <= 1e-4 * (1.0 + np.linalg.norm(xo))
+        assert np.linalg.norm(ym - yo) <= 1e-4 * (1.0 + np.linalg.norm(yo))
