@@ -26,6 +26,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int BM = 128, BN = 128, BK = 32, PAD = 8, THREADS = 256;
+constexpr int VR = BK / 8;   // 16-byte vectors along k in one row of a tile
 
 __device__ __forceinline__ uint16_t f2bf(float f)
 {
@@ -66,7 +67,7 @@ __device__ __forceinline__ void epilogue(const Epi &e, int r, int c, float v)
 // while they run (unpacking it at load time would make every load wait at
 // once).
 template <bool KC> struct TileLoader {
-    static constexpr int kVec = (BM * BK) / (8 * THREADS);   // vectors per thread (2)
+    static constexpr int kVec = (BM * BK) / (8 * THREADS);   // vectors per thread
     uint4 w[kVec];
     __device__ void load(const uint16_t *__restrict__ src, int ld, int r0, int rows, int k0, int kend, bool vec)
     {
@@ -74,8 +75,8 @@ template <bool KC> struct TileLoader {
 #pragma unroll
         for (int q = 0; q < kVec; ++q) {
             // KC: vector along k of one row; else: vector along rows of one k
-            const int r = KC ? r0 + (t >> 2) + q * (THREADS / 4) : r0 + ((t & 15) << 3);
-            const int k = KC ? k0 + ((t & 3) << 3) : k0 + (t >> 4) + q * (THREADS / 16);
+            const int r = KC ? r0 + t / VR + q * (THREADS / VR) : r0 + ((t & 15) << 3);
+            const int k = KC ? k0 + (t % VR) * 8 : k0 + (t >> 4) + q * (THREADS / 16);
             const bool full = KC ? (r < rows && k + 8 <= kend) : (k < kend && r + 8 <= rows);
             if (vec && full) {
                 w[q] = *reinterpret_cast<const uint4 *>(KC ? src + (size_t)r * ld + k : src + (size_t)k * ld + r);
@@ -97,7 +98,7 @@ template <bool KC> struct TileLoader {
 #pragma unroll
         for (int q = 0; q < kVec; ++q) {
             if (KC) {
-                const int r = (t >> 2) + q * (THREADS / 4), k = (t & 3) << 3;
+                const int r = t / VR + q * (THREADS / VR), k = (t % VR) * 8;
                 *reinterpret_cast<uint4 *>(&lds[r][k]) = w[q];
             } else {
                 const int r = (t & 15) << 3, k = (t >> 4) + q * (THREADS / 16);
