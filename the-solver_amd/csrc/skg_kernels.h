@@ -215,8 +215,19 @@ int skk_pdhg_promote(const skk_pdhg *P);
  * from FP64 products at the new anchor.  absorb = 0 keeps the anchor
  * (set-up from hx, hy as uploaded). */
 int skk_pdhg_reanchor(const skk_pdhg *P, int absorb);
-/* Store the final iterate (or the average) as FP64 into x_out/y_out. */
-int skk_pdhg_export(const skk_pdhg *P, int prec, int use_average, double *x_out, double *y_out);
+/* Slot order -> the caller's order: output position o takes slot
+ * col_slot[o] (row_slot[o]) times col_scale[o] (row_scale[o]); device
+ * arrays in the original indexing. */
+typedef struct skk_unpermute {
+    const int *col_slot, *row_slot;
+    const double *col_scale, *row_scale;
+} skk_unpermute;
+/* Store the final iterate (or the average) as FP64 into x_out/y_out, in
+ * the original order and unscaled through U. */
+int skk_pdhg_export(const skk_pdhg *P, int prec, int use_average, const skk_unpermute *U, double *x_out,
+                    double *y_out);
+/* v[0..n) = value on the library stream. */
+int skk_fill_f64(double *v, double value, int n);
 /* ||A||_2 by power iteration on A'A in FP64; v, v2 length n, w length m.
  * Leaves ||A'A v||^2 of the last step in *out (device). */
 int skk_power_norm(const skk_pdhg *P, double *v, double *w, double *v2, int iters, double *out);

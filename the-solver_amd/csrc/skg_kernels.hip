@@ -1353,17 +1353,21 @@ __global__ void k_anchor_data(const double *__restrict__ c, const double *__rest
 template <typename T>
 __global__ void k_export(Iter<T> I, const skk_state *__restrict__ st, int n, int m, int use_avg,
                          double *xo, double *yo, Anchor H, const double *__restrict__ l,
-                         const double *__restrict__ u)
+                         const double *__restrict__ u, skk_unpermute U)
 {
     const int cur = st->cur;
     const int stride = gridDim.x * blockDim.x;
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    // output position o of the original order holds slot U.col_slot[o]
+    // (gathered), unscaled by the column factor: x = D_c x'
+    for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += stride) {
+        const int j = U.col_slot[o];
         const double v = static_cast<double>(use_avg ? I.xavg[j] : I.x(cur)[j]);
-        xo[j] = H.x ? fmin(fmax(H.x[j] + v, l[j]), u[j]) : v;
+        xo[o] = (H.x ? fmin(fmax(H.x[j] + v, l[j]), u[j]) : v) * U.col_scale[o];
     }
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+    for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < m; o += stride) {
+        const int i = U.row_slot[o];
         const double v = static_cast<double>(use_avg ? I.yavg[i] : I.y(cur)[i]);
-        yo[i] = H.x ? H.y[i] + v : v;
+        yo[o] = (H.x ? H.y[i] + v : v) * U.row_scale[o];
     }
 }
 
@@ -2295,17 +2299,25 @@ int skk_pdhg_reanchor(const skk_pdhg *P, int absorb)
     return launch_status();
 }
 
-int skk_pdhg_export(const skk_pdhg *P, int prec, int use_average, double *x_out, double *y_out)
+int skk_pdhg_export(const skk_pdhg *P, int prec, int use_average, const skk_unpermute *U, double *x_out,
+                    double *y_out)
 {
     if (prec == SKK_F32) {
         Iter<float> I(P->it[SKK_F32]);
         hipLaunchKernelGGL(k_export<float>, dim3(P->elem_grid), dim3(kBlock), 0, stream(), I, P->st,
-                           P->n, P->m, use_average, x_out, y_out, anchor_of(*P, true), P->ld, P->ud);
+                           P->n, P->m, use_average, x_out, y_out, anchor_of(*P, true), P->ld, P->ud, *U);
     } else {
         Iter<double> I(P->it[SKK_F64]);
         hipLaunchKernelGGL(k_export<double>, dim3(P->elem_grid), dim3(kBlock), 0, stream(), I, P->st,
-                           P->n, P->m, use_average, x_out, y_out, anchor_of(*P, false), P->ld, P->ud);
+                           P->n, P->m, use_average, x_out, y_out, anchor_of(*P, false), P->ld, P->ud, *U);
     }
+    return launch_status();
+}
+
+int skk_fill_f64(double *v, double value, int n)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 2048)), dim3(kBlock), 0, stream(), v, value, n);
     return launch_status();
 }
 

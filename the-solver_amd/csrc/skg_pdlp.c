@@ -170,6 +170,7 @@ typedef struct engine {
     skk_state *h_state;
     void *graph[2];
     int graph_period[2];
+    skk_unpermute U;             /* slot -> original order on the device */
     long long launches;
     int *rp_host[3];             /* host row offsets of A, A^T, Q (autotune) */
     int tile_nnz, grid_cap, stream, tuned;   /* tiling of the products    */
@@ -507,6 +508,21 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
             (rc = dalloc(E, (void **)&E->pv, 8 * nn)) || (rc = dalloc(E, (void **)&E->pw, 8 * mm)) ||
             (rc = dalloc(E, (void **)&E->pv2, 8 * nn)) || (rc = dalloc(E, (void **)&E->pnorm, 8)))
             goto done;
+        {
+            /* the result leaves the device in the caller's order, unscaled */
+            int *cs, *rs;
+            double *cf2, *rf2;
+            if ((rc = dalloc(E, (void **)&cs, 4 * nn)) || (rc = dalloc(E, (void **)&rs, 4 * mm)) ||
+                (rc = dalloc(E, (void **)&cf2, 8 * nn)) || (rc = dalloc(E, (void **)&rf2, 8 * mm)))
+                goto done;
+            if ((rc = skk_h2d(cs, inv_c, sizeof(int) * (size_t)n)) ||
+                (rc = skk_h2d(rs, inv_r, sizeof(int) * (size_t)m)) ||
+                (rc = skk_h2d(cf2, E->C, sizeof(double) * (size_t)n)) ||
+                (rc = skk_h2d(rf2, E->R, sizeof(double) * (size_t)m))) {
+                rc = skg_hip_fail("upload", rc); goto done;
+            }
+            E->U.col_slot = cs; E->U.row_slot = rs; E->U.col_scale = cf2; E->U.row_scale = rf2;
+        }
         if (P->restart_rule == SK_GPU_RESTART_NDG) {
             Q->ndg = 1;
             if ((rc = dalloc(E, (void **)&Q->nx, 8 * nn)) || (rc = dalloc(E, (void **)&Q->ng, 8 * nn)) ||
@@ -808,18 +824,23 @@ static int session_start(sk_gpu_pdlp_session *S, int prec, const double *x0, con
     if (prec == SKK_F32 && E->P.anchored) {
         /* anchored FP32 set: the start point (warm or 0) is the first anchor,
            in FP64; the corrections start at 0 */
-        double *tx = (double *)calloc((size_t)cols + 1, sizeof(double));
-        double *ty = (double *)calloc((size_t)rows + 1, sizeof(double));
-        if (!tx || !ty) { free(tx); free(ty); return skg_fail("out of host memory"); }
-        for (k = 0; k < cols && x0; ++k) {
-            const int col = E->perm_c[k];
-            tx[k] = x0[col] / E->C[col];
+        if (x0 || y0) {
+            double *tx = (double *)calloc((size_t)cols + 1, sizeof(double));
+            double *ty = (double *)calloc((size_t)rows + 1, sizeof(double));
+            if (!tx || !ty) { free(tx); free(ty); return skg_fail("out of host memory"); }
+            for (k = 0; k < cols && x0; ++k) {
+                const int col = E->perm_c[k];
+                tx[k] = x0[col] / E->C[col];
+            }
+            for (k = 0; k < rows && y0; ++k) ty[k] = y0[E->perm_r[k]] / E->R[E->perm_r[k]];
+            rc = skk_h2d(E->P.hx, tx, sizeof(double) * (size_t)cols);
+            if (!rc) rc = skk_h2d(E->P.hy, ty, sizeof(double) * (size_t)rows);
+            free(tx); free(ty);
+        } else {
+            rc = skk_fill_f64(E->P.hx, 0.0, cols);
+            if (!rc) rc = skk_fill_f64(E->P.hy, 0.0, rows);
         }
-        for (k = 0; k < rows && y0; ++k) ty[k] = y0[E->perm_r[k]] / E->R[E->perm_r[k]];
-        rc = skk_h2d(E->P.hx, tx, sizeof(double) * (size_t)cols);
-        if (!rc) rc = skk_h2d(E->P.hy, ty, sizeof(double) * (size_t)rows);
-        free(tx); free(ty);
-        if (rc) return skg_hip_fail("warm start upload", rc);
+        if (rc) return skg_hip_fail("start point upload", rc);
         if ((rc = skk_pdhg_init(&E->P, prec, S->eta_global, omega, S->P.eta_min_factor / S->norm,
                                 S->P.eta_max_factor / S->norm, 0, 0)) ||
             (rc = skk_pdhg_reanchor(&E->P, 0)))
@@ -1037,23 +1058,16 @@ static int loop_run(sk_gpu_pdlp_session *S, int budget, double *x_out, double *y
             }
         }
     }
-    if ((rc = skk_pdhg_export(&E->P, L->prec, L->use_avg_final, E->pv, E->pw)) ||
+    if ((rc = skk_pdhg_export(&E->P, L->prec, L->use_avg_final, &E->U, E->pv, E->pw)) ||
         (rc = skk_d2h_async(E->h_state, E->P.st, sizeof(skk_state))) || (rc = skk_sync()))
         return skg_hip_fail("export", rc);
     L->loop_seconds += skg_now() - t0;
-    {
-        double *xs = (double *)malloc(sizeof(double) * (size_t)(cols + 1));
-        double *ys = (double *)malloc(sizeof(double) * (size_t)(rows + 1));
-        if (!xs || !ys) { free(xs); free(ys); return skg_fail("out of host memory"); }
-        if ((rc = skk_d2h(xs, E->pv, sizeof(double) * (size_t)cols)) ||
-            (rc = skk_d2h(ys, E->pw, sizeof(double) * (size_t)rows))) {
-            free(xs); free(ys); return skg_hip_fail("result copy", rc);
-        }
-        stats->host_syncs += 2;
-        for (k = 0; k < cols; ++k) x_out[E->perm_c[k]] = xs[k] * E->C[E->perm_c[k]];
-        for (k = 0; k < rows; ++k) y_out[E->perm_r[k]] = ys[k] * E->R[E->perm_r[k]];
-        free(xs); free(ys);
-    }
+    /* the export kernel wrote the caller's order, unscaled (x = D_c x',
+       y = D_r y'): the copies land in the output arrays as they are */
+    if ((rc = skk_d2h(x_out, E->pv, sizeof(double) * (size_t)cols)) ||
+        (rc = skk_d2h(y_out, E->pw, sizeof(double) * (size_t)rows)))
+        return skg_hip_fail("result copy", rc);
+    stats->host_syncs += 2;
     stats->loop_seconds = L->loop_seconds;
     stats->iterations = L->iteration;
     stats->primal_objective = L->best.pobj;
