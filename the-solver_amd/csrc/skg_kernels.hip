@@ -162,23 +162,40 @@ __device__ double combine(const double *part, int count, bool is_max)
 // workgroup tree as combine() (so the totals are bit-identical to it), but
 // every partial of every slot is loaded up front, so the loads of all slots
 // are in flight together instead of one slot's after another's.
-template <int S, int kPer>
-__device__ void combine_sums(const double *const (&part)[S], const int (&count)[S], double (&out)[S])
+// `mid` runs once every thread's partials have arrived, before the
+// workgroup tree: work placed there waits for no load of its own.
+struct NoMid {
+    __device__ void operator()() const {}
+};
+
+template <int S, int kPer, class Mid = NoMid>
+__device__ void combine_sums(const double *const (&part)[S], const int (&count)[S], double (&out)[S],
+                             const Mid &mid = Mid{})
 {
     const int tid = threadIdx.x;
     bool fits = true;
 #pragma unroll
     for (int s = 0; s < S; ++s) fits = fits && count[s] <= kPer * kBlock;
     if (!fits) {
+        mid();
 #pragma unroll
         for (int s = 0; s < S; ++s) out[s] = combine(part[s], count[s], false);
         return;
     }
+    // Branch-free loads (an index past the end reads the slot's first
+    // partial, which is then not used): a conditional load per element
+    // lets the compiler consume one slot's values before it requests the
+    // next slot's, one memory round trip per slot.
     double v[S][kPer];
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) v[s][k] = tid + k * kBlock < count[s] ? part[s][tid + k * kBlock] : 0.0;
+        for (int k = 0; k < kPer; ++k) {
+            const int i = tid + k * kBlock;
+            const double t = part[s][min(i, max(count[s] - 1, 0))];
+            v[s][k] = i < count[s] ? t : 0.0;
+        }
+    __builtin_amdgcn_sched_barrier(0);   // every load requested before the first is used
     double acc[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -196,6 +213,7 @@ __device__ void combine_sums(const double *const (&part)[S], const int (&count)[
             if (k >= next && tid + k * kBlock < count[s]) a0 += v[s][k];
         acc[s] = (a0 + a1) + (a2 + a3);
     }
+    mid();
     block_reduce<S>(acc, out);
 }
 
@@ -761,8 +779,9 @@ template <typename T, typename AT> struct EpiProbe {
 // `in` after the decision.
 // The step rule's factors after `updates` decisions: (1 - (k+1)^-0.3,
 // 1 + (k+1)^-0.6) with k = updates + 1.  The decision kernels have one
-// thread of a second wave compute them while the others reduce the probe
-// partials, so the two pow calls are off the decision's critical path.
+// thread of a second wave compute them from the state loaded together with
+// the probe partials, once those have arrived: the two pow calls then add
+// no memory wait of their own.
 struct StepFactors {
     double shrink, grow;
 };
@@ -803,15 +822,15 @@ __device__ void decide(const skk_state &in, double dx2, double dq, double dy2, d
 // The decision's four sums, reduced by one workgroup in a fixed order.
 // kPer: partials per thread held in registers (counts above kPer * kBlock
 // fall back to one slot at a time).
-template <int kPer>
+template <int kPer, class Mid>
 __device__ void probe_sums(const double *part, int stride, int dx_blocks, int dq_blocks, int dy_blocks,
-                           double &dx2, double &dq, double &dy2, double &dya)
+                           double &dx2, double &dq, double &dy2, double &dya, const Mid &mid)
 {
     const double *const slots[4] = {part + (size_t)R_DX * stride, part + (size_t)R_DQ * stride,
                                     part + (size_t)R_DY * stride, part + (size_t)R_INTER * stride};
     const int counts[4] = {dx_blocks, dq_blocks, dy_blocks, dy_blocks};
     double out[4];
-    combine_sums<4, kPer>(slots, counts, out);
+    combine_sums<4, kPer>(slots, counts, out, mid);
     dx2 = out[0];
     dq = dq_blocks > 0 ? out[1] : 0.0;
     dy2 = out[2];
@@ -824,11 +843,19 @@ __global__ __launch_bounds__(kBlock) void k_probe_finalize(skk_state *st, const 
 {
     __shared__ StepFactors s_f;
     double dx2, dq, dy2, dya;
-    if (threadIdx.x == 64) s_f = step_factors(st->updates);   // visible after the reduction's barriers
-    probe_sums<SKK_SPMV_GRID / kBlock>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya);
+    // the state is loaded with the partials (one request per wave), so
+    // neither the factors nor the decision wait for a load of their own;
+    // the opaque zero makes these vector loads (scalar loads would be
+    // waited for before the partials are requested)
+    int zero;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
+    const skk_state in = st[zero];
+    probe_sums<SKK_SPMV_GRID / kBlock>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya, [&] {
+        if (threadIdx.x == 64) s_f = step_factors(in.updates);   // visible after the tree's barriers
+    });
     if (threadIdx.x != 0) return;
     skk_state next;
-    decide(*st, dx2, dq, dy2, dya, s_f, next);
+    decide(in, dx2, dq, dy2, dya, s_f, next);
     *st = next;
 }
 
@@ -844,13 +871,21 @@ __global__ __launch_bounds__(kBlock) void k_dual_decide(Iter<T> I, Data<T> D, co
     __shared__ skk_state s_next;
     __shared__ StepFactors s_f;
     double dx2, dq, dy2, dya;
-    if (threadIdx.x == 64) s_f = step_factors(snap->updates);   // visible after the reduction's barriers
+    int zero;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
+    const skk_state in = snap[zero];   // vector loads, in flight with the partials (k_probe_finalize)
     // fused only while the products have at most SKK_FUSE_PARTIALS
     // workgroups together
-    probe_sums<SKK_FUSE_PER>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya);
+    probe_sums<SKK_FUSE_PER>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya, [&] {
+        if (threadIdx.x == 64) s_f = step_factors(in.updates);   // visible after the tree's barriers
+    });
     if (threadIdx.x == 0) {
-        decide(*snap, dx2, dq, dy2, dya, s_f, s_next);
-        if (blockIdx.x == 0) *st = s_next;
+        // decided in registers, then published (a decision written straight
+        // into LDS would keep `in` in memory too, stored right after its loads)
+        skk_state next;
+        decide(in, dx2, dq, dy2, dya, s_f, next);
+        s_next = next;
+        if (blockIdx.x == 0) *st = next;
     }
     __syncthreads();
     const int cur = s_next.cur, pending = s_next.pending;
