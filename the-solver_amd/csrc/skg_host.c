@@ -437,24 +437,41 @@ int skg_plan_tiles(const int *rp, int rows, int tile_nnz, int **tile_row)
     return nt;
 }
 
+/* Device tile descriptors of the nt tiles whose first rows are tile_row[]:
+ * pairs (first row, first entry), nt + 1 of them, so that one load gives a
+ * workgroup both ends of its tile's rows and entries. */
+static int upload_tiles(const int *rp, const int *tile_row, int nt, int **dev)
+{
+    int rc, t, *h = (int *)malloc(sizeof(int) * 2 * ((size_t)nt + 1));
+    *dev = NULL;
+    if (!h) return skg_fail("out of host memory");
+    for (t = 0; t <= nt; ++t) {
+        h[2 * t] = tile_row[t];
+        h[2 * t + 1] = rp[tile_row[t]];
+    }
+    if ((rc = skk_malloc((void **)dev, sizeof(int) * 2 * ((size_t)nt + 1))) ||
+        (rc = skk_h2d(*dev, h, sizeof(int) * 2 * ((size_t)nt + 1)))) {
+        free(h); skk_free(*dev); *dev = NULL;
+        return skg_hip_fail("tile upload", rc);
+    }
+    free(h);
+    return 0;
+}
+
 /* Replace a device CSR's row tiles: entry cap tile_nnz, grid cap grid_cap
  * (<= SKK_SPMV_GRID), non-temporal matrix loads when stream is set.  rp is
  * the host copy of the row offsets the matrix was uploaded with. */
 int dcsr_retile(dcsr *d, const int *rp, int tile_nnz, int grid_cap, int stream)
 {
-    int rc, nt, *tiles = NULL, *dt = NULL;
+    int nt, *tiles = NULL, *dt = NULL;
     if (grid_cap < 1 || grid_cap > SKK_SPMV_GRID) return skg_fail("invalid grid cap %d", grid_cap);
     nt = skg_plan_tiles(rp, d->k.rows, tile_nnz, &tiles);
     if (nt < 0) return skg_fail("invalid tile size %d or out of host memory", tile_nnz);
-    if ((rc = skk_malloc((void **)&dt, sizeof(int) * ((size_t)nt + 1))) ||
-        (rc = skk_h2d(dt, tiles, sizeof(int) * ((size_t)nt + 1)))) {
-        free(tiles); skk_free(dt);
-        return skg_hip_fail("tile upload", rc);
-    }
+    if (upload_tiles(rp, tiles, nt, &dt)) { free(tiles); return -1; }
     free(tiles);
     skk_free(d->tiles);
     d->tiles = dt;
-    d->k.tile_row = dt;
+    d->k.tiles = dt;
     d->k.ntiles = nt;
     d->k.grid = nt < grid_cap ? nt : grid_cap;
     d->k.stream = stream;
@@ -480,14 +497,13 @@ int dcsr_upload(dcsr *d, const hcsr *a, int with_f32)
     d->k.stream = (long long)a->nnz * 12 > SKK_STREAM_BYTES;
     if ((rc = skk_malloc((void **)&d->rp, sizeof(int) * ((size_t)a->rows + 1))) ||
         (rc = skk_malloc((void **)&d->ci, sizeof(int) * (size_t)(a->nnz ? a->nnz : 1))) ||
-        (rc = skk_malloc((void **)&d->vd, sizeof(double) * (size_t)(a->nnz ? a->nnz : 1))) ||
-        (rc = skk_malloc((void **)&d->tiles, sizeof(int) * ((size_t)nt + 1))))
+        (rc = skk_malloc((void **)&d->vd, sizeof(double) * (size_t)(a->nnz ? a->nnz : 1))))
         goto fail;
     if ((rc = skk_h2d(d->rp, a->p, sizeof(int) * ((size_t)a->rows + 1))) ||
         (rc = skk_h2d(d->ci, a->j, sizeof(int) * (size_t)a->nnz)) ||
-        (rc = skk_h2d(d->vd, a->v, sizeof(double) * (size_t)a->nnz)) ||
-        (rc = skk_h2d(d->tiles, tiles, sizeof(int) * ((size_t)nt + 1))))
+        (rc = skk_h2d(d->vd, a->v, sizeof(double) * (size_t)a->nnz)))
         goto fail;
+    if (upload_tiles(a->p, tiles, nt, &d->tiles)) { free(tiles); dcsr_free(d); return -1; }
     if (with_f32) {
         float *tmp = (float *)malloc(sizeof(float) * (size_t)(a->nnz ? a->nnz : 1));
         int k;
@@ -499,7 +515,7 @@ int dcsr_upload(dcsr *d, const hcsr *a, int with_f32)
         free(tmp);
     }
     free(tiles);
-    d->k.rp = d->rp; d->k.ci = d->ci; d->k.vd = d->vd; d->k.vf = d->vf; d->k.tile_row = d->tiles;
+    d->k.rp = d->rp; d->k.ci = d->ci; d->k.vd = d->vd; d->k.vf = d->vf; d->k.tiles = d->tiles;
     return 0;
 fail:
     free(tiles);

@@ -53,7 +53,8 @@ typedef struct skk_bins {
     skk_bin b[SKK_MAX_BINS];
 } skk_bins;
 
-/* CSR-stream row tiles: tile t covers rows [tile_row[t], tile_row[t+1]),
+/* CSR-stream row tiles: tile t covers rows [tiles[2t], tiles[2t+2]) and
+ * entries [tiles[2t+1], tiles[2t+3]),
  * consecutive rows holding at most a per-matrix entry cap together (a tile
  * of one row longer than SKK_TILE_NNZ is reduced by the whole workgroup)
  * and at most SKK_TILE_ROWS rows.  The cap is SKK_TILE_NNZ (the LDS
@@ -100,7 +101,7 @@ typedef struct skk_csr {
     const int *rp, *ci;
     const float *vf;         /* FP32 values (may be NULL) */
     const double *vd;        /* FP64 values               */
-    const int *tile_row;     /* ntiles + 1 row offsets    */
+    const int *tiles;        /* ntiles + 1 pairs (first row, first entry) */
     int ntiles, grid;
     int stream;              /* 1: non-temporal matrix loads */
 } skk_csr;

@@ -356,15 +356,17 @@ __device__ __forceinline__ void tile_products(const int *__restrict__ ci, const 
 // workgroup straight from memory; otherwise the tile's entries are
 // multiplied by consecutive lanes into LDS (`prod`) and each row is summed
 // from there.  Every thread of the workgroup calls this for the same tile.
+// The tile's descriptor gives its rows [r0, r1) and entries [p0, p1)
+// together, so its entries are requested one memory round trip after the
+// descriptor rather than two.
 template <typename VT, typename XT, typename AT, bool NT, class Epi>
 __device__ __forceinline__ void spmv_tile(const int *__restrict__ rp, const int *__restrict__ ci,
                                           const VT *__restrict__ val, const XT *__restrict__ x, int r0,
-                                          int r1, const Epi &e, double *part, AT *prod)
+                                          int r1, int p0, int p1, const Epi &e, double *part, AT *prod)
 {
     using In = decltype(epi_load(e, 0, 0));
     constexpr int kPer = SKK_TILE_NNZ / kBlock;
     const int tid = threadIdx.x;
-    const int p0 = rp[r0], p1 = rp[r1];
     const int nr = r1 - r0;
     if (p1 - p0 > SKK_TILE_NNZ) {
         // one long row: workgroup-strided partial sums, wave and LDS combine
@@ -453,15 +455,17 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
                                                  const int *__restrict__ ci,
                                                  const VT *__restrict__ val, const XT *x0,
                                                  const XT *x1, const skk_state *sel,
-                                                 const int *__restrict__ tile_row, int ntiles, Epi epi)
+                                                 const int *__restrict__ tiles, int ntiles, Epi epi)
 {
     __shared__ AT prod[SKK_TILE_NNZ];
     Epi e = epi;
     e.begin(epi);
     const XT *__restrict__ x = (sel && (sel->cur ^ 1)) ? x1 : x0;
     double part[Epi::kSlots > 0 ? Epi::kSlots : 1] = {};
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x)
-        spmv_tile<VT, XT, AT, NT>(rp, ci, val, x, tile_row[t], tile_row[t + 1], e, part, prod);
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int *d = tiles + 2 * t;
+        spmv_tile<VT, XT, AT, NT>(rp, ci, val, x, d[0], d[2], d[1], d[3], e, part, prod);
+    }
     e.end(part);
 }
 
@@ -470,7 +474,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
 // dealt round robin over the workgroups; each epilogue keeps its own
 // partials.
 struct TileSet {
-    const int *rp, *ci, *tile_row;
+    const int *rp, *ci, *tiles;
     int ntiles;
 };
 
@@ -489,11 +493,13 @@ __global__ __launch_bounds__(kBlock) void k_spmv_pair(TileSet m1, const VT *__re
     double part2[Epi2::kSlots > 0 ? Epi2::kSlots : 1] = {};
     const int total = m1.ntiles + m2.ntiles;
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        if (t < m1.ntiles)
-            spmv_tile<VT, XT, AT, NT1>(m1.rp, m1.ci, v1, x, m1.tile_row[t], m1.tile_row[t + 1], e1, part1, prod);
-        else
-            spmv_tile<VT, XT, AT, NT2>(m2.rp, m2.ci, v2, x, m2.tile_row[t - m1.ntiles],
-                                       m2.tile_row[t - m1.ntiles + 1], e2, part2, prod);
+        if (t < m1.ntiles) {
+            const int *d = m1.tiles + 2 * t;
+            spmv_tile<VT, XT, AT, NT1>(m1.rp, m1.ci, v1, x, d[0], d[2], d[1], d[3], e1, part1, prod);
+        } else {
+            const int *d = m2.tiles + 2 * (t - m1.ntiles);
+            spmv_tile<VT, XT, AT, NT2>(m2.rp, m2.ci, v2, x, d[0], d[2], d[1], d[3], e2, part2, prod);
+        }
     }
     e1.end(part1);
     e2.end(part2);
@@ -514,10 +520,10 @@ int launch_spmv(const skk_csr &A, const VT *vals, const XT *x0, const XT *x1,
     if (A.ntiles == 0) return 0;
     if (A.stream)
         hipLaunchKernelGGL((k_spmv<VT, XT, AT, Epi, true>), dim3(A.grid), dim3(kBlock), 0,
-                           stream(), A.rp, A.ci, vals, x0, x1, sel, A.tile_row, A.ntiles, epi);
+                           stream(), A.rp, A.ci, vals, x0, x1, sel, A.tiles, A.ntiles, epi);
     else
         hipLaunchKernelGGL((k_spmv<VT, XT, AT, Epi, false>), dim3(A.grid), dim3(kBlock), 0,
-                           stream(), A.rp, A.ci, vals, x0, x1, sel, A.tile_row, A.ntiles, epi);
+                           stream(), A.rp, A.ci, vals, x0, x1, sel, A.tiles, A.ntiles, epi);
     return launch_status();
 }
 
@@ -1701,7 +1707,7 @@ template <typename T> int qa_probe(const skk_pdhg &P, const Iter<T> &I, bool sna
     EpiProbe<T, T> pr{};
     pr.I = I; pr.st = P.st; pr.part = P.part; pr.stride = P.part_capacity;
     pr.snap = snapshot ? P.snap : nullptr;
-    const TileSet q{P.Q.rp, P.Q.ci, P.Q.tile_row, P.Q.ntiles}, a{P.A.rp, P.A.ci, P.A.tile_row, P.A.ntiles};
+    const TileSet q{P.Q.rp, P.Q.ci, P.Q.tiles, P.Q.ntiles}, a{P.A.rp, P.A.ci, P.A.tiles, P.A.ntiles};
     const T *vq = vals_of<T>(P.Q), *va = vals_of<T>(P.A);
     using EQ = EpiQProbe<T, T>;
     using EA = EpiProbe<T, T>;
