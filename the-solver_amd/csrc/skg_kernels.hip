@@ -450,21 +450,38 @@ __device__ __forceinline__ void spmv_tile(const int *__restrict__ rp, const int 
     __syncthreads();   // prod is reused by the next tile
 }
 
+// The gathered vector is x1 when *cur is 0 and x0 when it is 1: `cur` points
+// at the device state's buffer index (candidate x1 = X[cur ^ 1] of the pair
+// x0 = X[0], x1 = X[1]), or at the tile table's first entry (always 0) with
+// x1 = x0 when there is nothing to select.  The first tile's descriptor, the
+// selector and the epilogue's state are requested before any of them is
+// waited for (the grid never exceeds the tile count, so tile blockIdx.x
+// exists): a workgroup's prologue costs one memory round trip, not three.
 template <typename VT, typename XT, typename AT, class Epi, bool NT>
 __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
                                                  const int *__restrict__ ci,
                                                  const VT *__restrict__ val, const XT *x0,
-                                                 const XT *x1, const skk_state *sel,
+                                                 const XT *x1, const int *__restrict__ cur,
                                                  const int *__restrict__ tiles, int ntiles, Epi epi)
 {
     __shared__ AT prod[SKK_TILE_NNZ];
+    int t = blockIdx.x;
+    const int *d = tiles + 2 * t;
+    int r0 = d[0], p0 = d[1], r1 = d[2], p1 = d[3];
     Epi e = epi;
     e.begin(epi);
-    const XT *__restrict__ x = (sel && (sel->cur ^ 1)) ? x1 : x0;
+    const XT *__restrict__ x = (*cur ^ 1) ? x1 : x0;
     double part[Epi::kSlots > 0 ? Epi::kSlots : 1] = {};
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int *d = tiles + 2 * t;
-        spmv_tile<VT, XT, AT, NT>(rp, ci, val, x, d[0], d[2], d[1], d[3], e, part, prod);
+    for (;;) {
+        // the next tile's descriptor (clamped to the last) is in flight
+        // while this tile is processed
+        const int tn = t + gridDim.x;
+        d = tiles + 2 * min(tn, ntiles - 1);
+        const int n0 = d[0], n1 = d[1], n2 = d[2], n3 = d[3];
+        spmv_tile<VT, XT, AT, NT>(rp, ci, val, x, r0, r1, p0, p1, e, part, prod);
+        if (tn >= ntiles) break;
+        t = tn;
+        r0 = n0; p0 = n1; r1 = n2; p1 = n3;
     }
     e.end(part);
 }
@@ -481,25 +498,31 @@ struct TileSet {
 template <typename VT, typename XT, typename AT, class Epi1, class Epi2, bool NT1, bool NT2>
 __global__ __launch_bounds__(kBlock) void k_spmv_pair(TileSet m1, const VT *__restrict__ v1, TileSet m2,
                                                       const VT *__restrict__ v2, const XT *x0, const XT *x1,
-                                                      const skk_state *sel, Epi1 epi1, Epi2 epi2)
+                                                      const int *__restrict__ cur, Epi1 epi1, Epi2 epi2)
 {
     __shared__ AT prod[SKK_TILE_NNZ];
+    const int total = m1.ntiles + m2.ntiles;
+    // as in k_spmv: the first descriptor, the selector and both states
+    // requested together
+    int t = blockIdx.x;
+    const int *d = t < m1.ntiles ? m1.tiles + 2 * t : m2.tiles + 2 * (t - m1.ntiles);
+    int r0 = d[0], p0 = d[1], r1 = d[2], p1 = d[3];
+    const XT *__restrict__ x = (*cur ^ 1) ? x1 : x0;
     Epi1 e1 = epi1;
     e1.begin(epi1);
     Epi2 e2 = epi2;
     e2.begin(epi2);
-    const XT *__restrict__ x = (sel && (sel->cur ^ 1)) ? x1 : x0;
     double part1[Epi1::kSlots > 0 ? Epi1::kSlots : 1] = {};
     double part2[Epi2::kSlots > 0 ? Epi2::kSlots : 1] = {};
-    const int total = m1.ntiles + m2.ntiles;
-    for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        if (t < m1.ntiles) {
-            const int *d = m1.tiles + 2 * t;
-            spmv_tile<VT, XT, AT, NT1>(m1.rp, m1.ci, v1, x, d[0], d[2], d[1], d[3], e1, part1, prod);
-        } else {
-            const int *d = m2.tiles + 2 * (t - m1.ntiles);
-            spmv_tile<VT, XT, AT, NT2>(m2.rp, m2.ci, v2, x, d[0], d[2], d[1], d[3], e2, part2, prod);
-        }
+    for (;;) {
+        const int tn = t + gridDim.x, tc = min(tn, total - 1);
+        d = tc < m1.ntiles ? m1.tiles + 2 * tc : m2.tiles + 2 * (tc - m1.ntiles);
+        const int n0 = d[0], n1 = d[1], n2 = d[2], n3 = d[3];
+        if (t < m1.ntiles) spmv_tile<VT, XT, AT, NT1>(m1.rp, m1.ci, v1, x, r0, r1, p0, p1, e1, part1, prod);
+        else spmv_tile<VT, XT, AT, NT2>(m2.rp, m2.ci, v2, x, r0, r1, p0, p1, e2, part2, prod);
+        if (tn >= total) break;
+        t = tn;
+        r0 = n0; p0 = n1; r1 = n2; p1 = n3;
     }
     e1.end(part1);
     e2.end(part2);
@@ -518,12 +541,16 @@ int launch_spmv(const skk_csr &A, const VT *vals, const XT *x0, const XT *x1,
                 const skk_state *sel, const Epi &epi)
 {
     if (A.ntiles == 0) return 0;
+    // without a state to select by, the tile table's first entry (0) picks
+    // x1, which is then x0
+    const int *cur = sel ? &sel->cur : A.tiles;
+    if (!sel) x1 = x0;
     if (A.stream)
         hipLaunchKernelGGL((k_spmv<VT, XT, AT, Epi, true>), dim3(A.grid), dim3(kBlock), 0,
-                           stream(), A.rp, A.ci, vals, x0, x1, sel, A.tiles, A.ntiles, epi);
+                           stream(), A.rp, A.ci, vals, x0, x1, cur, A.tiles, A.ntiles, epi);
     else
         hipLaunchKernelGGL((k_spmv<VT, XT, AT, Epi, false>), dim3(A.grid), dim3(kBlock), 0,
-                           stream(), A.rp, A.ci, vals, x0, x1, sel, A.tiles, A.ntiles, epi);
+                           stream(), A.rp, A.ci, vals, x0, x1, cur, A.tiles, A.ntiles, epi);
     return launch_status();
 }
 
@@ -1713,13 +1740,13 @@ template <typename T> int qa_probe(const skk_pdhg &P, const Iter<T> &I, bool sna
     using EA = EpiProbe<T, T>;
     const dim3 g(P.probe_grid), b(kBlock);
     if (P.Q.stream && P.A.stream)
-        hipLaunchKernelGGL((k_spmv_pair<T, T, T, EQ, EA, true, true>), g, b, 0, stream(), q, vq, a, va, I.X[0], I.X[1], P.st, eq, pr);
+        hipLaunchKernelGGL((k_spmv_pair<T, T, T, EQ, EA, true, true>), g, b, 0, stream(), q, vq, a, va, I.X[0], I.X[1], &P.st->cur, eq, pr);
     else if (P.Q.stream)
-        hipLaunchKernelGGL((k_spmv_pair<T, T, T, EQ, EA, true, false>), g, b, 0, stream(), q, vq, a, va, I.X[0], I.X[1], P.st, eq, pr);
+        hipLaunchKernelGGL((k_spmv_pair<T, T, T, EQ, EA, true, false>), g, b, 0, stream(), q, vq, a, va, I.X[0], I.X[1], &P.st->cur, eq, pr);
     else if (P.A.stream)
-        hipLaunchKernelGGL((k_spmv_pair<T, T, T, EQ, EA, false, true>), g, b, 0, stream(), q, vq, a, va, I.X[0], I.X[1], P.st, eq, pr);
+        hipLaunchKernelGGL((k_spmv_pair<T, T, T, EQ, EA, false, true>), g, b, 0, stream(), q, vq, a, va, I.X[0], I.X[1], &P.st->cur, eq, pr);
     else
-        hipLaunchKernelGGL((k_spmv_pair<T, T, T, EQ, EA, false, false>), g, b, 0, stream(), q, vq, a, va, I.X[0], I.X[1], P.st, eq, pr);
+        hipLaunchKernelGGL((k_spmv_pair<T, T, T, EQ, EA, false, false>), g, b, 0, stream(), q, vq, a, va, I.X[0], I.X[1], &P.st->cur, eq, pr);
     return launch_status();
 }
 
