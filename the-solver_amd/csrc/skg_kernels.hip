@@ -883,9 +883,16 @@ __global__ __launch_bounds__(kBlock) void k_probe_finalize(skk_state *st, const 
     int zero;
     asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
     const skk_state in = st[zero];
-    probe_sums<SKK_SPMV_GRID / kBlock>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya, [&] {
+    auto factors = [&] {
         if (threadIdx.x == 64) s_f = step_factors(in.updates);   // visible after the tree's barriers
-    });
+    };
+    // as few partials per thread as the counts need (loads past a count are
+    // issued too): the sums are the same for any kPer that covers them
+    const int most = max(max(dx_blocks, dq_blocks), dy_blocks);
+    if (most <= kBlock) probe_sums<1>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya, factors);
+    else if (most <= 4 * kBlock)
+        probe_sums<4>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya, factors);
+    else probe_sums<SKK_SPMV_GRID / kBlock>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya, factors);
     if (threadIdx.x != 0) return;
     skk_state next;
     decide(in, dx2, dq, dy2, dya, s_f, next);
@@ -909,9 +916,12 @@ __global__ __launch_bounds__(kBlock) void k_dual_decide(Iter<T> I, Data<T> D, co
     const skk_state in = snap[zero];   // vector loads, in flight with the partials (k_probe_finalize)
     // fused only while the products have at most SKK_FUSE_PARTIALS
     // workgroups together
-    probe_sums<SKK_FUSE_PER>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya, [&] {
+    auto factors = [&] {
         if (threadIdx.x == 64) s_f = step_factors(in.updates);   // visible after the tree's barriers
-    });
+    };
+    if (max(max(dx_blocks, dq_blocks), dy_blocks) <= kBlock)
+        probe_sums<1>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya, factors);
+    else probe_sums<SKK_FUSE_PER>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya, factors);
     if (threadIdx.x == 0) {
         // decided in registers, then published (a decision written straight
         // into LDS would keep `in` in memory too, stored right after its loads)
