@@ -619,10 +619,16 @@ template <> Data<double> data_of<double>(const skk_pdhg &P) { return {P.cd, P.ld
 // three cases of the prox (below, between, above the bounds) give
 // v - sigma rl, -yc and v - sigma ru, so no term of the anchor's size is
 // added and then cancelled in FP32.  Infinite bounds never bind.
+// The dual prox from row i's bounds (and, anchored, its centre yc).
+template <typename T> __device__ __forceinline__ T dual_prox_of(T v, T sigma, T rl, T ru, T yc, bool anchored)
+{
+    if (anchored) return fmin(fmax(-yc, v - sigma * ru), v - sigma * rl);
+    return v - sigma * clampv(v / sigma, rl, ru);
+}
+
 template <typename T> __device__ __forceinline__ T dual_prox(T v, T sigma, const Data<T> &D, int i)
 {
-    if (D.yc) return fmin(fmax(-D.yc[i], v - sigma * D.ru[i]), v - sigma * D.rl[i]);
-    return v - sigma * clampv(v / sigma, D.rl[i], D.ru[i]);
+    return dual_prox_of(v, sigma, D.rl[i], D.ru[i], D.yc ? D.yc[i] : T(0), D.yc != nullptr);
 }
 
 // Anchor of an FP32 set in evaluations and exports (null x: none).
@@ -914,6 +920,18 @@ __global__ __launch_bounds__(kBlock) void k_dual_decide(Iter<T> I, Data<T> D, co
     int zero;
     asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
     const skk_state in = snap[zero];   // vector loads, in flight with the partials (k_probe_finalize)
+    // This thread's first row with every buffer the decision can pick (y of
+    // either slot, A x of all three), its bounds and running sum, requested
+    // with the partials: the prox after the decision waits for no load.
+    // (A row index past m reads row 0 -- every buffer holds at least one
+    // element -- and is not stored.)
+    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ic = max(min(i0, m - 1), 0);
+    const T py0 = I.Y[0][ic], py1 = I.Y[1][ic];
+    const T pa0 = I.AX[0][ic], pa1 = I.AX[1][ic], pa2 = I.AX[2][ic];
+    const T prl = D.rl[ic], pru = D.ru[ic];
+    const T pyc = (D.yc ? D.yc : D.rl)[ic];
+    const double psum = I.ysum.get(ic);
     // fused only while the products have at most SKK_FUSE_PARTIALS
     // workgroups together
     auto factors = [&] {
@@ -937,8 +955,21 @@ __global__ __launch_bounds__(kBlock) void k_dual_decide(Iter<T> I, Data<T> D, co
     const T *__restrict__ ax = I.ax(s_next.a0);
     const T *__restrict__ axp = I.ax(s_next.a1);
     T *__restrict__ yc = I.y(cur ^ 1);
+    if (i0 < m) {
+        // the same arithmetic as the loop below, on the prefetched values
+        const T yi = cur ? py1 : py0;
+        const int a0 = s_next.a0, a1 = s_next.a1;
+        const T axv = a0 == 0 ? pa0 : (a0 == 1 ? pa1 : pa2);
+        const T axpv = a1 == 0 ? pa0 : (a1 == 1 ? pa1 : pa2);
+        const T v = yi + sigma * (T(2) * axv - axpv);
+        yc[i0] = dual_prox_of(v, sigma, prl, pru, pyc, D.yc != nullptr);
+        if (pending) {
+            if (I.ysum.wide) static_cast<double *>(I.ysum.p)[i0] = psum + static_cast<double>(yi);
+            else static_cast<T *>(I.ysum.p)[i0] = static_cast<T>(psum) + yi;
+        }
+    }
     const int stride_e = gridDim.x * blockDim.x;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride_e) {
+    for (int i = i0 + stride_e; i < m; i += stride_e) {
         const T yi = y[i];
         const T v = yi + sigma * (T(2) * ax[i] - axp[i]);
         yc[i] = dual_prox(v, sigma, D, i);
