@@ -190,8 +190,9 @@ def _tiling_params(args):
     tuner picked without timing candidates inside the profiled process."""
     kw = dict(use_graph=0 if args.no_graph else 1)
     if args.tiling:
-        t, g, st = (int(v) for v in args.tiling.split(","))
-        kw.update(autotune=0, tile_nnz=t, spmv_grid=g, stream_loads=st)
+        v = [int(x) for x in args.tiling.split(",")] + [0, 0]
+        kw.update(autotune=0, tile_nnz=v[0], spmv_grid=v[1], stream_loads=v[2], probe_grid=v[3],
+                  xcache_bytes=v[4])
     else:
         kw.update(autotune=0 if args.no_tune else 1)
     return kw
@@ -224,6 +225,7 @@ def workload_lp1(args):
                              restarts=st.restarts, reanchors=st.reanchors, host_syncs=st.host_syncs,
                              kernel_launches=st.kernel_launches),
                  tiling=dict(tuned=st.tuned, tile_nnz=st.tile_nnz, spmv_grid=st.spmv_grid,
+                             probe_grid=st.probe_grid, xcache_bytes=st.xcache_bytes,
                              stream_loads=st.stream_loads, tune_s=round(st.tune_seconds, 3)))
         # the inner loop alone: raw iterations, no evaluation or restart
         S.iterate(args.warmup)
@@ -658,7 +660,8 @@ def main():
     ap.add_argument("--solve-time-limit", type=float, default=45.0)
     ap.add_argument("--skip-ndg", action="store_true", help="lp1: no NDG-restart solve")
     ap.add_argument("--no-tune", action="store_true", help="lp1: fixed tiling instead of the autotuner")
-    ap.add_argument("--tiling", default="", help="lp1: fix the tiling as TILE,GRID,STREAM (no autotuner)")
+    ap.add_argument("--tiling", default="",
+                    help="lp1: fix the tiling as TILE,GRID,STREAM[,PROBE_GRID,XCACHE_BYTES] (no autotuner)")
     ap.add_argument("--train", type=int, default=None,
                     help="warm4: training instances (default 1000); mlp4: rows (default 4096)")
     ap.add_argument("--test", type=int, default=100, help="warm4: held-out instances")

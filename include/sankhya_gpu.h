@@ -351,7 +351,11 @@ typedef struct sk_gpu_pdlp_params {  /* src/cuda/GpuPdlp.hpp:19-67 PdlpParams */
                                 the adaptive loop for each candidate of the
                                 products' row-tile entry cap {256 .. 2048},
                                 grid cap {1024, 4096} and streaming matrix
-                                loads {off, on}, and keep the fastest.  The
+                                loads {off, on}, then for an LP the best of
+                                them with an LDS copy of x's longest
+                                columns {32, 64 KB} in the step probe on
+                                {512, 1024} workgroups, and keep the
+                                fastest.  The
                                 choice changes the summation order, so
                                 results are reproducible for a fixed
                                 choice, not across tunings.  0 (default):
@@ -364,6 +368,14 @@ typedef struct sk_gpu_pdlp_params {  /* src/cuda/GpuPdlp.hpp:19-67 PdlpParams */
     int    spmv_grid;        /* autotune 0: workgroup cap of the products,
                                 1..4096 (0: 4096)                           */
     int    stream_loads;     /* autotune 0: 0 the rule, 1 off, 2 on         */
+    int    probe_grid;       /* autotune 0: workgroup cap of the step probe
+                                (A x, with Q x for a QP) alone, 1..4096
+                                (0: spmv_grid)                              */
+    int    xcache_bytes;     /* autotune 0, LP: the step probe gathers the
+                                last columns of x in slot order (the
+                                longest) from a copy of this many bytes in
+                                each workgroup's LDS; a multiple of 16 up
+                                to 65,536 (0: none)                         */
     int    pad;
 } sk_gpu_pdlp_params;
 
@@ -407,8 +419,10 @@ typedef struct sk_gpu_pdlp_stats {   /* src/cuda/GpuPdlp.hpp:69-92 PdlpStats */
     int    tile_nnz;             /* row-tile entry cap of A and A^T         */
     int    spmv_grid;            /* workgroup cap of the products           */
     int    stream_loads;         /* non-temporal matrix loads (A and A^T)   */
-    int    tuned;                /* the autotuner chose the three above     */
+    int    tuned;                /* the autotuner chose the tiling          */
     double tune_seconds;         /* time the autotuner took                 */
+    int    probe_grid;           /* workgroup cap of the step probe         */
+    int    xcache_bytes;         /* LDS copy of x in the step probe (bytes) */
 } sk_gpu_pdlp_stats;
 
 void sk_gpu_pdlp_params_default(sk_gpu_pdlp_params *p);

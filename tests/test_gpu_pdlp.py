@@ -252,11 +252,30 @@ def test_autotuned_session_solves_and_reports_its_choice(prec):
     rc, x, y, st = gpu_pdlp(model, precision=prec, eps_optimal=1e-7, autotune=1)
     assert rc == 0 and st.converged, sg.last_error()
     assert st.tuned == 1 and st.tile_nnz in (256, 512, 1024, 2048) and st.spmv_grid in (1024, 4096)
+    assert st.probe_grid in (512, 1024, 4096) and st.xcache_bytes in (0, 32768, 65536)
     assert st.stream_loads in (0, 1) and 0.0 < st.tune_seconds < 5.0
     ref = float(g["simplex_objective"])
     assert abs(float(model.vec("c") @ x) - ref) <= 1e-6 * (1 + abs(ref))
     rc, x, y, st = gpu_pdlp(model, precision=prec, eps_optimal=1e-7)
     assert st.tuned == 0 and st.spmv_grid == 4096
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("n_cols", [5, 998, 1003])
+def test_x_copy_in_lds_edges_reach_the_simplex_objective(prec, n_cols):
+    """The step probe's LDS copy of x when it covers every column (and
+    fewer than one 16-byte chunk of them, or a column count that is not a
+    multiple of a chunk, whose last columns are then gathered from memory):
+    the solve reaches the reference simplex objective."""
+    m = max(3, n_cols // 2)
+    model = sg.Model.generated("skw_bounded_lp", m, n_cols, 3, 4242 + n_cols)
+    T = dict(autotune=0, tile_nnz=256, spmv_grid=4096, probe_grid=512, xcache_bytes=65536, stream_loads=1)
+    rc, x, y, st = gpu_pdlp(model, precision=prec, eps_optimal=1e-7, **T)
+    assert rc == 0 and st.converged and st.xcache_bytes == 65536, sg.last_error()
+    rc0, x0, y0, st0 = gpu_pdlp(model, precision=prec, eps_optimal=1e-7)
+    assert rc0 == 0 and st0.converged and st0.xcache_bytes == 0
+    f, f0 = float(model.vec("c") @ x), float(model.vec("c") @ x0)
+    assert abs(f - f0) <= 1e-5 * (1 + abs(f0))
 
 
 @pytest.mark.parametrize("tile", [256, 512, 1024, 2048])
@@ -265,7 +284,9 @@ def test_every_tiling_matches_the_oracle_on_a_power_law_lp(tile):
     a 20,000 x 40,000 instance of the power-law generator (column degrees up
     to 2,000, so A^T has rows longer than every cap -- the workgroup-wide
     long-row path -- next to tiles of one to hundreds of rows, and 1, 2, 4
-    or 8 entries per lane), both grid caps.  The FP64 engine after 8 adaptive
+    or 8 entries per lane), both grid caps, and a step probe on its own
+    grid cap gathering x's longest columns (4,096 to 16,384 of the 40,000,
+    by precision and copy size) from an LDS copy.  The FP64 engine after 8 adaptive
     iterations against the oracle's FP64 PDLP within summation-order
     rounding; the mixed engine (FP32 products) within the FP32 drift of 8
     iterations."""
@@ -278,11 +299,13 @@ def test_every_tiling_matches_the_oracle_on_a_power_law_lp(tile):
     P = dict(max_iterations=8, use_graph=0)
     rc_o, xo, yo, sto = so.first_order_pdlp(model, sg.default_pdlp_params(precision=sg.PREC_FP64, **P))
     assert rc_o == 0 and sto.iterations == 8
-    for grid in (1024, 4096):
-        T = dict(autotune=0, tile_nnz=tile, spmv_grid=grid, stream_loads=1 + (grid == 4096))
+    for grid, probe, xc in ((1024, 1024, 0), (4096, 4096, 0), (4096, 512, 65536), (1024, 1024, 32768)):
+        T = dict(autotune=0, tile_nnz=tile, spmv_grid=grid, probe_grid=probe, xcache_bytes=xc,
+                 stream_loads=1 + (grid == 4096))
         rc, x, y, st = gpu_pdlp(model, precision=sg.PREC_FP64, **P, **T)
         assert rc == 0 and st.iterations == 8, sg.last_error()
-        assert st.tile_nnz == tile and st.spmv_grid == grid
+        assert st.tile_nnz == tile and st.spmv_grid == grid and st.probe_grid == probe
+        assert st.xcache_bytes == xc
         assert abs(st.matrix_norm - sto.matrix_norm) <= 1e-10 * sto.matrix_norm
         assert np.linalg.norm(x - xo) <= 1e-9 * (1.0 + np.linalg.norm(xo))
         assert np.linalg.norm(y - yo) <= 1e-9 * (1.0 + np.linalg.norm(yo))

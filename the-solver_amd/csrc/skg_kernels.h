@@ -95,6 +95,9 @@ typedef struct skk_bins {
 #ifndef SKK_STREAM_BYTES
 #define SKK_STREAM_BYTES (64ll << 20)
 #endif
+/* Largest LDS copy of x (bytes per workgroup) an LP's step probe can keep:
+ * with the 8 KB product buffer, two workgroups fit a CU's 160 KB. */
+#define SKK_XCACHE_MAX 65536
 
 typedef struct skk_csr {
     int rows, cols, nnz;
@@ -158,6 +161,10 @@ typedef struct skk_pdhg {
     double *eval_out;                        /* [2][8] residual sums        */
     int part_capacity;                       /* blocks per partial slot     */
     int elem_grid;                           /* fixed grid for reductions   */
+    int xcache_bytes;                        /* LP step probe: LDS bytes of the
+                                                last (longest) columns of x per
+                                                workgroup, <= SKK_XCACHE_MAX
+                                                (0: none)                    */
     int probe_grid;                          /* workgroups of the step probe
                                                 (A, or Q and A in one launch) */
     /* Anchored refinement of the FP32 set (MIXED): its iterates hold the

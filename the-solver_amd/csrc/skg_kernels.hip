@@ -272,9 +272,9 @@ constexpr int kRowsPerLane = (SKK_TILE_ROWS + kBlock - 1) / kBlock;
 // entry again and its product is not added), so a lane has eight gathers
 // in flight -- a per-entry branch would make each gather wait for the
 // previous one.
-template <typename VT, typename XT, typename AT, bool NT = true>
+template <typename VT, typename XT, typename AT, bool NT = true, class XS = const XT *>
 __device__ __forceinline__ AT row_part(const int *__restrict__ ci, const VT *__restrict__ val,
-                                       const XT *__restrict__ x, int p, int e, int L)
+                                       const XS &x, int p, int e, int L)
 {
     AT s0 = 0, s1 = 0;
     const int last = e - 1;
@@ -308,10 +308,9 @@ __device__ __forceinline__ AT row_part(const int *__restrict__ ci, const VT *__r
 // read the tile's last entry and are not stored), so the gathers are in
 // flight together.  mid() runs after the gathers are issued and before
 // their results are stored.
-template <int K, typename VT, typename XT, typename AT, bool NT, class Mid>
+template <int K, typename VT, typename XT, typename AT, bool NT, class XS, class Mid>
 __device__ __forceinline__ void tile_products(const int *__restrict__ ci, const VT *__restrict__ val,
-                                              const XT *__restrict__ x, int p0, int p1, AT *prod,
-                                              const Mid &mid)
+                                              const XS &x, int p0, int p1, AT *prod, const Mid &mid)
 {
     const int tid = threadIdx.x, last = p1 - 1;
     int c[K];
@@ -359,10 +358,10 @@ __device__ __forceinline__ void tile_products(const int *__restrict__ ci, const 
 // The tile's descriptor gives its rows [r0, r1) and entries [p0, p1)
 // together, so its entries are requested one memory round trip after the
 // descriptor rather than two.
-template <typename VT, typename XT, typename AT, bool NT, class Epi>
+template <typename VT, typename XT, typename AT, bool NT, class Epi, class XS>
 __device__ __forceinline__ void spmv_tile(const int *__restrict__ rp, const int *__restrict__ ci,
-                                          const VT *__restrict__ val, const XT *__restrict__ x, int r0,
-                                          int r1, int p0, int p1, const Epi &e, double *part, AT *prod)
+                                          const VT *__restrict__ val, const XS &x, int r0, int r1, int p0,
+                                          int p1, const Epi &e, double *part, AT *prod)
 {
     using In = decltype(epi_load(e, 0, 0));
     constexpr int kPer = SKK_TILE_NNZ / kBlock;
@@ -370,7 +369,7 @@ __device__ __forceinline__ void spmv_tile(const int *__restrict__ rp, const int 
     const int nr = r1 - r0;
     if (p1 - p0 > SKK_TILE_NNZ) {
         // one long row: workgroup-strided partial sums, wave and LDS combine
-        AT sum = row_part<VT, XT, AT, NT>(ci, val, x, p0 + tid, p1, kBlock);
+        AT sum = row_part<VT, XT, AT, NT, XS>(ci, val, x, p0 + tid, p1, kBlock);
         sum = wave_sum(sum);
         if ((tid & 63) == 0) prod[tid >> 6] = sum;
         __syncthreads();
@@ -457,12 +456,51 @@ __device__ __forceinline__ void spmv_tile(const int *__restrict__ rp, const int 
 // selector and the epilogue's state are requested before any of them is
 // waited for (the grid never exceeds the tile count, so tile blockIdx.x
 // exists): a workgroup's prologue costs one memory round trip, not three.
-template <typename VT, typename XT, typename AT, class Epi, bool NT>
+// Gathers of x[c] with the columns [c0, c0 + kc) served from an LDS copy:
+// an LDS read for every lane (clamped) and a global load only for the lanes
+// outside the copy.  The pointers carry their address spaces, so the two
+// stay an LDS and a global load (a select between generic pointers would
+// become one flat load).
+template <typename XT> struct XCached {
+    const __attribute__((address_space(1))) XT *x;
+    const __attribute__((address_space(3))) XT *xs;
+    int c0, kc;
+    __device__ __forceinline__ XT operator[](int c) const
+    {
+        const unsigned o = static_cast<unsigned>(c - c0);
+        const bool in = o < static_cast<unsigned>(kc);
+        const XT lv = xs[in ? o : 0u];
+        XT v = lv;
+        if (!in) v = x[c];
+        return v;
+    }
+};
+
+// Copy n 16-byte chunks from global memory to LDS: eight loads per lane in
+// flight, then their stores (an index past the end copies the last chunk
+// again, so no load or store is conditional).
+__device__ __forceinline__ void fill_lds(uint4 *dst, const uint4 *__restrict__ src, int n)
+{
+    for (int base = 0; base < n; base += 8 * kBlock) {
+        uint4 t[8];
+        int i[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            i[u] = min(base + u * kBlock + static_cast<int>(threadIdx.x), n - 1);
+            t[u] = src[i[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) dst[i[u]] = t[u];
+    }
+}
+
+template <typename VT, typename XT, typename AT, class Epi, bool NT, bool kXC = false>
 __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
                                                  const int *__restrict__ ci,
                                                  const VT *__restrict__ val, const XT *x0,
                                                  const XT *x1, const int *__restrict__ cur,
-                                                 const int *__restrict__ tiles, int ntiles, Epi epi)
+                                                 const int *__restrict__ tiles, int ntiles, Epi epi,
+                                                 int c0, int kc)
 {
     __shared__ AT prod[SKK_TILE_NNZ];
     int t = blockIdx.x;
@@ -472,16 +510,29 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
     e.begin(epi);
     const XT *__restrict__ x = (*cur ^ 1) ? x1 : x0;
     double part[Epi::kSlots > 0 ? Epi::kSlots : 1] = {};
-    for (;;) {
-        // the next tile's descriptor (clamped to the last) is in flight
-        // while this tile is processed
-        const int tn = t + gridDim.x;
-        d = tiles + 2 * min(tn, ntiles - 1);
-        const int n0 = d[0], n1 = d[1], n2 = d[2], n3 = d[3];
-        spmv_tile<VT, XT, AT, NT>(rp, ci, val, x, r0, r1, p0, p1, e, part, prod);
-        if (tn >= ntiles) break;
-        t = tn;
-        r0 = n0; p0 = n1; r1 = n2; p1 = n3;
+    auto run = [&](const auto &xs) {
+        for (;;) {
+            // the next tile's descriptor (clamped to the last) is in flight
+            // while this tile is processed
+            const int tn = t + gridDim.x;
+            d = tiles + 2 * min(tn, ntiles - 1);
+            const int n0 = d[0], n1 = d[1], n2 = d[2], n3 = d[3];
+            spmv_tile<VT, XT, AT, NT>(rp, ci, val, xs, r0, r1, p0, p1, e, part, prod);
+            if (tn >= ntiles) break;
+            t = tn;
+            r0 = n0; p0 = n1; r1 = n2; p1 = n3;
+        }
+    };
+    if constexpr (kXC) {
+        // the workgroup's copy of x[c0, c0 + kc) (kc * sizeof(XT) a
+        // multiple of 16, c0 * sizeof(XT) too), then its tiles
+        extern __shared__ uint4 xdyn[];
+        fill_lds(xdyn, reinterpret_cast<const uint4 *>(x + c0), static_cast<int>(kc * sizeof(XT) / 16));
+        __syncthreads();
+        run(XCached<XT>{(const __attribute__((address_space(1))) XT *)x,
+                        (const __attribute__((address_space(3))) XT *)xdyn, c0, kc});
+    } else {
+        run(x);
     }
     e.end(part);
 }
@@ -547,10 +598,10 @@ int launch_spmv(const skk_csr &A, const VT *vals, const XT *x0, const XT *x1,
     if (!sel) x1 = x0;
     if (A.stream)
         hipLaunchKernelGGL((k_spmv<VT, XT, AT, Epi, true>), dim3(A.grid), dim3(kBlock), 0,
-                           stream(), A.rp, A.ci, vals, x0, x1, cur, A.tiles, A.ntiles, epi);
+                           stream(), A.rp, A.ci, vals, x0, x1, cur, A.tiles, A.ntiles, epi, 0, 0);
     else
         hipLaunchKernelGGL((k_spmv<VT, XT, AT, Epi, false>), dim3(A.grid), dim3(kBlock), 0,
-                           stream(), A.rp, A.ci, vals, x0, x1, cur, A.tiles, A.ntiles, epi);
+                           stream(), A.rp, A.ci, vals, x0, x1, cur, A.tiles, A.ntiles, epi, 0, 0);
     return launch_status();
 }
 
@@ -1764,7 +1815,24 @@ template <typename T> int probe_spmv(const skk_pdhg &P, const Iter<T> &I, bool s
     EpiProbe<T, T> pr{};
     pr.I = I; pr.st = P.st; pr.part = P.part; pr.stride = P.part_capacity;
     pr.snap = snapshot ? P.snap : nullptr;
-    return launch_spmv<T, T, T>(P.A, vals_of<T>(P.A), I.X[0], I.X[1], P.st, pr);
+    const int kc = P.xcache_bytes / static_cast<int>(sizeof(T));
+    if (kc <= 0 || P.A.ntiles == 0) return launch_spmv<T, T, T>(P.A, vals_of<T>(P.A), I.X[0], I.X[1], P.st, pr);
+    // the last kc columns -- the longest, in slot order -- from LDS; the
+    // copy starts and ends on 16-byte boundaries (columns past its end are
+    // gathered from memory)
+    constexpr int q = 16 / static_cast<int>(sizeof(T));
+    const int c0a = ((P.n > kc ? P.n - kc : 0) + q - 1) / q * q;
+    const int kcn = (P.n - c0a) / q * q;
+    if (kcn <= 0) return launch_spmv<T, T, T>(P.A, vals_of<T>(P.A), I.X[0], I.X[1], P.st, pr);
+    using E = EpiProbe<T, T>;
+    const size_t lds = sizeof(T) * (size_t)kcn;
+    if (P.A.stream)
+        hipLaunchKernelGGL((k_spmv<T, T, T, E, true, true>), dim3(P.A.grid), dim3(kBlock), lds, stream(), P.A.rp,
+                           P.A.ci, vals_of<T>(P.A), I.X[0], I.X[1], &P.st->cur, P.A.tiles, P.A.ntiles, pr, c0a, kcn);
+    else
+        hipLaunchKernelGGL((k_spmv<T, T, T, E, false, true>), dim3(P.A.grid), dim3(kBlock), lds, stream(), P.A.rp,
+                           P.A.ci, vals_of<T>(P.A), I.X[0], I.X[1], &P.st->cur, P.A.tiles, P.A.ntiles, pr, c0a, kcn);
+    return launch_status();
 }
 
 // Q x_cand and A x_cand in one launch of P.probe_grid workgroups (QP).

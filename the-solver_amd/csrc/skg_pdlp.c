@@ -173,7 +173,7 @@ typedef struct engine {
     skk_unpermute U;             /* slot -> original order on the device */
     long long launches;
     int *rp_host[3];             /* host row offsets of A, A^T, Q (autotune) */
-    int tile_nnz, grid_cap, stream, tuned;   /* tiling of the products    */
+    int tile_nnz, grid_cap, probe_cap, xcache, stream, tuned;   /* tiling of the products */
     double tune_seconds;
     int unfused;
 } engine;
@@ -268,29 +268,34 @@ done:
     return rc;
 }
 
-/* Row tiles of every product with entry cap `tile`, grid cap `grid` and
- * streaming matrix loads `stream` (A and A^T; a Hessian keeps its own
- * streaming rule), and the launch shapes that follow from them. */
-static int apply_tiling(engine *E, int tile, int grid, int stream)
+/* Row tiles of every product with entry cap `tile` and streaming matrix
+ * loads `stream` (A and A^T; a Hessian keeps its own streaming rule); grid
+ * cap `grid` for the A^T product and `probe` for the step probe (A, with Q
+ * for a QP); an LP's step probe keeps the last `xcache` bytes of x (its
+ * longest columns) in LDS (0: none); and the launch shapes that follow. */
+static int apply_tiling(engine *E, int tile, int grid, int probe, int xcache, int stream)
 {
     skk_pdhg *Q = &E->P;
-    if (dcsr_retile(&E->A, E->rp_host[0], tile, grid, stream) ||
+    if (dcsr_retile(&E->A, E->rp_host[0], tile, probe, stream) ||
         dcsr_retile(&E->AT, E->rp_host[1], tile, grid, stream))
         return -1;
     Q->A = E->A.k;
     Q->AT = E->AT.k;
     Q->probe_grid = E->A.k.grid;
     if (E->has_q) {
-        if (dcsr_retile(&E->Q, E->rp_host[2], tile, grid, E->Q.k.stream)) return -1;
+        if (dcsr_retile(&E->Q, E->rp_host[2], tile, probe, E->Q.k.stream)) return -1;
         Q->Q = E->Q.k;
         {
             const long long both = (long long)E->A.k.ntiles + E->Q.k.ntiles;
-            Q->probe_grid = both < grid ? (int)both : grid;
+            Q->probe_grid = both < probe ? (int)both : probe;
         }
     }
     Q->fuse_decision = !E->unfused && E->AT.k.grid + Q->probe_grid <= FUSE_MAX_PARTIALS;
+    Q->xcache_bytes = E->has_q ? 0 : xcache;
     E->tile_nnz = tile;
     E->grid_cap = grid;
+    E->probe_cap = probe;
+    E->xcache = Q->xcache_bytes;
     E->stream = stream;
     return 0;
 }
@@ -409,10 +414,11 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
         fprintf(stderr, "set-up: transpose %.3f s, scaling %.3f s, ordering %.3f s, permuted copies %.3f s, "
                 "upload %.3f s\n", tp[1] - tp[0], tp[2] - tp[1], tp[3] - tp[2], tp[4] - tp[3], tp[5] - tp[4]);
     E->tile_nnz = skg_tile_nnz(Ap.nnz);
-    E->grid_cap = SKK_SPMV_GRID;
+    E->grid_cap = E->probe_cap = SKK_SPMV_GRID;
+    E->xcache = 0;
     E->stream = E->A.k.stream;
     E->unfused = P->unfused;
-    if (P->autotune || P->tile_nnz || P->spmv_grid || P->stream_loads) {
+    if (P->autotune || P->tile_nnz || P->spmv_grid || P->probe_grid || P->xcache_bytes || P->stream_loads) {
         E->rp_host[0] = (int *)malloc(sizeof(int) * ((size_t)m + 1));
         E->rp_host[1] = (int *)malloc(sizeof(int) * ((size_t)n + 1));
         if (!E->rp_host[0] || !E->rp_host[1]) { rc = skg_fail("out of host memory"); goto done; }
@@ -540,14 +546,18 @@ static int engine_build(engine *E, int m, int n, const int *rp, const int *ci, c
         /* a tiling fixed by the caller (a tuned choice reused, say) */
         const int tile = P->tile_nnz ? P->tile_nnz : E->tile_nnz;
         const int grid = P->spmv_grid ? P->spmv_grid : SKK_SPMV_GRID;
+        const int probe = P->probe_grid ? P->probe_grid : grid;
         const int stream = P->stream_loads ? P->stream_loads == 2 : E->stream;
         if (tile < SKK_TILE_MIN_NNZ || tile > SKK_TILE_NNZ || grid < 1 || grid > SKK_SPMV_GRID ||
+            probe < 1 || probe > SKK_SPMV_GRID || P->xcache_bytes < 0 || P->xcache_bytes > SKK_XCACHE_MAX ||
+            P->xcache_bytes % 16 ||
             P->stream_loads < 0 || P->stream_loads > 2) {
-            rc = skg_fail("invalid tiling: tile_nnz %d, spmv_grid %d, stream_loads %d", P->tile_nnz,
-                          P->spmv_grid, P->stream_loads);
+            rc = skg_fail("invalid tiling: tile_nnz %d, spmv_grid %d, probe_grid %d, xcache_bytes %d, "
+                          "stream_loads %d", P->tile_nnz, P->spmv_grid, P->probe_grid, P->xcache_bytes,
+                          P->stream_loads);
             goto done;
         }
-        if ((rc = apply_tiling(E, tile, grid, stream))) goto done;
+        if ((rc = apply_tiling(E, tile, grid, probe, P->xcache_bytes, stream))) goto done;
     }
     rc = 0;
 done:
@@ -685,35 +695,67 @@ static int check_params(const sk_gpu_pdlp_params *P)
 static int session_start(sk_gpu_pdlp_session *S, int prec, const double *x0, const double *y0,
                          double *omega_out);
 
-/* The autotuner: for each candidate tiling, a cold start and a timed run
- * of 32 adaptive iterations (after 8 untimed) on the library stream; the
+/* One autotuner candidate: a cold start and a timed run of 32 adaptive
+ * iterations (after 8 untimed) on the library stream; milliseconds, or a
+ * negative value on a HIP error. */
+static double tune_probe(sk_gpu_pdlp_session *S, int prec, int tile, int grid, int probe, int xcache, int stream)
+{
+    engine *E = &S->E;
+    double omega, ms;
+    int rc;
+    if (apply_tiling(E, tile, grid, probe, xcache, stream) || session_start(S, prec, NULL, NULL, &omega))
+        return -1.0;
+    if ((rc = S->P.adaptive_step ? skk_pdhg_adaptive_run(&E->P, prec, 8) : skk_pdhg_fixed_iter(&E->P, prec)) ||
+        (rc = skk_timeline_mark(0)) ||
+        (rc = S->P.adaptive_step ? skk_pdhg_adaptive_run(&E->P, prec, 32) : skk_pdhg_fixed_iter(&E->P, prec)) ||
+        (rc = skk_timeline_mark(1)) || (rc = skk_sync())) {
+        skg_hip_fail("autotune probe", rc);
+        return -1.0;
+    }
+    ms = skk_timeline_ms(0, 1);
+    if (S->P.verbosity > 0)
+        fprintf(stderr, "autotune tile %d grid %d probe %d xcache %d stream %d: %.4f ms\n", tile, grid, probe,
+                xcache, stream, ms);
+    return ms > 0.0 ? ms : INFINITY;
+}
+
+/* The autotuner: every tile cap x grid cap x streaming choice with one grid
+ * cap for both products and no LDS copy of x; then, for an LP, the best of
+ * those with the step probe's x gathered partly from an LDS copy of its
+ * longest columns (32 or 64 KB per workgroup) on 512 or 1,024 workgroups
+ * (a power-law matrix gathers most of x from a few thousand columns).  The
  * fastest is kept. */
 static int autotune(sk_gpu_pdlp_session *S)
 {
     static const int tiles[4] = {256, 512, 1024, 2048};
     static const int grids[2] = {1024, SKK_SPMV_GRID};
+    static const int caches[2] = {SKK_XCACHE_MAX / 2, SKK_XCACHE_MAX};
+    static const int probes[2] = {512, 1024};
     engine *E = &S->E;
     const int prec = E->mixed ? SKK_F32 : SKK_F64;
     const double t0 = skg_now();
-    double best_ms = INFINITY, omega;
-    int best[3] = {E->tile_nnz, E->grid_cap, E->stream}, t, g, st, rc;
+    double best_ms = INFINITY, ms;
+    int best[5] = {E->tile_nnz, E->grid_cap, E->probe_cap, 0, E->stream}, t, g, st, rc;
     if ((rc = skk_timeline_reserve(2))) return skg_hip_fail("event timeline", rc);
     for (t = 0; t < 4; ++t)
         for (g = 0; g < 2; ++g)
             for (st = 0; st < 2; ++st) {
-                double ms;
-                if (apply_tiling(E, tiles[t], grids[g], st) || session_start(S, prec, NULL, NULL, &omega)) return -1;
-                if ((rc = S->P.adaptive_step ? skk_pdhg_adaptive_run(&E->P, prec, 8) : skk_pdhg_fixed_iter(&E->P, prec)) ||
-                    (rc = skk_timeline_mark(0)) ||
-                    (rc = S->P.adaptive_step ? skk_pdhg_adaptive_run(&E->P, prec, 32) : skk_pdhg_fixed_iter(&E->P, prec)) ||
-                    (rc = skk_timeline_mark(1)) || (rc = skk_sync()))
-                    return skg_hip_fail("autotune probe", rc);
-                ms = skk_timeline_ms(0, 1);
-                if (S->P.verbosity > 0)
-                    fprintf(stderr, "autotune tile %d grid %d stream %d: %.4f ms\n", tiles[t], grids[g], st, ms);
-                if (ms > 0.0 && ms < best_ms) { best_ms = ms; best[0] = tiles[t]; best[1] = grids[g]; best[2] = st; }
+                if ((ms = tune_probe(S, prec, tiles[t], grids[g], grids[g], 0, st)) < 0.0) return -1;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best[0] = tiles[t]; best[1] = best[2] = grids[g]; best[3] = 0; best[4] = st;
+                }
             }
-    if (apply_tiling(E, best[0], best[1], best[2])) return -1;
+    if (!E->has_q) {
+        const int t1 = best[0], g1 = best[1], s1 = best[4];
+        int c, pg;
+        for (c = 0; c < 2; ++c)
+            for (pg = 0; pg < 2; ++pg) {
+                if ((ms = tune_probe(S, prec, t1, g1, probes[pg], caches[c], s1)) < 0.0) return -1;
+                if (ms < best_ms) { best_ms = ms; best[2] = probes[pg]; best[3] = caches[c]; }
+            }
+    }
+    if (apply_tiling(E, best[0], best[1], best[2], best[3], best[4])) return -1;
     E->tuned = 1;
     E->tune_seconds = skg_now() - t0;
     return 0;
@@ -1115,6 +1157,8 @@ int sk_gpu_pdlp_session_solve(sk_gpu_pdlp_session *S, const double *x0, const do
     S->stats.matrix_norm = S->norm;
     S->stats.tile_nnz = E->tile_nnz;
     S->stats.spmv_grid = E->grid_cap;
+    S->stats.probe_grid = E->probe_cap;
+    S->stats.xcache_bytes = E->xcache;
     S->stats.stream_loads = E->stream;
     S->stats.tuned = E->tuned;
     S->stats.tune_seconds = E->tune_seconds;

@@ -101,7 +101,8 @@ class PdlpParams(C.Structure):
                 ("use_graph", C.c_int), ("verbosity", C.c_int), ("primal_weight", C.c_double),
                 ("unfused", C.c_int), ("refine", C.c_int), ("cancel", C.POINTER(C.c_int)),
                 ("restart_rule", C.c_int), ("autotune", C.c_int), ("tile_nnz", C.c_int),
-                ("spmv_grid", C.c_int), ("stream_loads", C.c_int), ("pad", C.c_int)]
+                ("spmv_grid", C.c_int), ("stream_loads", C.c_int), ("probe_grid", C.c_int),
+                ("xcache_bytes", C.c_int), ("pad", C.c_int)]
 
 
 class PdlpStats(C.Structure):
@@ -118,7 +119,7 @@ class PdlpStats(C.Structure):
                 ("reanchors", C.c_int), ("iterations_to_merit", C.c_int * 4),
                 ("seconds_to_merit", C.c_double * 4), ("restart_merit", C.c_double * 2),
                 ("tile_nnz", C.c_int), ("spmv_grid", C.c_int), ("stream_loads", C.c_int), ("tuned", C.c_int),
-                ("tune_seconds", C.c_double)]
+                ("tune_seconds", C.c_double), ("probe_grid", C.c_int), ("xcache_bytes", C.c_int)]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
