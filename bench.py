@@ -659,7 +659,7 @@ def main():
                     help="launch restart windows directly instead of replaying hipGraphs (profilers)")
     ap.add_argument("--solve-time-limit", type=float, default=45.0)
     ap.add_argument("--skip-ndg", action="store_true", help="lp1: no NDG-restart solve")
-    ap.add_argument("--no-tune", action="store_true", help="lp1: fixed tiling instead of the autotuner")
+    ap.add_argument("--no-tune", action="store_true", help="lp1, pipe1: fixed tiling instead of the autotuner")
     ap.add_argument("--tiling", default="",
                     help="lp1: fix the tiling as TILE,GRID,STREAM[,PROBE_GRID,XCACHE_BYTES] (no autotuner)")
     ap.add_argument("--train", type=int, default=None,
@@ -743,7 +743,8 @@ def workload_pipe1(args):
     problem = sg.LpProblem.from_model(model)
     log(f"LpProblem built in {time.perf_counter() - t:.1f}s ({problem.rows} x {problem.cols}, nnz {problem.nnz})")
     opts = sg.lp_solver_options(use_presolve=1, eps_optimal=1e-6, time_limit_seconds=args.solve_time_limit,
-                                precision=sg.PREC_MIXED, use_graph=0 if args.no_graph else 1)
+                                precision=sg.PREC_MIXED, use_graph=0 if args.no_graph else 1,
+                                autotune=0 if args.no_tune else 1)
     t = time.perf_counter()
     sol, _x = sg.solve_lp(problem, opts)
     wall = time.perf_counter() - t
