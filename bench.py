@@ -411,7 +411,7 @@ def _engines(lib, model, arrays, q, args, use_graph=0, repeats=3):
     for name, prec in (("mixed", sg.PREC_MIXED), ("fp64", sg.PREC_FP64)):
         t = time.perf_counter()
         S = _session_qp(lib, model, arrays, q, precision=prec, use_graph=use_graph,
-                        time_limit_seconds=args.solve_time_limit)
+                        time_limit_seconds=args.solve_time_limit, autotune=0 if args.no_tune else 1)
         setup_s = time.perf_counter() - t
         S.iterate(args.warmup)
         t = time.perf_counter()
@@ -427,6 +427,9 @@ def _engines(lib, model, arrays, q, args, use_graph=0, repeats=3):
                 solves.append(time.perf_counter() - t)
                 loops.append(st.loop_seconds)
             kkt = _kkt(lib, model, arrays, q, x, y)
+            r.update(tiling=dict(tuned=st.tuned, tile_nnz=st.tile_nnz, spmv_grid=st.spmv_grid,
+                                 probe_grid=st.probe_grid, xcache_bytes=st.xcache_bytes,
+                                 stream_loads=st.stream_loads, tune_s=round(st.tune_seconds, 3)))
             r.update(converged=bool(st.converged), iterations=st.iterations, fp32_iterations=st.fp32_iterations,
                      fp64_iterations=st.fp64_iterations, loop_s=float(np.median(loops)), loop_s_all=loops,
                      end_to_end_s=setup_s + float(np.median(solves)), setup_s=setup_s, restarts=st.restarts,
@@ -659,7 +662,7 @@ def main():
                     help="launch restart windows directly instead of replaying hipGraphs (profilers)")
     ap.add_argument("--solve-time-limit", type=float, default=45.0)
     ap.add_argument("--skip-ndg", action="store_true", help="lp1: no NDG-restart solve")
-    ap.add_argument("--no-tune", action="store_true", help="lp1, pipe1: fixed tiling instead of the autotuner")
+    ap.add_argument("--no-tune", action="store_true", help="lp0, lp1, qp2, pipe1: fixed tiling instead of the autotuner")
     ap.add_argument("--tiling", default="",
                     help="lp1: fix the tiling as TILE,GRID,STREAM[,PROBE_GRID,XCACHE_BYTES] (no autotuner)")
     ap.add_argument("--train", type=int, default=None,
