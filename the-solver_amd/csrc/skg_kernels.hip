@@ -698,7 +698,7 @@ template <> const float *vals_of<float>(const skk_csr &A) { return A.vf; }
 template <> const double *vals_of<double>(const skk_csr &A) { return A.vd; }
 
 // Partial-slot regions (each part_capacity doubles).
-enum { R_DX = 0, R_DY = 1, R_INTER = 2, R_EVAL = 3, R_POWER = 11, R_VERIFY = 12, R_DQ = 13 };
+enum { R_DX = 0, R_DY = 1, R_INTER = 2, R_EVAL = 3, R_POWER = 11, R_VERIFY = 12, R_DQ = 13, R_DXF = 14 };
 enum { C_PROBE = 0, C_EVAL = 1, C_POWER = 2, C_VERIFY = 3, C_KKT = 4 };
 
 __global__ void k_init_state(skk_state *st, double eta0, double omega, double eta_min,
@@ -830,6 +830,8 @@ template <typename T, typename AT> struct EpiProbe {
     skk_state *snap;    // or null
     double *part;       // region base
     int stride;         // part_capacity
+    const double *dx_in;  // or null: the A^T product's ||dx||^2 partials to fold
+    int dx_count;
     const T *y, *yc, *ax;
     T *axc;
     // Reads the buffer table from the kernel-argument copy `k`: selecting
@@ -861,6 +863,17 @@ template <typename T, typename AT> struct EpiProbe {
         // the state this launch ran with, for a decision fused into the
         // next launch (st itself is rewritten there)
         if (snap && blockIdx.x == 0 && threadIdx.x == 0) *snap = *st;
+        // this workgroup's share of the A^T product's ||dx||^2 partials (g,
+        // g + G, ... for G probe workgroups), summed by the first wave in a
+        // fixed order: the decision then reduces one per probe workgroup
+        if (dx_in && threadIdx.x < 64) {
+            double a = 0.0;
+            for (int k = blockIdx.x + static_cast<int>(threadIdx.x * gridDim.x); k < dx_count;
+                 k += 64 * static_cast<int>(gridDim.x))
+                a += dx_in[k];
+            a = wave_sum(a);
+            if (threadIdx.x == 0) part[(size_t)R_DXF * stride + blockIdx.x] = a;
+        }
     }
 };
 
@@ -914,9 +927,10 @@ __device__ void decide(const skk_state &in, double dx2, double dq, double dy2, d
 // fall back to one slot at a time).
 template <int kPer, class Mid>
 __device__ void probe_sums(const double *part, int stride, int dx_blocks, int dq_blocks, int dy_blocks,
-                           double &dx2, double &dq, double &dy2, double &dya, const Mid &mid)
+                           double &dx2, double &dq, double &dy2, double &dya, const Mid &mid,
+                           int dx_region = R_DX)
 {
-    const double *const slots[4] = {part + (size_t)R_DX * stride, part + (size_t)R_DQ * stride,
+    const double *const slots[4] = {part + (size_t)dx_region * stride, part + (size_t)R_DQ * stride,
                                     part + (size_t)R_DY * stride, part + (size_t)R_INTER * stride};
     const int counts[4] = {dx_blocks, dq_blocks, dy_blocks, dy_blocks};
     double out[4];
@@ -928,8 +942,11 @@ __device__ void probe_sums(const double *part, int stride, int dx_blocks, int dq
 }
 
 // One workgroup after the A x_cand launch: the decision, committed to st.
+// dx_region: R_DX (the A^T product's partials) or R_DXF (folded by the
+// probe launch, one per probe workgroup).
 __global__ __launch_bounds__(kBlock) void k_probe_finalize(skk_state *st, const double *part, int stride,
-                                                           int dx_blocks, int dq_blocks, int dy_blocks)
+                                                           int dx_blocks, int dq_blocks, int dy_blocks,
+                                                           int dx_region)
 {
     __shared__ StepFactors s_f;
     double dx2, dq, dy2, dya;
@@ -946,10 +963,13 @@ __global__ __launch_bounds__(kBlock) void k_probe_finalize(skk_state *st, const 
     // as few partials per thread as the counts need (loads past a count are
     // issued too): the sums are the same for any kPer that covers them
     const int most = max(max(dx_blocks, dq_blocks), dy_blocks);
-    if (most <= kBlock) probe_sums<1>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya, factors);
+    if (most <= kBlock)
+        probe_sums<1>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya, factors, dx_region);
     else if (most <= 4 * kBlock)
-        probe_sums<4>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya, factors);
-    else probe_sums<SKK_SPMV_GRID / kBlock>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya, factors);
+        probe_sums<4>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya, factors, dx_region);
+    else
+        probe_sums<SKK_SPMV_GRID / kBlock>(part, stride, dx_blocks, dq_blocks, dy_blocks, dx2, dq, dy2, dya,
+                                           factors, dx_region);
     if (threadIdx.x != 0) return;
     skk_state next;
     decide(in, dx2, dq, dy2, dya, s_f, next);
@@ -1810,11 +1830,27 @@ int grid_for(int n, int cap)
 template <typename T> constexpr int prec_index() { return std::is_same<T, float>::value ? SKK_F32 : SKK_F64; }
 
 // A x_cand with the probe epilogue (x_cand is X[cur ^ 1]).
+// The probe launch folds the A^T product's ||dx||^2 partials when the
+// one-workgroup decision follows it and the probe has fewer workgroups.
+bool fold_dx(const skk_pdhg &P)
+{
+    return !P.fuse_decision && P.AT.grid > P.probe_grid;
+}
+
+template <typename T> void fold_into(const skk_pdhg &P, EpiProbe<T, T> &pr)
+{
+    if (fold_dx(P)) {
+        pr.dx_in = P.part + (size_t)R_DX * P.part_capacity;
+        pr.dx_count = P.AT.grid;
+    }
+}
+
 template <typename T> int probe_spmv(const skk_pdhg &P, const Iter<T> &I, bool snapshot = false)
 {
     EpiProbe<T, T> pr{};
     pr.I = I; pr.st = P.st; pr.part = P.part; pr.stride = P.part_capacity;
     pr.snap = snapshot ? P.snap : nullptr;
+    fold_into(P, pr);
     const int kc = P.xcache_bytes / static_cast<int>(sizeof(T));
     if (kc <= 0 || P.A.ntiles == 0) return launch_spmv<T, T, T>(P.A, vals_of<T>(P.A), I.X[0], I.X[1], P.st, pr);
     // the last kc columns -- the longest, in slot order -- from LDS; the
@@ -1843,6 +1879,7 @@ template <typename T> int qa_probe(const skk_pdhg &P, const Iter<T> &I, bool sna
     EpiProbe<T, T> pr{};
     pr.I = I; pr.st = P.st; pr.part = P.part; pr.stride = P.part_capacity;
     pr.snap = snapshot ? P.snap : nullptr;
+    fold_into(P, pr);
     const TileSet q{P.Q.rp, P.Q.ci, P.Q.tiles, P.Q.ntiles}, a{P.A.rp, P.A.ci, P.A.tiles, P.A.ntiles};
     const T *vq = vals_of<T>(P.Q), *va = vals_of<T>(P.A);
     using EQ = EpiQProbe<T, T>;
@@ -1869,8 +1906,10 @@ template <typename T> int step_probe(const skk_pdhg &P, const Iter<T> &I, bool s
 // The single-workgroup accept/reject decision.
 int probe_decide(const skk_pdhg &P)
 {
+    const bool fold = fold_dx(P);
     hipLaunchKernelGGL(k_probe_finalize, dim3(1), dim3(kBlock), 0, stream(), P.st, P.part, P.part_capacity,
-                       P.AT.grid, P.has_q ? P.probe_grid : 0, P.probe_grid);
+                       fold ? P.probe_grid : P.AT.grid, P.has_q ? P.probe_grid : 0, P.probe_grid,
+                       fold ? static_cast<int>(R_DXF) : static_cast<int>(R_DX));
     return launch_status();
 }
 
