@@ -449,13 +449,6 @@ __device__ __forceinline__ void spmv_tile(const int *__restrict__ rp, const int 
     __syncthreads();   // prod is reused by the next tile
 }
 
-// The gathered vector is x1 when *cur is 0 and x0 when it is 1: `cur` points
-// at the device state's buffer index (candidate x1 = X[cur ^ 1] of the pair
-// x0 = X[0], x1 = X[1]), or at the tile table's first entry (always 0) with
-// x1 = x0 when there is nothing to select.  The first tile's descriptor, the
-// selector and the epilogue's state are requested before any of them is
-// waited for (the grid never exceeds the tile count, so tile blockIdx.x
-// exists): a workgroup's prologue costs one memory round trip, not three.
 // Gathers of x[c] with the columns [c0, c0 + kc) served from an LDS copy:
 // an LDS read for every lane (clamped) and a global load only for the lanes
 // outside the copy.  The pointers carry their address spaces, so the two
@@ -494,6 +487,15 @@ __device__ __forceinline__ void fill_lds(uint4 *dst, const uint4 *__restrict__ s
     }
 }
 
+// The gathered vector is x1 when *cur is 0 and x0 when it is 1: `cur` points
+// at the device state's buffer index (candidate x1 = X[cur ^ 1] of the pair
+// x0 = X[0], x1 = X[1]), or at the tile table's first entry (always 0) with
+// x1 = x0 when there is nothing to select.  The first tile's descriptor, the
+// selector and the epilogue's state are requested before any of them is
+// waited for (the grid never exceeds the tile count, so tile blockIdx.x
+// exists): a workgroup's prologue costs one memory round trip, not three.
+// With kXC each workgroup first copies x[c0, c0 + kc) into its dynamic LDS
+// and gathers those columns from there.
 template <typename VT, typename XT, typename AT, class Epi, bool NT, bool kXC = false>
 __global__ __launch_bounds__(kBlock) void k_spmv(const int *__restrict__ rp,
                                                  const int *__restrict__ ci,
