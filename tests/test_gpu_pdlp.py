@@ -278,6 +278,18 @@ def test_x_copy_in_lds_edges_reach_the_simplex_objective(prec, n_cols):
     assert abs(f - f0) <= 1e-5 * (1 + abs(f0))
 
 
+def test_invalid_lds_copy_sizes_are_rejected():
+    """A fixed tiling's LDS copy of x must be a multiple of 16 bytes and at
+    most 65,536: anything else is an error at session creation, not a
+    launch that could read past the copy."""
+    model = sg.Model.generated("skw_bounded_lp", 40, 100, 3, 7)
+    for xc in (100, 65536 + 16, -16):
+        rc, x, y, st = gpu_pdlp(model, autotune=0, tile_nnz=256, xcache_bytes=xc)
+        assert rc != 0 and "xcache_bytes" in sg.last_error()
+    rc, x, y, st = gpu_pdlp(model, autotune=0, tile_nnz=256, probe_grid=5000)
+    assert rc != 0 and "probe_grid" in sg.last_error()
+
+
 @pytest.mark.parametrize("tile", [256, 512, 1024, 2048])
 def test_every_tiling_matches_the_oracle_on_a_power_law_lp(tile):
     """The products' tile paths at every entry cap the autotuner can pick:
