@@ -661,6 +661,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="launch restart windows directly instead of replaying hipGraphs (profilers)")
     ap.add_argument("--solve-time-limit", type=float, default=45.0)
+    ap.add_argument("--engine", default="mixed", choices=["mixed", "fp64"], help="lp1ndg: which engine")
     ap.add_argument("--skip-ndg", action="store_true", help="lp1: no NDG-restart solve")
     ap.add_argument("--no-tune", action="store_true", help="lp0, lp1, qp2, pipe1: fixed tiling instead of the autotuner")
     ap.add_argument("--tiling", default="",
@@ -781,8 +782,45 @@ def workload_pipe1(args):
     return line
 
 
+def workload_lp1ndg(args):
+    """The headline LP to relative KKT 1e-6 with NDG restarts -- the one
+    restart criterion under which it gets there -- for one engine
+    (--engine mixed|fp64): one solve, the session resumed every 60 s (each
+    resume prints a progress line), up to --solve-time-limit seconds.
+    Minutes long: not part of the default run."""
+    lib = sg.load_gpu()
+    model, _ = make_workload(args.quick)
+    rp, cj, v, vec = _arrays(model)
+    prec = sg.PREC_FP64 if args.engine == "fp64" else sg.PREC_MIXED
+    S = Session(lib, model, (rp, cj, v, vec), precision=prec, restart_rule=sg.RESTART_NDG,
+                use_graph=0 if args.no_graph else 1, autotune=0 if args.no_tune else 1)
+    chunk = min(60.0, args.solve_time_limit)
+    S.set_limits(100_000_000, chunk, 1e-6)
+    t0 = time.perf_counter()
+    rc, x, y, st = S.solve()
+    while rc == 0 and not st.converged and st.loop_seconds < args.solve_time_limit:
+        log(f"{args.engine} NDG: {st.iterations} iterations, {st.loop_seconds:.0f} s, relative KKT "
+            f"{st.relative_primal_residual:.2e} {st.relative_dual_residual:.2e} {st.relative_gap:.2e}")
+        x = np.zeros(model.ncol); y = np.zeros(model.nrow); st = sg.PdlpStats()
+        rc = lib.sk_gpu_pdlp_session_resume(S.h, sg.dptr(x), sg.dptr(y), C.byref(st))
+    wall = time.perf_counter() - t0
+    kkt = np.zeros(5)
+    lib.sk_gpu_relative_kkt(model.nrow, model.ncol, int(rp[-1]), sg.iptr(rp), sg.iptr(cj), sg.dptr(v),
+                            sg.dptr(vec["c"]), sg.dptr(vec["rlow"]), sg.dptr(vec["rupp"]), sg.dptr(vec["clow"]),
+                            sg.dptr(vec["cupp"]), sg.dptr(x), sg.dptr(y), sg.dptr(kkt))
+    d = st.as_dict()
+    S.close()
+    return _line(round(st.iterations / max(st.loop_seconds, 1e-9), 1), "PDHG iterations/s", args,
+                 f"LP {model.nrow} x {model.ncol} (power-law generator) to 1e-6, NDG restarts",
+                 {"engine": args.engine, "converged": bool(st.converged), "iterations": st.iterations,
+                  "time_to_kkt_1e-6_s": st.loop_seconds if st.converged else None, "wall_s": wall,
+                  "seconds_to_merit": d["seconds_to_merit"], "restarts": st.restarts,
+                  "original_kkt": kkt[:3].tolist()})
+
+
 WORKLOADS = {"lp1": workload_lp1, "lp0": workload_lp0, "qp2": workload_qp2, "batch3": workload_batch3,
-             "warm4": workload_warm4, "mlp4": workload_mlp4, "pipe1": workload_pipe1}
+             "warm4": workload_warm4, "mlp4": workload_mlp4, "pipe1": workload_pipe1,
+             "lp1ndg": workload_lp1ndg}
 
 if __name__ == "__main__":
     main()
