@@ -346,7 +346,11 @@ typedef struct sk_gpu_pdlp_params {  /* src/cuda/GpuPdlp.hpp:19-67 PdlpParams */
     int    restart_rule;     /* SK_GPU_RESTART_KKT (0, the reference's
                                 relative-KKT merit, GpuPdlp.cpp:437-451) or
                                 SK_GPU_RESTART_NDG (1, normalized duality
-                                gap evaluated in FP64)                      */
+                                gap evaluated in FP64; LPs only).  On LPs
+                                where KKT restarts stall short of the
+                                tolerance (the 1M x 2M power-law LP: a
+                                relative gap of 1e-3 after 900 s) NDG
+                                restarts can converge (1e-6 in 310 s)      */
     int    autotune;         /* 1: at session creation, time a short run of
                                 the adaptive loop for each candidate of the
                                 products' row-tile entry cap {256 .. 2048},
